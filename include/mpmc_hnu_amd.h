@@ -1,0 +1,138 @@
+/*
+ * mpmc_hnu_amd — C ABI of the MI355X (gfx950) PR76 per-cell compositional kernel.
+ *
+ * Plain pointers and sizes only. Every function returns an mpmc_rc code and never
+ * throws. Device buffers are caller owned; mixtures are library-owned device
+ * residents addressed by opaque handles. Calls are ordered on the HIP stream that
+ * is passed as `stream` (NULL = the null stream).
+ *
+ * Layouts (all device memory, structure of arrays, cell index fastest):
+ *   p[ncell], t[ncell]                  Pa, K
+ *   z[nc][ncell]                        feed mole fractions, z[i*ncell + c]
+ *   per-phase scalars  v[3][ncell]      v[ph*ncell + c]
+ *   per-phase vectors  x[3][nc][ncell]  x[(ph*nc + i)*ncell + c]
+ */
+#ifndef MPMC_HNU_AMD_H
+#define MPMC_HNU_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPMC_MAX_COMPONENTS 12
+#define MPMC_MAX_PHASES 3
+
+/* Return codes. */
+enum mpmc_rc {
+    MPMC_OK = 0,
+    MPMC_E_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument / length_error */
+    MPMC_E_DOMAIN = 2,           /* reference: std::domain_error / range_error (input data) */
+    MPMC_E_DEVICE = 3,           /* HIP runtime failure */
+    MPMC_E_NO_MEMORY = 4,
+    MPMC_E_NOT_BUILT = 5         /* no gfx950 code object for this device */
+};
+
+/* Per-cell outcome codes, mirroring the reference enums value by value. */
+/* PtPhaseSetStatus (modules/flash/include/mpmc/flash/pt_phase_set.hpp:36-40) */
+enum mpmc_phase_set_status { MPMC_SET_ACCEPTED = 0, MPMC_SET_PHASE_SET_UNSTABLE = 1,
+                             MPMC_SET_INDETERMINATE = 2 };
+/* Pr76PtMax3Status (modules/flash/include/mpmc/flash/pr76_three_phase.hpp:133-140) */
+enum mpmc_max3_status { MPMC_MAX3_SINGLE = 0, MPMC_MAX3_TWO = 1, MPMC_MAX3_THREE = 2,
+                        MPMC_MAX3_BOUNDARY_UNRESOLVED = 3, MPMC_MAX3_HIGHER = 4,
+                        MPMC_MAX3_INDETERMINATE = 5 };
+/* PtSplitStatus (modules/flash/include/mpmc/flash/pt_split.hpp:71-74) */
+enum mpmc_split_status { MPMC_SPLIT_SINGLE = 0, MPMC_SPLIT_TWO = 1, MPMC_SPLIT_UNSTABLE = 2,
+                         MPMC_SPLIT_INDETERMINATE = 3 };
+
+/* status word per cell: set | max3 << 4 | split << 8 | flags << 12 */
+#define MPMC_STATUS_SET(s) ((s) & 0xF)
+#define MPMC_STATUS_MAX3(s) (((s) >> 4) & 0xF)
+#define MPMC_STATUS_SPLIT(s) (((s) >> 8) & 0xF)
+/* flag: the cell hit a fixed-capacity limit of the device path (see DESIGN). */
+#define MPMC_FLAG_CAPACITY 0x1000
+
+/* branch word per phase: provider root index | smooth << 8 ; -1 when the slot is empty */
+#define MPMC_BRANCH_INDEX(b) ((b) & 0xFF)
+#define MPMC_BRANCH_SMOOTH(b) (((b) >> 8) & 0x1)
+
+/* counters per cell, counters[k*ncell + c]:
+ *   k=0 E_prop : EOS phase evaluations (one mixing + cubic + ln(phi) at one composition)
+ *   k=1 E_tpd  : tangent-plane-distance evaluations
+ *   k=2 I_rr   : Rachford-Rice bisection iterations
+ *   k=3 I_rr3  : three-phase generalized RR Newton iterations */
+#define MPMC_NUM_COUNTERS 4
+
+typedef struct mpmc_mixture* mpmc_mixture_t;
+
+/*
+ * Prepare one PR76 mixture and keep its constants resident on the current device.
+ * Replaces PrParameterSet::create + Pr76Phase::from_parameters
+ * (modules/thermodynamics/include/mpmc/thermodynamics/pr_parameters.hpp:53-154,
+ *  pr76_mixture.hpp:229-251, pr76_pure.hpp:181-199). kij is dense row-major nc*nc and
+ * must be symmetric with a zero diagonal.
+ */
+int mpmc_mixture_create(int nc, const double* tc_k, const double* pc_pa, const double* omega,
+                        const double* kij, mpmc_mixture_t* out);
+int mpmc_mixture_destroy(mpmc_mixture_t mixture);
+int mpmc_mixture_components(mpmc_mixture_t mixture);
+
+typedef struct mpmc_flash_options {
+    /* PtSplitIterationOptions::fugacity_tolerance (pt_split.hpp:31); default 1e-11 */
+    double fugacity_tolerance;
+    /* 2: solve_pr76_pt_phase_set (VLE, pr76_phase_set.hpp:42-52)
+     * 3: Pr76PtFlashBackend::solve (max3, pr76_pt_flash_backend.hpp:190-237) */
+    int32_t max_phases;
+    /* 0: every evaluation in FP64. 1: mixed precision (see DESIGN). */
+    int32_t precision;
+    int32_t reserved[4];
+} mpmc_flash_options;
+
+typedef struct mpmc_flash_outputs {
+    int32_t* status;      /* [ncell] status word */
+    int32_t* phase_count; /* [ncell] candidate phase-set size (0 = no candidate set) */
+    double* beta;         /* [3][ncell] mole phase fractions */
+    double* zfactor;      /* [3][ncell] compressibility factor, NaN when not published */
+    int32_t* branch;      /* [3][ncell] branch word */
+    double* x;            /* [3][nc][ncell] phase compositions */
+    double* ln_phi;       /* [3][nc][ncell] ln(fugacity coefficient) */
+    uint32_t* counters;   /* [4][ncell] or NULL */
+} mpmc_flash_outputs;
+
+void mpmc_flash_default_options(mpmc_flash_options* options);
+
+/*
+ * Batched PT flash, one result per cell, identical in meaning to calling
+ * PtFlashBackend::solve (modules/flash/include/mpmc/flash/pt_flash_backend.hpp:187-195)
+ * once per cell with {p, T, feed}. Numerical failure is a per-cell status, never an
+ * error code; only argument and runtime failures return non-zero.
+ */
+int mpmc_flash_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                     const double* z, const mpmc_flash_options* options,
+                     const mpmc_flash_outputs* out, void* stream);
+
+/*
+ * Candidate-phase PR76 evaluation, one composition per cell: all roots, then ln(phi) of
+ * every root (admissibility is in root_flags). Replaces Pr76Phase::roots_full + evaluate_full
+ * (modules/thermodynamics/include/mpmc/thermodynamics/pr76_phase.hpp:125-148).
+ *   root_status[ncell]            Pr76RootStatus (success=0, near_multiple=1,
+ *                                 iteration_limit=2, unrepresentable=3)
+ *   root_count[ncell]
+ *   zroot[3][ncell]               Z of each root (increasing), NaN past count
+ *   root_flags[3][ncell]          slope_sign+1 | derivative_valid << 4
+ *   ln_phi[3][nc][ncell]          ln(phi) of each root, NaN past count
+ */
+int mpmc_phase_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                     const double* x, int32_t* root_status, int32_t* root_count, double* zroot,
+                     int32_t* root_flags, double* ln_phi, void* stream);
+
+/* Last error text for this thread (never NULL). */
+const char* mpmc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPMC_HNU_AMD_H */

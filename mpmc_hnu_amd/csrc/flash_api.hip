@@ -1,0 +1,190 @@
+// C ABI of include/mpmc_hnu_amd.h: mixture residency, batched flash and batched
+// phase evaluation. Argument checks mirror the reference's input contracts; numerical
+// outcomes are per-cell statuses written by the kernels.
+#include "flash_kernels.hpp"
+
+#include "../../include/mpmc_hnu_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace mpmc::dev {
+MPMC_FOR_EACH_NC(MPMC_DECLARE_LAUNCH)
+} // namespace mpmc::dev
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& what) {
+    g_last_error = what;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    return fail(MPMC_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+bool is_device_ptr(const void* ptr) {
+    if (!ptr) { return false; }
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+} // namespace
+
+// Library-owned device residency of one prepared mixture.
+struct mpmc_mixture {
+    int nc;
+    int device;
+    double* dev_constants; // sqrt_tc, ac, b, kappa [nc each], factor [nc*nc], r_gas
+};
+
+extern "C" {
+
+const char* mpmc_last_error(void) { return g_last_error.c_str(); }
+
+void mpmc_flash_default_options(mpmc_flash_options* o) {
+    if (!o) { return; }
+    std::memset(o, 0, sizeof(*o));
+    o->fugacity_tolerance = 1e-11;
+    o->max_phases = 3;
+    o->precision = 0;
+}
+
+int mpmc_mixture_create(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
+                        mpmc_mixture_t* out) {
+    if (!out) { return fail(MPMC_E_INVALID_ARGUMENT, "mixture_create: null output handle"); }
+    *out = nullptr;
+    if (nc < 1 || nc > MPMC_MAX_COMPONENTS) { return fail(MPMC_E_INVALID_ARGUMENT, "mixture_create: nc outside [1, 12]"); }
+    if (!tc || !pc || !omega || !kij) { return fail(MPMC_E_INVALID_ARGUMENT, "mixture_create: null parameter array"); }
+    // PrParameterSet / Pr76Pure validation (pr_parameters.hpp:81-89, pr76_pure.hpp:168-199)
+    for (int i = 0; i < nc; ++i) {
+        if (!std::isfinite(tc[i]) || !(tc[i] > 0) || !std::isfinite(pc[i]) || !(pc[i] > 0) || !std::isfinite(omega[i])) {
+            return fail(MPMC_E_DOMAIN, "mixture_create: Tc, Pc must be finite and positive, omega finite");
+        }
+        for (int j = 0; j < nc; ++j) {
+            const double k = kij[i * nc + j];
+            if (!std::isfinite(k) || (i == j && k != 0.0) || k != kij[j * nc + i]) {
+                return fail(MPMC_E_DOMAIN, "mixture_create: kij must be finite, symmetric, zero diagonal");
+            }
+        }
+    }
+    // Pr76Pure constructor arithmetic (pr76_pure.hpp:187-194), Pr76Mixture factors (pr76_mixture.hpp:236-250)
+    const double r_gas = static_cast<double>(8.31446261815324L);
+    std::vector<double> h(4 * nc + nc * nc + 1);
+    double* sqrt_tc = h.data();
+    double* ac = sqrt_tc + nc;
+    double* b = ac + nc;
+    double* kappa = b + nc;
+    double* factor = kappa + nc;
+    for (int i = 0; i < nc; ++i) {
+        sqrt_tc[i] = std::sqrt(tc[i]);
+        kappa[i] = static_cast<double>(0.37464L) +
+                   omega[i] * (static_cast<double>(1.54226L) - static_cast<double>(0.26992L) * omega[i]);
+        const double rtc = r_gas * tc[i];
+        ac[i] = (static_cast<double>(0.45724L) * rtc) * (rtc / pc[i]);
+        b[i] = (static_cast<double>(0.07780L) * rtc) / pc[i];
+        if (!std::isfinite(kappa[i]) || !std::isfinite(ac[i]) || !(ac[i] > 0) || !std::isfinite(b[i]) || !(b[i] > 0)) {
+            return fail(MPMC_E_DOMAIN, "mixture_create: precomputed PR76 coefficient out of range");
+        }
+    }
+    for (int k = 0; k < nc * nc; ++k) { factor[k] = 1.0 - kij[k]; }
+    h[4 * nc + nc * nc] = r_gas;
+    auto* m = new (std::nothrow) mpmc_mixture{nc, 0, nullptr};
+    if (!m) { return fail(MPMC_E_NO_MEMORY, "mixture_create: host allocation"); }
+    hipError_t e = hipGetDevice(&m->device);
+    if (e == hipSuccess) { e = hipMalloc(&m->dev_constants, h.size() * sizeof(double)); }
+    if (e == hipSuccess) { e = hipMemcpy(m->dev_constants, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice); }
+    if (e != hipSuccess) {
+        if (m->dev_constants) { (void)hipFree(m->dev_constants); }
+        delete m;
+        return hip_fail(e, "mixture_create");
+    }
+    *out = m;
+    return MPMC_OK;
+}
+
+int mpmc_mixture_destroy(mpmc_mixture_t m) {
+    if (!m) { return MPMC_OK; }
+    const hipError_t e = m->dev_constants ? hipFree(m->dev_constants) : hipSuccess;
+    delete m;
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "mixture_destroy");
+}
+
+int mpmc_mixture_components(mpmc_mixture_t m) { return m ? m->nc : 0; }
+
+int mpmc_flash_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                     const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* stream) {
+    if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: null mixture or outputs"); }
+    if (ncell < 0 || ncell > (int64_t{1} << 40)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
+    mpmc_flash_options o;
+    mpmc_flash_default_options(&o);
+    if (options) { o = *options; }
+    if (o.max_phases != 2 && o.max_phases != 3) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: max_phases must be 2 or 3");
+    }
+    if (!std::isfinite(o.fugacity_tolerance) || !(o.fugacity_tolerance > 0)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: fugacity tolerance must be finite and positive");
+    }
+    if (o.precision != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: precision mode not available"); }
+    if (ncell == 0) { return MPMC_OK; }
+    const void* ptrs[] = {p, t, z, out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
+                          out->ln_phi};
+    for (const void* ptr : ptrs) {
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: buffers must be device memory"); }
+    }
+    if (out->counters && !is_device_ptr(out->counters)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: counters must be device memory");
+    }
+    const mpmc::dev::FlashOut fo{out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
+                                 out->ln_phi, out->counters};
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipErrorInvalidValue;
+    switch (m->nc) {
+#define MPMC_CASE(N)                                                                                              \
+    case N:                                                                                                       \
+        e = mpmc::dev::launch_flash<N>(m->dev_constants, ncell, p, t, z, o.max_phases, o.fugacity_tolerance, fo, s); \
+        break;
+        MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: unsupported nc");
+    }
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_batch launch");
+}
+
+int mpmc_phase_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* x,
+                     int32_t* root_status, int32_t* root_count, double* zroot, int32_t* root_flags, double* ln_phi,
+                     void* stream) {
+    if (!m) { return fail(MPMC_E_INVALID_ARGUMENT, "phase_batch: null mixture"); }
+    if (ncell < 0 || ncell > (int64_t{1} << 40)) { return fail(MPMC_E_INVALID_ARGUMENT, "phase_batch: bad ncell"); }
+    if (ncell == 0) { return MPMC_OK; }
+    const void* ptrs[] = {p, t, x, root_status, root_count, zroot, root_flags, ln_phi};
+    for (const void* ptr : ptrs) {
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "phase_batch: buffers must be device memory"); }
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipErrorInvalidValue;
+    switch (m->nc) {
+#define MPMC_CASE(N)                                                                                                   \
+    case N:                                                                                                            \
+        e = mpmc::dev::launch_phase<N>(m->dev_constants, ncell, p, t, x, root_status, root_count, zroot, root_flags, \
+                                       ln_phi, s);                                                                     \
+        break;
+        MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "phase_batch: unsupported nc");
+    }
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "phase_batch launch");
+}
+
+} // extern "C"

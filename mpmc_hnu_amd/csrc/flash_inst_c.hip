@@ -1,0 +1,8 @@
+// Explicit gfx950 instantiations of the flash kernels for nc in {7,8,9}.
+#include "flash_kernels.hpp"
+
+namespace mpmc::dev {
+MPMC_INSTANTIATE_LAUNCH(7)
+MPMC_INSTANTIATE_LAUNCH(8)
+MPMC_INSTANTIATE_LAUNCH(9)
+} // namespace mpmc::dev

@@ -1,0 +1,253 @@
+// gfx950 flash kernels: one cell per lane, 64-lane workgroups (one wavefront), the
+// mixture constants staged in LDS per workgroup. The component count is a
+// compile-time template argument so per-cell arrays stay in registers; each
+// supported nc is instantiated in its own translation unit (flash_inst_*.hip).
+#pragma once
+
+#include "flash_device.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mpmc::dev {
+
+struct FlashOut {
+    int32_t* status;
+    int32_t* phase_count;
+    double* beta;
+    double* zfactor;
+    int32_t* branch;
+    double* x;
+    double* ln_phi;
+    uint32_t* counters;
+};
+
+template <int NC>
+__device__ __forceinline__ void load_mix(const double* __restrict__ src, Mix<NC>& dst) {
+    double* d = reinterpret_cast<double*>(&dst);
+    constexpr int n = 4 * NC + NC * NC + 1;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) { d[k] = src[k]; }
+}
+
+template <int NC>
+struct PhaseView {
+    double beta;
+    const double* x;
+    const Activity<NC>* act;
+    double z;
+};
+
+template <int NC>
+__device__ void write_cell(const FlashOut& o, int64_t N, int64_t c, int set, int max3, int split, int np,
+                           const PhaseView<NC>* ph, const Counters& cnt, int flags) {
+    o.status[c] = set | (max3 << 4) | (split << 8) | flags;
+    o.phase_count[c] = np;
+    const double nan = __builtin_nan("");
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const bool have = k < np;
+        o.beta[k * N + c] = have ? ph[k].beta : nan;
+        o.zfactor[k * N + c] = have ? ph[k].z : nan;
+        o.branch[k * N + c] = have ? (ph[k].act->branch | (ph[k].act->smooth ? 0x100 : 0)) : -1;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            o.x[(k * NC + i) * N + c] = have ? ph[k].x[i] : nan;
+            o.ln_phi[(k * NC + i) * N + c] = have ? ph[k].act->lnphi[i] : nan;
+        }
+    }
+    if (o.counters) {
+        o.counters[0 * N + c] = cnt.prop;
+        o.counters[1 * N + c] = cnt.tpd;
+        o.counters[2 * N + c] = cnt.rr;
+        o.counters[3 * N + c] = cnt.rr3;
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ int pair_view(const SplitState<NC>& s, PhaseView<NC>* ph) {
+    ph[0] = {1.0 - s.fr.beta, s.fr.x, &s.liq, s.zl};
+    ph[1] = {s.fr.beta, s.fr.y, &s.vap, s.zv};
+    return 2;
+}
+
+// project_pt_vle_phase_set (pt_vle_phase_set.hpp:204-258)
+template <int NC>
+__device__ void publish_vle(const FlashOut& o, int64_t N, int64_t c, const VleOut<NC>& V, int max3_code,
+                            const Counters& cnt) {
+    PhaseView<NC> ph[3];
+    switch (V.status) {
+    case V_SINGLE:
+        if (!V.init.has_ref) { write_cell<NC>(o, N, c, 2, max3_code, V.status, 0, ph, cnt, 0); return; }
+        ph[0] = {1.0, V.init.feed, &V.init.ref, __builtin_nan("")};
+        write_cell<NC>(o, N, c, 0, max3_code, V.status, 1, ph, cnt, 0);
+        return;
+    case V_TWO:
+        write_cell<NC>(o, N, c, 0, max3_code, V.status, pair_view<NC>(V.cand, ph), ph, cnt, 0);
+        return;
+    case V_UNSTABLE:
+        write_cell<NC>(o, N, c, 1, max3_code, V.status, V.has_candidate ? pair_view<NC>(V.cand, ph) : 0, ph, cnt, 0);
+        return;
+    default:
+        write_cell<NC>(o, N, c, 2, max3_code, V.status, V.has_candidate ? pair_view<NC>(V.cand, ph) : 0, ph, cnt, 0);
+        return;
+    }
+}
+
+// detail::project_max3_solution (pt_phase_set.hpp:90-150)
+template <int NC>
+__device__ void publish_max3(const FlashOut& o, int64_t N, int64_t c, const Max3Out<NC>& M, const Counters& cnt) {
+    PhaseView<NC> ph[3];
+    const int split = M.base.status;
+    switch (M.status) {
+    case M_SINGLE:
+        publish_vle<NC>(o, N, c, M.base, M.status, cnt);
+        return;
+    case M_TWO:
+        write_cell<NC>(o, N, c, 0, M.status, split, pair_view<NC>(M.use_neighbor ? M.neighbor.cand : M.base.cand, ph),
+                       ph, cnt, 0);
+        return;
+    case M_THREE:
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { ph[k] = {M.three.frac[k], M.three.comp[k], &M.three.act[k], M.three.z[k]}; }
+        write_cell<NC>(o, N, c, 0, M.status, split, 3, ph, cnt, 0);
+        return;
+    case M_HIGHER:
+        write_cell<NC>(o, N, c, 1, M.status, split, M.base.has_candidate ? pair_view<NC>(M.base.cand, ph) : 0, ph, cnt,
+                       0);
+        return;
+    default:
+        write_cell<NC>(o, N, c, 2, M.status, split, 0, ph, cnt, 0);
+        return;
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void flash_vle_kernel(const double* __restrict__ mixc, int64_t ncell,
+                                                       const double* __restrict__ p, const double* __restrict__ t,
+                                                       const double* __restrict__ z, double fug_tol, FlashOut out) {
+    __shared__ Mix<NC> smix;
+    load_mix<NC>(mixc, smix);
+    __syncthreads();
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) { return; }
+    double feed[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
+    Counters cnt{0, 0, 0, 0};
+    bool rejected = false;
+    VleOut<NC> V;
+    solve_vle<NC>(smix, p[c], t[c], feed, fug_tol, nullptr, 0, V, cnt, rejected);
+    if (rejected) {
+        PhaseView<NC> ph[3];
+        write_cell<NC>(out, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+        return;
+    }
+    publish_vle<NC>(out, ncell, c, V, 0xF, cnt);
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void flash_max3_kernel(const double* __restrict__ mixc, int64_t ncell,
+                                                        const double* __restrict__ p, const double* __restrict__ t,
+                                                        const double* __restrict__ z, double fug_tol, FlashOut out) {
+    __shared__ Mix<NC> smix;
+    load_mix<NC>(mixc, smix);
+    __syncthreads();
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) { return; }
+    double feed[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
+    Counters cnt{0, 0, 0, 0};
+    bool rejected = false;
+    Max3Out<NC> M;
+    solve_max3<NC>(smix, p[c], t[c], feed, fug_tol, M, cnt, rejected);
+    if (rejected) {
+        PhaseView<NC> ph[3];
+        write_cell<NC>(out, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+        return;
+    }
+    publish_max3<NC>(out, ncell, c, M, cnt);
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void phase_kernel(const double* __restrict__ mixc, int64_t ncell,
+                                                   const double* __restrict__ p, const double* __restrict__ t,
+                                                   const double* __restrict__ x, int32_t* root_status,
+                                                   int32_t* root_count, double* zroot, int32_t* root_flags,
+                                                   double* ln_phi) {
+    __shared__ Mix<NC> smix;
+    load_mix<NC>(mixc, smix);
+    __syncthreads();
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) { return; }
+    double w[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { w[i] = x[i * ncell + c]; }
+    const double nan = __builtin_nan("");
+    root_count[c] = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        zroot[k * ncell + c] = nan;
+        root_flags[k * ncell + c] = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { ln_phi[(k * NC + i) * ncell + c] = nan; }
+    }
+    PhaseCtx<NC> ctx;
+    const int e = phase_prepare<NC>(smix, p[c], t[c], w, ctx);
+    root_status[c] = e == REJECTED ? -1 : ctx.rs.status;
+    if (e != ISSUE_NONE) { return; }
+    root_count[c] = ctx.rs.count;
+    for (int k = 0; k < ctx.rs.count; ++k) {
+        zroot[k * ncell + c] = ctx.rs.r[k].z;
+        root_flags[k * ncell + c] = (ctx.rs.r[k].slope_sign + 1) | (ctx.rs.r[k].dvalid ? 0x10 : 0);
+        double lp[NC];
+        if (phase_lnphi<NC>(smix, p[c], t[c], ctx, k, lp, nullptr) == ISSUE_NONE) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) { ln_phi[(k * NC + i) * ncell + c] = lp[i]; }
+        }
+    }
+}
+
+} // namespace mpmc::dev
+
+
+namespace mpmc::dev {
+
+constexpr int kBlock = 64;
+
+template <int NC>
+hipError_t launch_flash(const double* constants, int64_t ncell, const double* p, const double* t, const double* z,
+                        int max_phases, double fug_tol, const FlashOut& out, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>((ncell + kBlock - 1) / kBlock));
+    if (max_phases == 2) {
+        hipLaunchKernelGGL(flash_vle_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, z, fug_tol, out);
+    } else {
+        hipLaunchKernelGGL(flash_max3_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, z, fug_tol, out);
+    }
+    return hipGetLastError();
+}
+
+template <int NC>
+hipError_t launch_phase(const double* constants, int64_t ncell, const double* p, const double* t, const double* x,
+                        int32_t* rs, int32_t* rc, double* zr, int32_t* rf, double* lp, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>((ncell + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(phase_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, x, rs, rc, zr, rf, lp);
+    return hipGetLastError();
+}
+
+#define MPMC_FOR_EACH_NC(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+#define MPMC_DECLARE_LAUNCH(N)                                                                                   \
+    extern template hipError_t launch_flash<N>(const double*, int64_t, const double*, const double*,            \
+                                               const double*, int, double, const FlashOut&, hipStream_t);       \
+    extern template hipError_t launch_phase<N>(const double*, int64_t, const double*, const double*,            \
+                                               const double*, int32_t*, int32_t*, double*, int32_t*, double*,   \
+                                               hipStream_t);
+#define MPMC_INSTANTIATE_LAUNCH(N)                                                                               \
+    template hipError_t launch_flash<N>(const double*, int64_t, const double*, const double*, const double*,     \
+                                        int, double, const FlashOut&, hipStream_t);                              \
+    template hipError_t launch_phase<N>(const double*, int64_t, const double*, const double*, const double*,     \
+                                        int32_t*, int32_t*, double*, int32_t*, double*, hipStream_t);
+
+} // namespace mpmc::dev

@@ -1,0 +1,173 @@
+"""Batched PR76 PT flash on the GPU (device tensors in, device tensors out).
+
+Mirrors the reference's backend boundary: ``PtFlashBackend::solve`` of one
+``PtFlashRequest{pressure_pa, temperature_k, feed}``
+(modules/flash/include/mpmc/flash/pt_flash_backend.hpp:111-195), applied to every
+cell of a batch. Results come back in the library's structure-of-arrays layout.
+
+torch provides device memory and streams only; the arithmetic is the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+
+# status word decoding (include/mpmc_hnu_amd.h)
+SET_ACCEPTED, SET_PHASE_SET_UNSTABLE, SET_INDETERMINATE = 0, 1, 2
+MAX3_NAMES = ("single_phase", "two_phase", "three_phase", "phase_boundary_unresolved",
+              "higher_phase_count_or_wrong_candidate", "indeterminate")
+FLAG_REJECTED = 0x2000
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class Mixture:
+    """A prepared PR76 mixture resident on the current device (PrParameterSet +
+    Pr76Phase::from_parameters, pr_parameters.hpp:53-154)."""
+
+    def __init__(self, tc, pc, omega, kij):
+        lib = native.library()
+        tc = np.ascontiguousarray(tc, dtype=np.float64)
+        pc = np.ascontiguousarray(pc, dtype=np.float64)
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        kij = np.ascontiguousarray(kij, dtype=np.float64)
+        nc = tc.size
+        if pc.size != nc or om.size != nc or kij.size != nc * nc:
+            raise ValueError("Mixture: Tc, Pc, omega must have nc entries and kij nc*nc")
+        self.nc = nc
+        self.params = {"tc": tc, "pc": pc, "omega": om, "kij": kij.reshape(nc, nc)}
+        h = ctypes.c_void_p()
+        P = ctypes.c_void_p
+        native.check(lib.mpmc_mixture_create(nc, tc.ctypes.data_as(P), pc.ctypes.data_as(P), om.ctypes.data_as(P),
+                                             kij.ctypes.data_as(P), ctypes.byref(h)), "mixture_create")
+        self._h = h
+
+    @classmethod
+    def from_dict(cls, d):
+        return cls(d["tc"], d["pc"], d["omega"], d["kij"])
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise native.MpmcError("mixture was destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            native.library().mpmc_mixture_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dev(a, dtype, device):
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
+
+
+def allocate_outputs(nc: int, ncell: int, device, counters: bool = True) -> dict:
+    torch = _torch()
+    f64, i32 = torch.float64, torch.int32
+    out = {
+        "status": torch.empty(ncell, dtype=i32, device=device),
+        "phase_count": torch.empty(ncell, dtype=i32, device=device),
+        "beta": torch.empty((3, ncell), dtype=f64, device=device),
+        "zfactor": torch.empty((3, ncell), dtype=f64, device=device),
+        "branch": torch.empty((3, ncell), dtype=i32, device=device),
+        "x": torch.empty((3, nc, ncell), dtype=f64, device=device),
+        "ln_phi": torch.empty((3, nc, ncell), dtype=f64, device=device),
+    }
+    out["counters"] = torch.empty((4, ncell), dtype=torch.int32, device=device) if counters else None
+    return out
+
+
+def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, max_phases: int = 3,
+                precision: int = 0, stream=None, out: dict | None = None, counters: bool = True) -> dict:
+    """Flash every cell (p[ncell] Pa, t[ncell] K, z[nc, ncell]); returns device tensors.
+
+    Asynchronous on ``stream`` (default: torch's current stream)."""
+    torch = _torch()
+    lib = native.library()
+    device = torch.device("cuda", torch.cuda.current_device())
+    p = _dev(p, torch.float64, device)
+    t = _dev(t, torch.float64, device)
+    z = _dev(z, torch.float64, device)
+    if z.dim() != 2 or z.shape[0] != mixture.nc or z.shape[1] != p.numel() or t.numel() != p.numel():
+        raise ValueError("flash_batch: expected p[ncell], t[ncell], z[nc, ncell]")
+    n = p.numel()
+    if out is None:
+        out = allocate_outputs(mixture.nc, n, device, counters)
+    opts = native.FlashOptions()
+    lib.mpmc_flash_default_options(ctypes.byref(opts))
+    opts.fugacity_tolerance = float(fugacity_tolerance)
+    opts.max_phases = int(max_phases)
+    opts.precision = int(precision)
+    o = native.FlashOutputs(
+        out["status"].data_ptr(), out["phase_count"].data_ptr(), out["beta"].data_ptr(), out["zfactor"].data_ptr(),
+        out["branch"].data_ptr(), out["x"].data_ptr(), out["ln_phi"].data_ptr(),
+        out["counters"].data_ptr() if out.get("counters") is not None else None)
+    native.check(lib.mpmc_flash_batch(mixture.handle, n, ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                                      ctypes.c_void_p(z.data_ptr()), ctypes.byref(opts), ctypes.byref(o),
+                                      _stream_ptr(stream)), "flash_batch")
+    out["_inputs"] = (p, t, z)  # keep inputs alive until the stream consumes them
+    return out
+
+
+def phase_batch(mixture: Mixture, p, t, x, stream=None) -> dict:
+    """All roots and the ln(phi) of each admissible root at x[nc, ncell]."""
+    torch = _torch()
+    lib = native.library()
+    device = torch.device("cuda", torch.cuda.current_device())
+    p = _dev(p, torch.float64, device)
+    t = _dev(t, torch.float64, device)
+    x = _dev(x, torch.float64, device)
+    n = p.numel()
+    nc = mixture.nc
+    out = {
+        "root_status": torch.empty(n, dtype=torch.int32, device=device),
+        "root_count": torch.empty(n, dtype=torch.int32, device=device),
+        "zroot": torch.empty((3, n), dtype=torch.float64, device=device),
+        "root_flags": torch.empty((3, n), dtype=torch.int32, device=device),
+        "ln_phi": torch.empty((3, nc, n), dtype=torch.float64, device=device),
+    }
+    V = ctypes.c_void_p
+    native.check(lib.mpmc_phase_batch(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(x.data_ptr()),
+                                      V(out["root_status"].data_ptr()), V(out["root_count"].data_ptr()),
+                                      V(out["zroot"].data_ptr()), V(out["root_flags"].data_ptr()),
+                                      V(out["ln_phi"].data_ptr()), _stream_ptr(stream)), "phase_batch")
+    out["_inputs"] = (p, t, x)
+    return out
+
+
+def to_host(res: dict) -> dict:
+    """Synchronise and copy a result dict to numpy (counters as uint32)."""
+    torch = _torch()
+    torch.cuda.synchronize()
+    h = {}
+    for k, v in res.items():
+        if k.startswith("_") or v is None:
+            continue
+        a = v.cpu().numpy()
+        if k == "counters":
+            a = a.view(np.uint32)
+        h[k] = a
+    return h

@@ -1,0 +1,82 @@
+"""ctypes binding of the C ABI in include/mpmc_hnu_amd.h.
+
+The library is the in-tree gfx950 build ``mpmc_hnu_amd/libmpmc_hnu_amd.so``. There is
+no fallback: when it is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmpmc_hnu_amd.so"
+
+MAX_COMPONENTS = 12
+
+# include/mpmc_hnu_amd.h symbols, each checked by tests/test_abi.py
+EXPORTED = (
+    "mpmc_mixture_create",
+    "mpmc_mixture_destroy",
+    "mpmc_mixture_components",
+    "mpmc_flash_default_options",
+    "mpmc_flash_batch",
+    "mpmc_phase_batch",
+    "mpmc_last_error",
+)
+
+
+class FlashOptions(ctypes.Structure):
+    _fields_ = [
+        ("fugacity_tolerance", ctypes.c_double),
+        ("max_phases", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class FlashOutputs(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_void_p),
+        ("phase_count", ctypes.c_void_p),
+        ("beta", ctypes.c_void_p),
+        ("zfactor", ctypes.c_void_p),
+        ("branch", ctypes.c_void_p),
+        ("x", ctypes.c_void_p),
+        ("ln_phi", ctypes.c_void_p),
+        ("counters", ctypes.c_void_p),
+    ]
+
+
+class MpmcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load the gfx950 library; raises MpmcError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise MpmcError(f"{LIB_PATH} is not built: run `python -m mpmc_hnu_amd.build`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.c_void_p
+    lib.mpmc_mixture_create.argtypes = [ctypes.c_int, P, P, P, P, ctypes.POINTER(P)]
+    lib.mpmc_mixture_create.restype = ctypes.c_int
+    lib.mpmc_mixture_destroy.argtypes = [P]
+    lib.mpmc_mixture_components.argtypes = [P]
+    lib.mpmc_flash_default_options.argtypes = [ctypes.POINTER(FlashOptions)]
+    lib.mpmc_flash_default_options.restype = None
+    lib.mpmc_flash_batch.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOptions),
+                                     ctypes.POINTER(FlashOutputs), P]
+    lib.mpmc_phase_batch.argtypes = [P, ctypes.c_int64, P, P, P, P, P, P, P, P, P]
+    lib.mpmc_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = library().mpmc_last_error().decode(errors="replace")
+        raise MpmcError(f"{what} failed (rc={rc}): {msg}")

@@ -1,0 +1,149 @@
+"""Seeded synthetic workloads for the PR76 per-cell kernel.
+
+Every generator is deterministic (numpy PCG64 with a fixed seed) and returns host
+arrays in the library's structure-of-arrays layout: ``p[ncell]``, ``t[ncell]``,
+``z[nc, ncell]``. Mixtures are plain dicts ``{tc, pc, omega, kij}`` in SI units.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+MPA = 1.0e6
+
+
+def _dirichlet(rng: np.random.Generator, alpha: float, nc: int, ncell: int) -> np.ndarray:
+    """Feed fractions z ~ Dirichlet(alpha), normalised so |sum(z)-1| stays within a few ulps."""
+    if alpha == 1.0:
+        g = -np.log(rng.random((nc, ncell)))
+    else:
+        g = rng.gamma(alpha, 1.0, size=(nc, ncell))
+    g = np.maximum(g, np.finfo(np.float64).tiny)
+    z = g / g.sum(axis=0, keepdims=True)
+    # One correction pass on the largest entry keeps the rounded sum at 1 within 64 eps.
+    resid = 1.0 - z.sum(axis=0)
+    j = np.argmax(z, axis=0)
+    z[j, np.arange(ncell)] += resid
+    return np.ascontiguousarray(z)
+
+
+def mixture_c1() -> dict:
+    """nc=3 hydrocarbon-like mixture: Tc 190/425/620 K, Pc 4.6/3.8/2.1 MPa,
+    omega 0.011/0.20/0.49, kij = 0 except 0.02 between components 1 and 3."""
+    kij = np.zeros((3, 3))
+    kij[0, 2] = kij[2, 0] = 0.02
+    return {
+        "tc": np.array([190.0, 425.0, 620.0]),
+        "pc": np.array([4.6, 3.8, 2.1]) * MPA,
+        "omega": np.array([0.011, 0.20, 0.49]),
+        "kij": kij,
+    }
+
+
+def mixture_linspaced(nc: int, seed: int = 2) -> dict:
+    """Linspaced mixture: Tc ascending over 190-800 K, Pc descending over 4.6-1.5 MPa,
+    omega over 0.01-0.9; symmetric kij ~ U[0, 0.15] in the first row/column, else 0."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kij = np.zeros((nc, nc))
+    row = rng.uniform(0.0, 0.15, size=nc)
+    for j in range(1, nc):
+        kij[0, j] = kij[j, 0] = row[j]
+    if nc == 1:
+        tc, pc, om = np.array([190.0]), np.array([4.6 * MPA]), np.array([0.01])
+    else:
+        tc = np.linspace(190.0, 800.0, nc)
+        pc = np.linspace(4.6, 1.5, nc) * MPA
+        om = np.linspace(0.01, 0.9, nc)
+    return {"tc": tc, "pc": pc, "omega": om, "kij": kij}
+
+
+def cells_c1(ncell: int = 4096, seed: int = 1):
+    """T ~ U[300,450] K, P ~ U[2,30] MPa, z ~ Dirichlet(1,1,1)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = rng.uniform(300.0, 450.0, size=ncell)
+    p = rng.uniform(2.0, 30.0, size=ncell) * MPA
+    z = _dirichlet(rng, 1.0, 3, ncell)
+    return p, t, z
+
+
+def gaussian_field(shape, sigma: float, rng: np.random.Generator) -> np.ndarray:
+    """White noise convolved with a periodic Gaussian kernel (FFT), standard scale."""
+    noise = rng.standard_normal(shape)
+    freqs = [np.fft.fftfreq(n) for n in shape]
+    k2 = np.zeros(shape)
+    for axis, f in enumerate(freqs):
+        s = [1] * len(shape)
+        s[axis] = len(f)
+        k2 = k2 + (f.reshape(s) ** 2)
+    kernel = np.exp(-2.0 * (math.pi ** 2) * (sigma ** 2) * k2)
+    field = np.real(np.fft.ifftn(np.fft.fftn(noise) * kernel))
+    return field
+
+
+def _minmax(field: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    f0, f1 = field.min(), field.max()
+    return lo + (hi - lo) * (field - f0) / (f1 - f0)
+
+
+def cells_grid(nc: int, grid=(128, 128, 64), sigma: float = 8.0, seed: int = 3, alpha: float = 2.0):
+    """P, T as seeded smooth Gaussian random fields on a structured grid
+    (P 5-40 MPa, T 320-420 K) and z ~ Dirichlet(alpha) per cell. Cell order is
+    C-order over the grid (x slowest)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = _minmax(gaussian_field(grid, sigma, rng), 5.0 * MPA, 40.0 * MPA).ravel()
+    t = _minmax(gaussian_field(grid, sigma, rng), 320.0, 420.0).ravel()
+    ncell = p.size
+    z = _dirichlet(rng, alpha, nc, ncell)
+    return np.ascontiguousarray(p), np.ascontiguousarray(t), z
+
+
+def cells_c2(ncell: int | None = None, seed: int = 3):
+    """nc=6 grid workload on 128x128x64 = 1,048,576 cells; ``ncell`` takes a prefix."""
+    p, t, z = cells_grid(6, seed=seed)
+    if ncell is not None:
+        p, t, z = p[:ncell].copy(), t[:ncell].copy(), np.ascontiguousarray(z[:, :ncell])
+    return p, t, z
+
+
+def cells_uniform(nc: int, ncell: int, seed: int = 5, alpha: float = 2.0):
+    """P ~ U[5,40] MPa, T ~ U[320,420] K, z ~ Dirichlet(alpha): small parity batches."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = rng.uniform(5.0, 40.0, size=ncell) * MPA
+    t = rng.uniform(320.0, 420.0, size=ncell)
+    z = _dirichlet(rng, alpha, nc, ncell)
+    return p, t, z
+
+
+def mixture_sour_gas() -> dict:
+    """Six-component acid gas (CO2, N2, H2S, C1, C2, C3) of Li & Firoozabadi (2012),
+    Table 3, as held by the reference's three-phase fixture
+    (tests/flash/pr76_three_phase/sour_gas_fixture.hpp:29-82)."""
+    tc = np.array([304.211, 126.2, 373.2, 190.564, 305.322, 369.825])
+    pc = np.array([73.819e5, 33.9e5, 89.4e5, 45.992e5, 48.718e5, 42.462e5])
+    om = np.array([0.225, 0.039, 0.081, 0.01141, 0.10574, 0.15813])
+    pairs = {(0, 1): -0.02, (0, 2): 0.12, (1, 2): 0.20, (0, 3): 0.125, (1, 3): 0.031, (2, 3): 0.10,
+             (0, 4): 0.135, (1, 4): 0.042, (2, 4): 0.08, (0, 5): 0.150, (1, 5): 0.091, (2, 5): 0.08}
+    kij = np.zeros((6, 6))
+    for (i, j), v in pairs.items():
+        kij[i, j] = kij[j, i] = v
+    return {"tc": tc, "pc": pc, "omega": om, "kij": kij}
+
+
+SOUR_GAS_FEED = np.array([0.5, 0.1195792770100074885969092518210906, 0.03346041255361154605487099189869971,
+                          0.1167540336306079379127238069303560, 0.1797093062836135883994826060317244,
+                          0.05049697052215943903601334331812921])
+
+
+def sour_gas_sweep(ncell: int = 2048, seed: int = 11):
+    """Cells scattered around the acid-gas three-phase point (2 MPa, 178.8 K):
+    P ~ U[1,4] MPa, T ~ U[160,200] K, feed perturbed by 30% lognormal-ish noise."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = rng.uniform(1.0e6, 4.0e6, ncell)
+    t = rng.uniform(160.0, 200.0, ncell)
+    z = np.abs(SOUR_GAS_FEED[:, None] * (1.0 + 0.3 * rng.standard_normal((6, ncell))))
+    z = z / z.sum(axis=0, keepdims=True)
+    resid = 1.0 - z.sum(axis=0)
+    j = np.argmax(z, axis=0)
+    z[j, np.arange(ncell)] += resid
+    return mixture_sour_gas(), p, t, np.ascontiguousarray(z)

@@ -1,0 +1,1473 @@
+// cpuref — the oracle. Test infrastructure only: an FP64 CPU restatement of the
+// reference's PR76 PT flash (TPD multistart stability, bisection cubic roots,
+// Rachford-Rice, logK-SSI split, three-phase SSI with generalized RR, max3
+// orchestration and phase-set publication). It is loaded by tests/ and by
+// bench.py's CPU-baseline leg and never by the product library.
+//
+// Each function cites the reference file:line it restates (paths relative to the
+// reference checkout). The restatement keeps the reference's arithmetic order
+// (compensated sums, fma, midpoint, log1p/expm1 branches) so that phase counts and
+// labels can be compared exactly; it replaces the reference's exceptions by status
+// codes and its vectors by fixed-capacity arrays (nc <= 12).
+//
+// Pinning: tests/test_oracle_*.py check this file against the Decimal(80) goldens the
+// reference's own tests hold (tests/thermodynamics/pr76_pt/pt_test.cpp:90-150,
+// tests/flash/pt_split/split_test.cpp:316-341, ternary_references.hpp,
+// tests/flash/pr76_three_phase/sour_gas_references.hpp), against vectors written by
+// the reference itself (tests/golden/*.json, oracle/ref_driver.cpp), and against
+// closed forms (Cardano roots in long double, RR material balance).
+#include "cpuref.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <thread>
+#include <vector>
+
+namespace cpuref {
+namespace {
+
+constexpr int NCMAX = 12;
+constexpr int MAXTRIALS = 20;
+constexpr double EPS = DBL_EPSILON;
+constexpr double INF = std::numeric_limits<double>::infinity();
+const double NaN = std::numeric_limits<double>::quiet_NaN();
+
+// Input-contract violation that the reference raises as std::domain_error /
+// invalid_argument outside the numerical-status channel. The whole cell is rejected.
+struct Rejected {};
+
+// ---------------------------------------------------------------- mixture constants
+// pr76_pure.hpp:181-199 (kappa Eq.18 Horner order, ac and b split products),
+// pr76_mixture.hpp:229-251 (factor 1-kij), pr76_pure.hpp:115-117 (R).
+struct Mixture {
+    int n;
+    double sqrt_tc[NCMAX], ac[NCMAX], b[NCMAX], kappa[NCMAX];
+    double factor[NCMAX * NCMAX];
+};
+const double R_GAS = static_cast<double>(8.31446261815324L);
+
+Mixture make_mixture(int n, const double* tc, const double* pc, const double* omega, const double* kij) {
+    Mixture m{};
+    m.n = n;
+    for (int i = 0; i < n; ++i) {
+        m.sqrt_tc[i] = std::sqrt(tc[i]);
+        m.kappa[i] = static_cast<double>(0.37464L) +
+                     omega[i] * (static_cast<double>(1.54226L) - static_cast<double>(0.26992L) * omega[i]);
+        const double rtc = R_GAS * tc[i];
+        m.ac[i] = (static_cast<double>(0.45724L) * rtc) * (rtc / pc[i]);
+        m.b[i] = (static_cast<double>(0.07780L) * rtc) / pc[i];
+    }
+    for (int k = 0; k < n * n; ++k) { m.factor[k] = 1.0 - kij[k]; }
+    return m;
+}
+
+// Kahan step, pt_stability.hpp:254-259.
+inline void kadd(double term, double& sum, double& c) {
+    const double inc = term - c;
+    const double next = sum + inc;
+    c = (next - sum) - inc;
+    sum = next;
+}
+inline double ksum(const double* v, int n) {
+    double s = 0, c = 0;
+    for (int i = 0; i < n; ++i) { kadd(v[i], s, c); }
+    return s;
+}
+inline double midpoint(double a, double b) { return (a + b) / 2; } // |a|,|b| <= max/2 here
+
+// ---------------------------------------------------------------- EOS phase
+enum Issue { NONE = -1, ROOT_TOPOLOGY = 0, ROOT_ITERATION_LIMIT, ROOT_RANGE, ILL_CONDITIONED,
+             NO_ADMISSIBLE, NONFINITE };
+enum RootStatus { RS_SUCCESS = 0, RS_NEAR_MULTIPLE, RS_ITERATION_LIMIT, RS_UNREPRESENTABLE };
+
+struct Root {
+    double z, x, resid, presid, slope_ratio;
+    bool dvalid;
+    int slope_sign, iters;
+};
+struct RootSet {
+    Root r[3];
+    int count;
+    double scale;
+    int status;
+};
+
+struct Cubic { double c2, c1, c0; };
+// pr76_roots.hpp:50-58
+Cubic cubic(double a, double b, double s) {
+    const double bs = b / s;
+    const double inv = 1.0 / s;
+    return {4.0 * bs - inv, (a / s) / s + 2.0 * bs * bs - 4.0 * bs * inv, ((-2.0 * bs) * bs) * inv};
+}
+inline double poly(double y, const Cubic& c) { return std::fma(std::fma(y + c.c2, y, c.c1), y, c.c0); }
+inline double poly_scale(double y, const Cubic& c) {
+    return ((std::fabs(y) + std::fabs(c.c2)) * std::fabs(y) + std::fabs(c.c1)) * std::fabs(y) + std::fabs(c.c0);
+}
+
+// pr76_roots.hpp:77-173 (bracketed bisection in y = (Z-B)/s).
+RootSet roots(double a, double b) {
+    RootSet rs{};
+    rs.status = RS_UNREPRESENTABLE;
+    rs.scale = std::max({1.0, b, std::sqrt(std::fabs(a))});
+    const Cubic c = cubic(a, b, rs.scale);
+    if (!std::isfinite(c.c2) || !std::isfinite(c.c1) || !std::isfinite(c.c0) || !(c.c0 < 0)) { return rs; }
+    const double upper = 1.0 + std::max({std::fabs(c.c2), std::fabs(c.c1), std::fabs(c.c0)});
+    double ends[4];
+    int ne = 1;
+    ends[0] = 0.0;
+    const double disc = std::fma(c.c2, c.c2, -3.0 * c.c1);
+    if (disc > 0) {
+        const double q = -c.c2 - std::copysign(std::sqrt(disc), c.c2);
+        double st[2] = {q / 3.0, c.c1 / q};
+        if (st[1] < st[0]) { std::swap(st[0], st[1]); }
+        for (double pt : st) {
+            if (pt > 0 && pt < upper) {
+                const double v = poly(pt, c);
+                const double mag = poly_scale(pt, c);
+                if (!std::isfinite(v) || !(mag > 0)) { return rs; }
+                if (std::fabs(v) <= 64.0 * EPS * mag) { rs.status = RS_NEAR_MULTIPLE; return rs; }
+                ends[ne++] = pt;
+            }
+        }
+    }
+    ends[ne++] = upper;
+    for (int iv = 1; iv < ne; ++iv) {
+        double left = ends[iv - 1], right = ends[iv];
+        double fl = poly(left, c);
+        const double fr = poly(right, c);
+        if (!std::isfinite(fl) || !std::isfinite(fr)) { return rs; }
+        if (std::signbit(fl) == std::signbit(fr)) { continue; }
+        double y = 0;
+        bool conv = false;
+        int it = 0;
+        for (; it < 2048; ++it) {
+            y = midpoint(left, right);
+            const double fy = poly(y, c);
+            if (fy == 0 || y == left || y == right ||
+                right - left <= 4.0 * EPS * std::max(y, DBL_MIN)) { conv = true; break; }
+            if (std::signbit(fy) == std::signbit(fl)) { left = y; fl = fy; } else { right = y; }
+        }
+        if (!conv) { rs.count = 0; rs.status = RS_ITERATION_LIMIT; return rs; }
+        const double x = y * rs.scale;
+        const double z = x + b;
+        const double mag = poly_scale(y, c);
+        const double resid = std::fabs(poly(y, c)) / mag;
+        const double dh = std::fma(3.0 * y + 2.0 * c.c2, y, c.c1);
+        const double dhs = 3.0 * y * y + 2.0 * std::fabs(c.c2) * y + std::fabs(c.c1);
+        const double sr = dhs > 0 ? std::fabs(dh) / dhs : 0.0;
+        const double bz = b / z;
+        const double rep = 1.0 / x;
+        const double att = ((a / z) / z) / (1.0 + 2.0 * bz - bz * bz);
+        const double pres = std::fabs(rep - att - 1.0) / (1.0 + std::fabs(rep) + std::fabs(att));
+        if (!std::isfinite(z) || !(z > b) || !std::isfinite(resid) || !std::isfinite(pres) ||
+            resid > 128.0 * EPS || pres > 512.0 * EPS || rs.count == 3) {
+            rs.count = 0; rs.status = RS_UNREPRESENTABLE; return rs;
+        }
+        if (rs.count != 0 && !(z > rs.r[rs.count - 1].z)) {
+            rs.count = 0; rs.status = RS_NEAR_MULTIPLE; return rs;
+        }
+        rs.r[rs.count++] = {z, x, resid, pres, sr, sr > 8.0 * std::sqrt(EPS),
+                            dh > 0 ? 1 : (dh < 0 ? -1 : 0), it + 1};
+    }
+    rs.status = rs.count > 0 ? RS_SUCCESS : RS_UNREPRESENTABLE;
+    return rs;
+}
+
+// Mixing, coefficients and roots at one composition: pr76_pure.hpp:144-165,
+// pr76_mixture.hpp:124-219 (full coordinates, row sums), pr76_phase.hpp:184-200.
+struct PhaseCtx {
+    double amix, bmix, A, B;
+    double S[NCMAX];
+    RootSet rs;
+};
+
+void check_fractions(const double* w, int n) {
+    long double sum = 0, comp = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(w[i]) || w[i] < 0 || w[i] > 1) { throw Rejected{}; }
+        const long double inc = w[i] - comp;
+        const long double next = sum + inc;
+        comp = (next - sum) - inc;
+        sum = next;
+    }
+    if (std::fabs(sum - 1.0L) > static_cast<long double>(64.0 * EPS)) { throw Rejected{}; }
+}
+
+// returns NONE, ROOT_RANGE (range_error) or the mapped root status
+int prepare(const Mixture& m, double p, double t, const double* w, PhaseCtx& ctx) {
+    const int n = m.n;
+    ctx.rs.status = RS_UNREPRESENTABLE;
+    ctx.rs.count = 0;
+    if (!std::isfinite(p) || !(p > 0)) { throw Rejected{}; }
+    if (!std::isfinite(t) || !(t > 0)) { throw Rejected{}; }
+    check_fractions(w, n);
+    double ai[NCMAX], root[NCMAX];
+    double att = 0, cov = 0;
+    for (int i = 0; i < n; ++i) {
+        const double rr = std::sqrt(t) / m.sqrt_tc[i];
+        const double br = 1.0 + m.kappa[i] * (1.0 - rr);
+        if (!std::isfinite(rr) || !std::isfinite(br)) { return ROOT_RANGE; }
+        const double a = (br * m.ac[i]) * br;
+        if (!std::isfinite(a) || (a == 0 && br != 0)) { return ROOT_RANGE; }
+        ai[i] = a;
+        att += (w[i] * a) * w[i];
+        cov += w[i] * m.b[i];
+        ctx.S[i] = w[i] * a;
+        if (n > 1) {
+            if (!(a > 0)) { throw Rejected{}; }
+            root[i] = std::sqrt(a);
+            if (!std::isfinite(root[i])) { return ROOT_RANGE; }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        for (int j = i + 1; j < n; ++j) {
+            const double f = m.factor[i * n + j];
+            const double cross = ((w[i] * root[i]) * (w[j] * root[j])) * f;
+            att += cross;
+            att += cross;
+            const double aij = (root[i] * f) * root[j];
+            ctx.S[i] += w[j] * aij;
+            ctx.S[j] += w[i] * aij;
+        }
+    }
+    if (!std::isfinite(att) || !std::isfinite(cov) || !(cov > 0)) { return ROOT_RANGE; }
+    ctx.amix = att;
+    ctx.bmix = cov;
+    const double rt = R_GAS * t;
+    if (!std::isfinite(rt) || !(rt > 0)) { return ROOT_RANGE; }
+    const double prt = p / rt;
+    ctx.A = (att / rt) * prt;
+    ctx.B = cov * prt;
+    if (!std::isfinite(ctx.A) || !std::isfinite(ctx.B) || !(ctx.B > 0) || (ctx.A == 0 && att != 0)) {
+        return ROOT_RANGE;
+    }
+    ctx.rs = roots(ctx.A, ctx.B);
+    switch (ctx.rs.status) {
+    case RS_NEAR_MULTIPLE: return ROOT_TOPOLOGY;
+    case RS_ITERATION_LIMIT: return ROOT_ITERATION_LIMIT;
+    case RS_UNREPRESENTABLE: return ROOT_RANGE;
+    default: return NONE;
+    }
+}
+
+// log1p(u)/u, pr76_phase.hpp:67-79 (series of degree digits/3+2 = 19 for |u| <= 1/8).
+double log1p_over_x(double u) {
+    if (std::fabs(u) <= 0.125) {
+        constexpr int degree = 53 / 3 + 2;
+        double v = (degree % 2 == 0 ? 1.0 : -1.0) / static_cast<double>(degree + 1);
+        for (int k = degree - 1; k >= 0; --k) { v = (k % 2 == 0 ? 1.0 : -1.0) / static_cast<double>(k + 1) + u * v; }
+        return v;
+    }
+    return std::log1p(u) / u;
+}
+
+// ln(phi) of root k, pr76_phase.hpp:193-258 (double arithmetic, no IFT).
+int lnphi(const Mixture& m, double p, double t, const PhaseCtx& ctx, int k, double* out, double* zout) {
+    for (int i = 0; i < m.n; ++i) { if (!std::isfinite(ctx.S[i])) { return ROOT_RANGE; } }
+    const Root& r = ctx.rs.r[k];
+    const double fv = r.x;
+    const double z = fv + ctx.B;
+    const double s2 = std::sqrt(2.0);
+    const double den = fv + (2.0 - s2) * ctx.B;
+    const double u = (2.0 * s2) * (ctx.B / den);
+    const double lf = log1p_over_x(u) / den;
+    double zm1 = z - 1.0;
+    double lfree = std::log(fv);
+    if (std::fabs(zm1) < 0.25 && ctx.B < 0.125) {
+        const double bz = ctx.B / z;
+        zm1 = ctx.B / fv - (ctx.A / z) / (1.0 + 2.0 * bz - bz * bz);
+        lfree = std::log1p(zm1 - ctx.B);
+    }
+    const double rt = R_GAS * t;
+    const double prt = p / rt;
+    for (int i = 0; i < m.n; ++i) {
+        const double ratio = m.b[i] / ctx.bmix;
+        const double coef = ((2.0 * ctx.S[i] - ctx.amix * ratio) / rt) * prt;
+        out[i] = ratio * zm1 - lfree - coef * lf;
+        if (!std::isfinite(out[i])) { return ROOT_RANGE; }
+    }
+    if (!std::isfinite(z)) { return ROOT_RANGE; }
+    if (zout) { *zout = z; }
+    return NONE;
+}
+
+struct Counters { uint64_t prop = 0, tpd = 0, rr = 0, rr3 = 0; };
+
+// ---------------------------------------------------------------- providers
+struct SPhase {
+    double lnphi[NCMAX];
+    int branch;
+    bool smooth;
+};
+struct RPhase {
+    SPhase act;
+    double z;
+};
+
+// Pr76StabilityEvaluator::operator(), pr76_stability.hpp:36-119.
+int stab_provider(const Mixture& m, double p, double t, const double* w, SPhase& out) {
+    const int n = m.n;
+    PhaseCtx ctx;
+    const int pi = prepare(m, p, t, w, ctx);
+    if (pi != NONE) { return pi; }
+    double cand[3][NCMAX];
+    int cb[3];
+    int nc = 0;
+    for (int k = 0; k < ctx.rs.count; ++k) {
+        if (ctx.rs.r[k].slope_sign <= 0) { continue; }
+        const int e = lnphi(m, p, t, ctx, k, cand[nc], nullptr);
+        if (e != NONE) { return e; }
+        cb[nc++] = k;
+    }
+    if (nc == 0) { return NO_ADMISSIBLE; }
+    auto diff = [&](int a, int b, double& v) {
+        double s = 0, c = 0;
+        for (int i = 0; i < n; ++i) { kadd(w[i] * (cand[a][i] - cand[b][i]), s, c); }
+        v = s;
+        return std::isfinite(s);
+    };
+    int best = 0;
+    for (int k = 1; k < nc; ++k) {
+        double d;
+        if (!diff(k, best, d)) { return NONFINITE; }
+        if (d < 0) { best = k; }
+    }
+    if (!ctx.rs.r[cb[best]].dvalid) { return ILL_CONDITIONED; }
+    bool smooth = true;
+    for (int k = 0; k < nc; ++k) {
+        if (k == best) { continue; }
+        double mag = 1;
+        for (int i = 0; i < n; ++i) { mag += w[i] * (std::fabs(cand[k][i]) + std::fabs(cand[best][i])); }
+        if (!std::isfinite(mag)) { return NONFINITE; }
+        double d;
+        if (!diff(k, best, d)) { return NONFINITE; }
+        if (std::fabs(d) <= 256.0 * EPS * mag) { smooth = false; }
+    }
+    for (int i = 0; i < n; ++i) { out.lnphi[i] = cand[best][i]; }
+    out.branch = cb[best];
+    out.smooth = smooth;
+    return NONE;
+}
+
+// Candidate branch on one side of the admissible roots: Pr76VleEvaluator role call
+// (pr76_split.hpp:188-234; liquid = lowest, vapor = highest admissible root) and
+// Pr76ThreePhaseEvaluator (pr76_three_phase.hpp:39-102; lower/upper admissible).
+int side_provider(const Mixture& m, double p, double t, const double* w, int upper, RPhase& out) {
+    PhaseCtx ctx;
+    const int pi = prepare(m, p, t, w, ctx);
+    if (pi != NONE) { return pi; }
+    int sel = -1;
+    for (int k = 0; k < ctx.rs.count; ++k) {
+        if (ctx.rs.r[k].slope_sign <= 0) { continue; }
+        sel = k;
+        if (!upper) { break; }
+    }
+    if (sel < 0) { return NO_ADMISSIBLE; }
+    if (!ctx.rs.r[sel].dvalid) { return ILL_CONDITIONED; }
+    const int e = lnphi(m, p, t, ctx, sel, out.act.lnphi, &out.z);
+    if (e != NONE) { return e; }
+    out.act.branch = sel;
+    out.act.smooth = true;
+    return NONE;
+}
+
+// ---------------------------------------------------------------- TPD / stability
+struct StabOptions {
+    double tpd_tol = 1e-10, stat_tol = 1e-8, armijo = 1e-4, log_step_limit = 4.0;
+    int max_iterations = 512, max_backtracks = 32;
+    uint64_t max_evaluations = 100000;
+};
+enum TrialStatus { T_NEGATIVE = 0, T_STATIONARY, T_ITERATION_LIMIT, T_EVALUATION_LIMIT,
+                   T_LINE_SEARCH_FAILED, T_PROPERTY_FAILURE, T_NONSMOOTH, T_UNREPRESENTABLE };
+enum StabStatus { S_UNSTABLE = 0, S_NO_INSTABILITY, S_INDETERMINATE };
+
+struct TpdPoint {
+    double w[NCMAX], res[NCMAX];
+    double value, guard, stat;
+    int branch;
+    bool smooth;
+};
+
+double check_composition(const double* w, int n) { // pt_stability.hpp:265-277
+    for (int i = 0; i < n; ++i) { if (!std::isfinite(w[i]) || w[i] < 0 || w[i] > 1) { throw Rejected{}; } }
+    const double s = ksum(w, n);
+    if (std::fabs(s - 1.0) > 64.0 * EPS) { throw Rejected{}; }
+    return s;
+}
+
+// tangent_plane_distance, pt_stability.hpp:333-375.
+int tpd(int n, const double* w, const double* z, const SPhase& trial, const SPhase& ref, TpdPoint& r) {
+    const double sum = check_composition(w, n);
+    (void)check_composition(z, n);
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(trial.lnphi[i]) || !std::isfinite(ref.lnphi[i])) { return NONFINITE; }
+    }
+    for (int i = 0; i < n; ++i) { r.w[i] = w[i]; r.res[i] = 0; }
+    r.value = 0; r.stat = 0; r.branch = trial.branch; r.smooth = trial.smooth;
+    double corr = 0, mag = 1, mc = 0;
+    bool boundary = false;
+    for (int i = 0; i < n; ++i) {
+        if (z[i] == 0) { if (w[i] != 0) { throw Rejected{}; } continue; }
+        if (w[i] == 0) { boundary = true; continue; }
+        const double lw = std::log(w[i]), lz = std::log(z[i]);
+        const double lr = std::fabs(w[i] - z[i]) < 0.5 * z[i] ? std::log1p((w[i] - z[i]) / z[i]) : lw - lz;
+        const double q = lr + (trial.lnphi[i] - ref.lnphi[i]);
+        r.res[i] = q;
+        kadd(w[i] * q, r.value, corr);
+        kadd(w[i] * (std::fabs(lw) + std::fabs(lz) + std::fabs(trial.lnphi[i]) + std::fabs(ref.lnphi[i])), mag, mc);
+    }
+    const double mean = r.value / sum;
+    for (int i = 0; i < n; ++i) {
+        if (z[i] == 0) { continue; }
+        r.res[i] = w[i] == 0 ? -INF : r.res[i] - mean;
+        r.stat = std::max(r.stat, std::fabs(r.res[i]));
+    }
+    r.guard = 256.0 * EPS * mag;
+    if (!std::isfinite(r.value) || !std::isfinite(r.guard) || (!boundary && !std::isfinite(r.stat))) { return NONFINITE; }
+    return NONE;
+}
+
+struct Trial {
+    double init[NCMAX];
+    bool has_point;
+    TpdPoint pt;
+    int status, iterations;
+};
+struct StabResult {
+    int status;
+    StabOptions opt;
+    double feed[NCMAX];
+    bool has_ref;
+    SPhase ref;
+    int ntrials;
+    Trial trials[MAXTRIALS];
+    uint64_t evaluations;
+};
+
+inline bool negative(const TpdPoint& p, const StabOptions& o) { return p.value < -o.tpd_tol - p.guard; }
+// pt_stability.hpp:312-326
+inline bool accept_tpd_step(const TpdPoint& pt, const TpdPoint& nx, double pred, double armijo, double rel) {
+    const double g = pt.guard + nx.guard;
+    if (pred <= g) { return nx.value <= pt.value + g && rel > 0 && nx.stat < pt.stat; }
+    return nx.value <= pt.value - armijo * pred;
+}
+
+// detail::test_pt_stability_impl, pt_stability.hpp:383-616. imposed == nullptr is the
+// feed-phase test; otherwise the common-tangent review of pt_stability.hpp:637-644.
+void stability(const Mixture& m, double p, double t, const double* feed_in, const double* imposed,
+               const double* const* extra, int nextra, const StabOptions& o, StabResult& R, Counters& cnt) {
+    const int n = m.n;
+    if (!std::isfinite(p) || p <= 0 || !std::isfinite(t) || t <= 0) { throw Rejected{}; }
+    const double isum = check_composition(feed_in, n);
+    if (imposed) { for (int i = 0; i < n; ++i) { if (!std::isfinite(imposed[i])) { throw Rejected{}; } } }
+    int active = 0;
+    for (int i = 0; i < n; ++i) { if (feed_in[i] > 0) { ++active; } }
+    for (int e = 0; e < nextra; ++e) {
+        (void)check_composition(extra[e], n);
+        for (int i = 0; i < n; ++i) {
+            if ((feed_in[i] == 0 && extra[e][i] != 0) || (feed_in[i] > 0 && extra[e][i] == 0)) { throw Rejected{}; }
+        }
+    }
+    R.status = S_INDETERMINATE;
+    R.opt = o;
+    R.has_ref = false;
+    R.ntrials = 0;
+    R.evaluations = 0;
+    for (int i = 0; i < n; ++i) { R.feed[i] = feed_in[i] / isum; }
+    for (int i = 0; i < n; ++i) { if (feed_in[i] > 0 && R.feed[i] == 0) { return; } }
+    auto add_start = [&](const double* v) {
+        Trial& tr = R.trials[R.ntrials++];
+        const double s = ksum(v, n);
+        for (int i = 0; i < n; ++i) { tr.init[i] = v[i] / s; }
+        tr.has_point = false; tr.status = T_EVALUATION_LIMIT; tr.iterations = 0;
+    };
+    add_start(R.feed);
+    if (active > 1) {
+        double u[NCMAX];
+        for (int i = 0; i < n; ++i) { u[i] = R.feed[i] > 0 ? 1.0 / static_cast<double>(active) : 0.0; }
+        add_start(u);
+        for (int i = 0; i < n; ++i) {
+            if (R.feed[i] == 0) { continue; }
+            double s[NCMAX];
+            for (int k = 0; k < n; ++k) { s[k] = R.feed[k] * 0.1; }
+            s[i] += 0.9;
+            add_start(s);
+        }
+    }
+    for (int e = 0; e < nextra; ++e) { add_start(extra[e]); }
+    SPhase imp;
+    if (!imposed) {
+        ++R.evaluations; ++cnt.prop;
+        const int e = stab_provider(m, p, t, R.feed, R.ref);
+        if (e != NONE) {
+            for (int k = 0; k < R.ntrials; ++k) { R.trials[k].status = T_PROPERTY_FAILURE; }
+            return;
+        }
+        R.has_ref = true;
+    } else {
+        for (int i = 0; i < n; ++i) { imp.lnphi[i] = R.feed[i] > 0 ? imposed[i] - std::log(R.feed[i]) : 0.0; }
+        imp.branch = 0; imp.smooth = true;
+        for (int i = 0; i < n; ++i) {
+            if (!std::isfinite(imp.lnphi[i])) {
+                for (int k = 0; k < R.ntrials; ++k) { R.trials[k].status = T_PROPERTY_FAILURE; }
+                return;
+            }
+        }
+    }
+    const SPhase& dref = imposed ? imp : R.ref;
+    bool neg_found = false, all_stat = true;
+    for (int k = 0; k < R.ntrials; ++k) {
+        Trial& tr = R.trials[k];
+        auto evaluate = [&](const double* w, TpdPoint& out) -> int {
+            ++R.evaluations; ++cnt.prop; ++cnt.tpd;
+            SPhase ph;
+            const int e = stab_provider(m, p, t, w, ph);
+            if (e != NONE) { return e; }
+            return tpd(n, w, R.feed, ph, dref, out);
+        };
+        bool lost = false;
+        for (int i = 0; i < n; ++i) { lost = lost || (R.feed[i] > 0 && tr.init[i] == 0); }
+        if (lost) { tr.status = T_UNREPRESENTABLE; all_stat = false; continue; }
+        if (R.evaluations >= o.max_evaluations) { tr.status = T_EVALUATION_LIMIT; all_stat = false; continue; }
+        if (evaluate(tr.init, tr.pt) != NONE) { tr.status = T_PROPERTY_FAILURE; all_stat = false; continue; }
+        tr.has_point = true;
+        for (;;) {
+            if (negative(tr.pt, o)) { tr.status = T_NEGATIVE; neg_found = true; break; }
+            if (!tr.pt.smooth) { tr.status = T_NONSMOOTH; break; }
+            if (tr.pt.stat <= o.stat_tol) { tr.status = T_STATIONARY; break; }
+            if (tr.iterations >= o.max_iterations) { tr.status = T_ITERATION_LIMIT; break; }
+            const TpdPoint pt = tr.pt;
+            const double scale = std::max(1.0, pt.stat / o.log_step_limit);
+            double dir[NCMAX], cand[NCMAX];
+            double slope = 0, sc = 0;
+            for (int i = 0; i < n; ++i) {
+                dir[i] = 0;
+                if (R.feed[i] == 0) { continue; }
+                dir[i] = -pt.res[i] / scale;
+                kadd((pt.w[i] * pt.res[i]) * dir[i], slope, sc);
+            }
+            if (!std::isfinite(slope) || !(slope < 0)) { tr.status = T_LINE_SEARCH_FAILED; break; }
+            bool accepted = false;
+            double alpha = 1;
+            tr.status = T_LINE_SEARCH_FAILED;
+            for (int bt = 0; bt < o.max_backtracks; ++bt, alpha *= 0.5) {
+                double largest = -INF;
+                for (int i = 0; i < n; ++i) {
+                    if (R.feed[i] > 0) {
+                        cand[i] = std::log(pt.w[i]) + alpha * dir[i];
+                        largest = std::max(largest, cand[i]);
+                    }
+                }
+                for (int i = 0; i < n; ++i) { cand[i] = R.feed[i] > 0 ? std::exp(cand[i] - largest) : 0.0; }
+                const double s = ksum(cand, n);
+                bool positive = std::isfinite(s) && s > 0;
+                for (int i = 0; i < n; ++i) {
+                    cand[i] /= s;
+                    positive = positive && (R.feed[i] == 0 || cand[i] > 0);
+                }
+                if (!positive) { continue; }
+                if (R.evaluations >= o.max_evaluations) { tr.status = T_EVALUATION_LIMIT; break; }
+                TpdPoint nx;
+                if (evaluate(cand, nx) != NONE) { continue; }
+                if (negative(nx, o) || accept_tpd_step(pt, nx, -alpha * slope, o.armijo, alpha / scale)) {
+                    tr.pt = nx; accepted = true; break;
+                }
+            }
+            if (!accepted) { break; }
+            ++tr.iterations;
+        }
+        all_stat = all_stat && tr.status == T_STATIONARY;
+    }
+    R.status = neg_found ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
+}
+
+// ---------------------------------------------------------------- Rachford-Rice
+enum RRStatus { RR_INTERIOR = 0, RR_NO_INTERIOR, RR_DEGENERATE, RR_ITERATION_LIMIT, RR_UNREPRESENTABLE };
+struct RRResult {
+    int status;
+    double beta;
+    double x[NCMAX], y[NCMAX];
+    double mass_abs, mass_rel;
+    int iterations;
+};
+
+double rr_log_sum(int n, const double* z, const double* lk, double sign) { // rachford_rice.hpp:42-52
+    double largest = -INF;
+    for (int i = 0; i < n; ++i) { if (z[i] > 0) { largest = std::max(largest, std::log(z[i]) + sign * lk[i]); } }
+    double s = 0, c = 0;
+    for (int i = 0; i < n; ++i) { if (z[i] > 0) { kadd(std::exp(std::log(z[i]) + sign * lk[i] - largest), s, c); } }
+    return largest + std::log(s);
+}
+double rr_value(double beta, int n, const double* z, const double* lk) { // rachford_rice.hpp:53-63
+    double v = 0, c = 0;
+    for (int i = 0; i < n; ++i) {
+        if (z[i] == 0) { continue; }
+        const double e = std::exp(-std::fabs(lk[i]));
+        const double den = lk[i] >= 0 ? beta + (1 - beta) * e : (1 - beta) + beta * e;
+        const double num = lk[i] >= 0 ? -std::expm1(-lk[i]) : std::expm1(lk[i]);
+        kadd((z[i] / den) * num, v, c);
+    }
+    return v;
+}
+
+// solve_rachford_rice, rachford_rice.hpp:70-147.
+void rachford_rice(int n, const double* z, const double* lk, RRResult& r, Counters& cnt) {
+    (void)check_composition(z, n);
+    double contrast = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(lk[i])) { throw Rejected{}; }
+        if (z[i] > 0) { contrast = std::max(contrast, std::fabs(lk[i])); }
+    }
+    r.status = RR_UNREPRESENTABLE; r.beta = NaN; r.mass_abs = 0; r.mass_rel = 0; r.iterations = 0;
+    if (contrast == 0) { r.status = RR_DEGENERATE; return; }
+    const double ls = rr_log_sum(n, z, lk, 1), rs = rr_log_sum(n, z, lk, -1);
+    if (!std::isfinite(ls) || !std::isfinite(rs)) { return; }
+    if (ls <= 64 * EPS || rs <= 64 * EPS) { r.status = RR_NO_INTERIOR; return; }
+    double left = 0, right = 1, beta = 0.5, value = 0;
+    bool resolved = false;
+    for (int it = 0; it < 192; ++it) {
+        r.iterations = it + 1;
+        ++cnt.rr;
+        beta = midpoint(left, right);
+        if (!(beta > 0 && beta < 1)) { break; }
+        value = rr_value(beta, n, z, lk);
+        if (!std::isfinite(value)) { return; }
+        if (value == 0 || beta == left || beta == right || right - left <= 8 * EPS * std::min(beta, 1 - beta)) {
+            resolved = true; break;
+        }
+        if (value > 0) { left = beta; } else { right = beta; }
+    }
+    if (!resolved) { r.status = RR_ITERATION_LIMIT; return; }
+    r.beta = beta;
+    if (std::fabs(value) > 64 * EPS) { return; }
+    for (int i = 0; i < n; ++i) {
+        r.x[i] = 0; r.y[i] = 0;
+        if (z[i] == 0) { continue; }
+        const double e = std::exp(-std::fabs(lk[i]));
+        const double den = lk[i] >= 0 ? beta + (1 - beta) * e : (1 - beta) + beta * e;
+        const double larger = z[i] / den;
+        r.x[i] = lk[i] >= 0 ? larger * e : larger;
+        r.y[i] = lk[i] >= 0 ? larger : larger * e;
+        if (!std::isfinite(larger) || !(r.x[i] > 0 && r.y[i] > 0)) { return; }
+    }
+    const double sl = ksum(r.x, n), sv = ksum(r.y, n);
+    if (std::fabs(sl - 1) > 64 * EPS || std::fabs(sv - 1) > 64 * EPS) { return; }
+    for (int i = 0; i < n; ++i) {
+        r.x[i] /= sl;
+        r.y[i] /= sv;
+        const double rec = std::fma(beta, r.y[i], (1 - beta) * r.x[i]);
+        const double err = std::fabs(rec - z[i]);
+        r.mass_abs = std::max(r.mass_abs, err);
+        if (z[i] > 0) {
+            if (!(r.x[i] > 0 && r.y[i] > 0)) { return; }
+            r.mass_rel = std::max(r.mass_rel, err / z[i]);
+        } else if (rec != 0) { return; }
+    }
+    (void)check_composition(r.x, n);
+    (void)check_composition(r.y, n);
+    r.status = RR_INTERIOR;
+}
+
+// ---------------------------------------------------------------- two-phase split
+struct SplitOptions {
+    double fug_tol = 1e-11, mass_abs = 1e-12, mass_rel = 1e-10, min_pf = 1e-10, lk_sep = 1e-7,
+           z_sep = 1e-8, max_log_step = 2, res_dec = 1e-4, armijo = 1e-4;
+    int max_iterations = 512, max_backtracks = 24;
+    uint64_t max_evaluations = 20000;
+};
+enum AttemptStatus { A_CONVERGED = 0, A_NO_INTERIOR, A_RR_FAILURE, A_INDISTINGUISHABLE, A_DISAPPEARANCE,
+                     A_PROPERTY_FAILURE, A_ITERATION_LIMIT, A_EVALUATION_LIMIT, A_LINE_SEARCH_FAILED,
+                     A_UNREPRESENTABLE_SEED, A_BALANCE_FAILURE };
+struct SplitState {
+    RRResult fr;
+    RPhase liq, vap;
+    double lk[NCMAX], r[NCMAX], common[NCMAX];
+    double norm, gibbs, guard;
+};
+struct SplitAttempt {
+    int status;
+    bool has_point;
+    SplitState pt;
+    uint64_t evaluations;
+};
+
+// iterate_pt_split, pt_split.hpp:274-408. Returns the attempt.
+void iterate_split(const Mixture& m, double p, double t, const double* z, const double* lk0,
+                   const SplitOptions& o, SplitAttempt& A, Counters& cnt) {
+    const int n = m.n;
+    (void)check_composition(z, n);
+    A.status = A_UNREPRESENTABLE_SEED; A.has_point = false; A.evaluations = 0;
+    // returns -1 on success, else an attempt status (rr failure / evaluation limit) or
+    // 100 for a property failure
+    auto evaluate = [&](const double* lk, SplitState& s) -> int {
+        rachford_rice(n, z, lk, s.fr, cnt);
+        if (s.fr.status != RR_INTERIOR) {
+            return s.fr.status == RR_DEGENERATE ? A_INDISTINGUISHABLE
+                 : (s.fr.status == RR_NO_INTERIOR ? A_NO_INTERIOR : A_RR_FAILURE);
+        }
+        for (int role = 0; role < 2; ++role) {
+            if (A.evaluations >= o.max_evaluations) { return A_EVALUATION_LIMIT; }
+            ++A.evaluations; ++cnt.prop;
+            RPhase& ph = role == 0 ? s.liq : s.vap;
+            if (side_provider(m, p, t, role == 0 ? s.fr.x : s.fr.y, role, ph) != NONE) { return 100; }
+            if (!std::isfinite(ph.z) || ph.z <= 0 || !ph.act.smooth) { return 100; }
+        }
+        double corr = 0, mag = 1, mc = 0;
+        const double beta = s.fr.beta;
+        s.norm = 0; s.gibbs = 0;
+        for (int i = 0; i < n; ++i) {
+            s.lk[i] = 0; s.r[i] = 0; s.common[i] = 0;
+            if (z[i] == 0) { continue; }
+            const double lx = std::log(s.fr.x[i]), ly = std::log(s.fr.y[i]);
+            const double mul = lx + s.liq.act.lnphi[i], muv = ly + s.vap.act.lnphi[i];
+            s.lk[i] = ly - lx;
+            s.r[i] = (lx - ly) + (s.liq.act.lnphi[i] - s.vap.act.lnphi[i]);
+            s.common[i] = midpoint(mul, muv);
+            s.norm = std::max(s.norm, std::fabs(s.r[i]));
+            kadd((1 - beta) * s.fr.x[i] * mul + beta * s.fr.y[i] * muv, s.gibbs, corr);
+            kadd((1 - beta) * s.fr.x[i] * (std::fabs(lx) + std::fabs(s.liq.act.lnphi[i])) +
+                 beta * s.fr.y[i] * (std::fabs(ly) + std::fabs(s.vap.act.lnphi[i])), mag, mc);
+            if (!std::isfinite(s.r[i]) || !std::isfinite(s.common[i])) { return 100; }
+        }
+        s.guard = 256 * EPS * mag;
+        if (!std::isfinite(s.gibbs) || !std::isfinite(s.guard)) { return 100; }
+        return -1;
+    };
+    {
+        const int e = evaluate(lk0, A.pt);
+        if (e != -1) { A.status = e == 100 ? A_PROPERTY_FAILURE : e; return; }
+        A.has_point = true;
+    }
+    int iterations = 0;
+    for (;;) {
+        const SplitState& pt = A.pt;
+        if (pt.norm <= o.fug_tol) {
+            const double beta = pt.fr.beta;
+            double contrast = 0;
+            for (int i = 0; i < n; ++i) { contrast = std::max(contrast, std::fabs(pt.lk[i])); }
+            if (!(pt.fr.mass_abs <= o.mass_abs && pt.fr.mass_rel <= o.mass_rel)) { A.status = A_BALANCE_FAILURE; }
+            else if (beta <= o.min_pf || 1 - beta <= o.min_pf) { A.status = A_DISAPPEARANCE; }
+            else if (contrast <= o.lk_sep ||
+                     pt.vap.z - pt.liq.z <= o.z_sep * std::max(pt.vap.z, pt.liq.z)) { A.status = A_INDISTINGUISHABLE; }
+            else { A.status = A_CONVERGED; }
+            return;
+        }
+        if (iterations >= o.max_iterations) { A.status = A_ITERATION_LIMIT; return; }
+        const double scale = std::max(1.0, pt.norm / o.max_log_step);
+        // split_gibbs_slope, pt_split.hpp:213-231
+        double slope;
+        {
+            double h = 0, hc = 0, f = 0, fc = 0, s2 = 0, s2c = 0;
+            for (int i = 0; i < n; ++i) {
+                if (z[i] == 0) { continue; }
+                const double x = pt.fr.x[i], y = pt.fr.y[i];
+                const double c = (x / z[i]) * y;
+                const double d = y - x;
+                const double r = pt.r[i];
+                kadd(d * (d / z[i]), h, hc);
+                kadd(c * r, f, fc);
+                kadd((c * r) * r, s2, s2c);
+            }
+            if (!(h > 0) || !std::isfinite(h) || !std::isfinite(f) || !std::isfinite(s2) || !std::isfinite(scale) ||
+                !(scale >= 1)) {
+                slope = NaN;
+            } else {
+                const double beta = pt.fr.beta;
+                slope = -(beta * (1 - beta) * s2 + (f / h) * f) / scale;
+            }
+        }
+        if (!std::isfinite(slope) || !(slope < 0)) { A.status = A_LINE_SEARCH_FAILED; return; }
+        bool accepted = false;
+        double alpha = 1;
+        A.status = A_LINE_SEARCH_FAILED;
+        for (int bt = 0; bt < o.max_backtracks; ++bt, alpha *= 0.5) {
+            double nlk[NCMAX];
+            for (int i = 0; i < n; ++i) { nlk[i] = pt.lk[i] + (alpha / scale) * pt.r[i]; }
+            SplitState nx;
+            const int e = evaluate(nlk, nx);
+            if (e == -1) {
+                const double pred = -alpha * slope;
+                // split_accept_step, pt_split.hpp:232-244
+                const double g = pt.guard + nx.guard;
+                bool ok;
+                if (pred > g) { ok = nx.gibbs <= pt.gibbs - o.armijo * pred; }
+                else {
+                    ok = nx.gibbs <= pt.gibbs + g &&
+                         (nx.norm <= o.fug_tol || pt.norm - nx.norm > o.res_dec * (alpha / scale) * pt.norm);
+                }
+                if (ok) { A.pt = nx; accepted = true; break; }
+            } else if (e == A_EVALUATION_LIMIT) {
+                A.status = A_EVALUATION_LIMIT; return;
+            }
+        }
+        if (!accepted) { return; }
+        ++iterations;
+    }
+}
+
+// split_seed, pt_split.hpp:245-267
+bool split_seed(int n, const double* z, const double* w, bool as_vapor, double* lk) {
+    double amount = 0.1;
+    for (int i = 0; i < n; ++i) { if (z[i] > 0) { amount = std::min(amount, 0.5 * (z[i] / w[i])); } }
+    if (!(amount > 0 && amount < 1)) { return false; }
+    double other[NCMAX];
+    for (int i = 0; i < n; ++i) {
+        other[i] = 0; lk[i] = 0;
+        if (z[i] == 0) { continue; }
+        other[i] = std::fma(-amount, w[i], z[i]) / (1 - amount);
+        if (!(other[i] > 0)) { return false; }
+    }
+    const double s = ksum(other, n);
+    if (!std::isfinite(s) || std::fabs(s - 1) > 64 * EPS) { return false; }
+    for (int i = 0; i < n; ++i) {
+        if (z[i] == 0) { continue; }
+        other[i] /= s;
+        lk[i] = (as_vapor ? 1.0 : -1.0) * (std::log(w[i]) - std::log(other[i]));
+        if (!std::isfinite(lk[i])) { return false; }
+    }
+    return true;
+}
+
+enum VleStatus { V_SINGLE = 0, V_TWO, V_UNSTABLE, V_INDETERMINATE };
+struct VleResult {
+    int status;
+    StabResult init;
+    bool has_candidate;
+    SplitAttempt cand;
+    bool has_final;
+    StabResult fin;
+    bool any_disappearance;
+};
+
+// solve_pt_vle, pt_split.hpp:414-525 (with Pr76VleEvaluator as both providers).
+void solve_vle(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
+               const double* const* starts, int nstarts, VleResult& V, Counters& cnt) {
+    const int n = m.n;
+    if (!std::isfinite(p) || p <= 0 || !std::isfinite(t) || t <= 0) { throw Rejected{}; }
+    (void)check_composition(feed, n);
+    for (int e = 0; e < nstarts; ++e) {
+        (void)check_composition(starts[e], n);
+        for (int i = 0; i < n; ++i) { if ((feed[i] > 0) != (starts[e][i] > 0)) { throw Rejected{}; } }
+    }
+    V.status = V_INDETERMINATE; V.has_candidate = false; V.has_final = false; V.any_disappearance = false;
+    StabOptions so_init;
+    stability(m, p, t, feed, nullptr, starts, nstarts, so_init, V.init, cnt);
+    if (V.init.status == S_NO_INSTABILITY) { V.status = V_SINGLE; return; }
+    if (V.init.status != S_UNSTABLE) { return; }
+    const double* z = V.init.feed;
+    int attempts = 0;
+    uint64_t split_evals = 0;
+    bool limit = false;
+    SplitAttempt A;
+    for (int k = 0; k < V.init.ntrials && !limit; ++k) {
+        const Trial& w = V.init.trials[k];
+        if (w.status != T_NEGATIVE || !w.has_point || !negative(w.pt, V.init.opt)) { continue; }
+        for (int side = 0; side < 2; ++side) {
+            const bool as_vapor = side == 0;
+            if (attempts >= 16 || split_evals >= so.max_evaluations) { limit = true; break; }
+            double lk[NCMAX];
+            if (split_seed(n, z, w.pt.w, as_vapor, lk)) {
+                SplitOptions it = so;
+                it.max_evaluations -= split_evals;
+                iterate_split(m, p, t, z, lk, it, A, cnt);
+            } else {
+                A.status = A_UNREPRESENTABLE_SEED; A.has_point = false; A.evaluations = 0;
+            }
+            split_evals += A.evaluations;
+            ++attempts;
+            if (A.status == A_DISAPPEARANCE) { V.any_disappearance = true; }
+            if (A.status == A_CONVERGED && (!V.has_candidate || A.pt.gibbs < V.cand.pt.gibbs)) {
+                V.cand = A; V.has_candidate = true;
+            }
+        }
+    }
+    if (!V.has_candidate) { return; }
+    const SplitState& c = V.cand.pt;
+    double feed_g = 0, corr = 0, mag = 1;
+    for (int i = 0; i < n; ++i) {
+        if (z[i] == 0) { continue; }
+        const double lz = std::log(z[i]);
+        const double ph = V.init.ref.lnphi[i];
+        kadd(z[i] * (lz + ph), feed_g, corr);
+        mag += z[i] * (std::fabs(lz) + std::fabs(ph));
+    }
+    const double dg = c.gibbs - feed_g;
+    const double guard = c.guard + 256 * EPS * mag;
+    StabOptions fo;
+    fo.tpd_tol += 0.5 * c.norm;
+    if (!std::isfinite(dg) || !std::isfinite(guard) || !std::isfinite(fo.tpd_tol)) { return; }
+    const double* fs[6];
+    int nf = 0;
+    for (int e = 0; e < nstarts; ++e) { fs[nf++] = starts[e]; }
+    fs[nf++] = c.fr.x;
+    fs[nf++] = c.fr.y;
+    stability(m, p, t, z, c.common, fs, nf, fo, V.fin, cnt);
+    V.has_final = true;
+    if (V.fin.status == S_UNSTABLE) { V.status = V_UNSTABLE; }
+    else if (dg > guard) { V.status = V_INDETERMINATE; }
+    else if (V.fin.status == S_NO_INSTABILITY) { V.status = V_TWO; }
+    else { V.status = V_INDETERMINATE; }
+}
+
+// ---------------------------------------------------------------- three phases
+struct P3Options {
+    double mu_tol = 1e-11, mass_abs = 1e-12, mass_rel = 1e-10, bal_tol = 2e-13, min_pf = 1e-10,
+           lsep = 1e-7, max_log_step = 2.0, res_dec = 1e-4;
+    int max_iterations = 512, max_backtracks = 32, max_bal_it = 192, max_bal_bt = 48;
+    uint64_t max_evaluations = 30000;
+};
+enum P3Status { P_CONVERGED = 0, P_NO_BALANCE, P_DEGENERATE, P_BAL_ITER, P_BAL_LS, P_BAL_FAILURE,
+                P_DISAPPEARANCE, P_INDISTINGUISHABLE, P_PROPERTY, P_ITERATION_LIMIT, P_EVALUATION_LIMIT,
+                P_LINE_SEARCH_FAILED };
+struct P3State {
+    double frac[3];
+    double comp[3][NCMAX];
+    SPhase act[3];
+    double z[3];
+    double lk1[NCMAX], lk2[NCMAX], r1[NCMAX], r2[NCMAX], common[NCMAX];
+    double norm, gibbs, guard, rr_res, mass_abs, mass_rel;
+};
+struct P3Result {
+    int status;
+    bool has_point;
+    P3State pt;
+    int disappearing;
+};
+
+struct RR3Eval {
+    double f[2], j[3];
+    double comp[3][NCMAX];
+};
+inline bool rr3_feasible(const double* b) {
+    return std::isfinite(b[0]) && std::isfinite(b[1]) && b[0] >= 0 && b[1] >= 0 && b[0] + b[1] <= 1;
+}
+// rr3_evaluate, pt_three_phase.hpp:187-236
+bool rr3_eval(int n, const double* z, const double* l1, const double* l2, const double* beta, RR3Eval& v) {
+    if (!rr3_feasible(beta)) { return false; }
+    const double b0 = 1.0 - beta[0] - beta[1];
+    double f0 = 0, f0c = 0, f1 = 0, f1c = 0, j00 = 0, j00c = 0, j01 = 0, j01c = 0, j11 = 0, j11c = 0;
+    for (int i = 0; i < n; ++i) {
+        v.comp[0][i] = v.comp[1][i] = v.comp[2][i] = 0;
+        if (z[i] == 0) { continue; }
+        const double sh = std::max({0.0, l1[i], l2[i]});
+        const double e0 = std::exp(-sh), e1 = std::exp(l1[i] - sh), e2 = std::exp(l2[i] - sh);
+        const double den = b0 * e0 + beta[0] * e1 + beta[1] * e2;
+        if (!std::isfinite(den) || !(den > 0)) { return false; }
+        const double x0 = z[i] * e0 / den, x1 = z[i] * e1 / den, x2 = z[i] * e2 / den;
+        if (!std::isfinite(x0) || !std::isfinite(x1) || !std::isfinite(x2) || !(x0 > 0) || !(x1 > 0) || !(x2 > 0)) {
+            return false;
+        }
+        v.comp[0][i] = x0; v.comp[1][i] = x1; v.comp[2][i] = x2;
+        const double d1 = x1 - x0, d2 = x2 - x0;
+        kadd(d1, f0, f0c);
+        kadd(d2, f1, f1c);
+        kadd(-(d1 * d1) / z[i], j00, j00c);
+        kadd(-(d1 * d2) / z[i], j01, j01c);
+        kadd(-(d2 * d2) / z[i], j11, j11c);
+    }
+    v.f[0] = f0; v.f[1] = f1;
+    v.j[0] = j00; v.j[1] = j01; v.j[2] = j11;
+    return true;
+}
+
+enum RR3Status { G_SUCCESS = 0, G_DEGENERATE, G_ITERATION_LIMIT, G_LINE_SEARCH, G_UNREPRESENTABLE };
+struct RR3Result {
+    int status;
+    double beta[2];
+    double comp[3][NCMAX];
+    double residual, mass_abs, mass_rel;
+};
+// solve_generalized_rr3, pt_three_phase.hpp:238-355
+void rr3_solve(int n, const double* z, const double* l1, const double* l2, const double* b_init,
+               const P3Options& o, RR3Result& r, Counters& cnt) {
+    r.status = G_UNREPRESENTABLE; r.residual = INF; r.mass_abs = 0; r.mass_rel = 0;
+    if (!rr3_feasible(b_init)) { return; }
+    for (int i = 0; i < n; ++i) { if (!std::isfinite(l1[i]) || !std::isfinite(l2[i])) { return; } }
+    double beta[2] = {b_init[0], b_init[1]};
+    RR3Eval ev;
+    for (int it = 0;; ++it) {
+        if (!rr3_eval(n, z, l1, l2, beta, ev)) { return; }
+        const double norm = std::max(std::fabs(ev.f[0]), std::fabs(ev.f[1]));
+        if (!std::isfinite(norm)) { return; }
+        if (norm <= o.bal_tol) { r.status = G_SUCCESS; r.residual = norm; break; }
+        if (it >= o.max_bal_it) { r.status = G_ITERATION_LIMIT; return; }
+        ++cnt.rr3;
+        const double j00 = ev.j[0], j01 = ev.j[1], j11 = ev.j[2];
+        const double det = j00 * j11 - j01 * j01;
+        const double ds = std::fabs(j00 * j11) + std::fabs(j01 * j01) + 1.0;
+        if (!std::isfinite(det) || std::fabs(det) <= 256.0 * EPS * ds) { r.status = G_DEGENERATE; return; }
+        const double f0 = ev.f[0], f1 = ev.f[1];
+        const double step[2] = {(-f0 * j11 + j01 * f1) / det, (j01 * f0 - j00 * f1) / det};
+        if (!std::isfinite(step[0]) || !std::isfinite(step[1])) { return; }
+        bool acc = false;
+        double alpha = 1.0;
+        for (int bt = 0; bt < o.max_bal_bt; ++bt, alpha *= 0.5) {
+            const double cand[2] = {beta[0] + alpha * step[0], beta[1] + alpha * step[1]};
+            if (!rr3_feasible(cand)) { continue; }
+            RR3Eval nx;
+            if (!rr3_eval(n, z, l1, l2, cand, nx)) { continue; }
+            const double nn = std::max(std::fabs(nx.f[0]), std::fabs(nx.f[1]));
+            if (!std::isfinite(nn)) { continue; }
+            if (nn < norm || nn <= o.bal_tol) { beta[0] = cand[0]; beta[1] = cand[1]; acc = true; break; }
+        }
+        if (!acc) { r.status = G_LINE_SEARCH; return; }
+    }
+    if (!rr3_eval(n, z, l1, l2, beta, ev)) { r.status = G_UNREPRESENTABLE; return; }
+    r.beta[0] = beta[0]; r.beta[1] = beta[1];
+    double raw[3];
+    for (int ph = 0; ph < 3; ++ph) {
+        raw[ph] = ksum(ev.comp[ph], n);
+        if (!std::isfinite(raw[ph]) || !(raw[ph] > 0)) { r.status = G_UNREPRESENTABLE; return; }
+        for (int i = 0; i < n; ++i) { r.comp[ph][i] = ev.comp[ph][i] / raw[ph]; }
+    }
+    const double fr[3] = {1.0 - beta[0] - beta[1], beta[0], beta[1]};
+    double ae = 0, re = 0;
+    for (int i = 0; i < n; ++i) {
+        const double rec = fr[0] * r.comp[0][i] + fr[1] * r.comp[1][i] + fr[2] * r.comp[2][i];
+        const double err = std::fabs(rec - z[i]);
+        ae = std::max(ae, err);
+        if (z[i] > 0) { re = std::max(re, err / z[i]); } else if (rec != 0) { re = INF; }
+    }
+    r.mass_abs = ae; r.mass_rel = re;
+}
+
+double log_distance(int n, const double* a, const double* b, const double* z) { // pt_three_phase.hpp:357-380
+    double d = 0;
+    for (int i = 0; i < n; ++i) {
+        if (z[i] == 0) { if (a[i] != 0 || b[i] != 0) { return INF; } continue; }
+        if (!(a[i] > 0) || !(b[i] > 0)) { return INF; }
+        const double v = std::fabs(std::log(a[i]) - std::log(b[i]));
+        if (!std::isfinite(v)) { return INF; }
+        d = std::max(d, v);
+    }
+    return d;
+}
+
+// iterate_pt_three_phase, pt_three_phase.hpp:402-674, with Pr76ThreePhaseEvaluator sides.
+void iterate_three(const Mixture& m, double p, double t, const double* feed_in, const double* lk1_0,
+                   const double* lk2_0, const double* b0, const int* sides, const P3Options& o, P3Result& R,
+                   Counters& cnt) {
+    const int n = m.n;
+    if (!rr3_feasible(b0)) { throw Rejected{}; }
+    for (int i = 0; i < n; ++i) { if (!std::isfinite(lk1_0[i]) || !std::isfinite(lk2_0[i])) { throw Rejected{}; } }
+    R.has_point = false; R.disappearing = -1; R.status = P_NO_BALANCE;
+    const double isum = check_composition(feed_in, n);
+    double feed[NCMAX];
+    for (int i = 0; i < n; ++i) { feed[i] = feed_in[i] / isum; }
+    for (int i = 0; i < n; ++i) { if (feed_in[i] > 0 && feed[i] == 0) { R.status = P_BAL_FAILURE; return; } }
+    uint64_t evaluations = 0;
+    // returns -1 on success, else a P3Status
+    auto evaluate = [&](const double* l1, const double* l2, const double* seed, P3State& s) -> int {
+        RR3Result bal;
+        rr3_solve(n, feed, l1, l2, seed, o, bal, cnt);
+        switch (bal.status) {
+        case G_SUCCESS: break;
+        case G_DEGENERATE: return P_DEGENERATE;
+        case G_ITERATION_LIMIT: return P_BAL_ITER;
+        case G_LINE_SEARCH: return P_BAL_LS;
+        default: return P_NO_BALANCE;
+        }
+        s.frac[0] = 1.0 - bal.beta[0] - bal.beta[1]; s.frac[1] = bal.beta[0]; s.frac[2] = bal.beta[1];
+        for (int ph = 0; ph < 3; ++ph) {
+            for (int i = 0; i < n; ++i) { s.comp[ph][i] = bal.comp[ph][i]; }
+            if (evaluations >= o.max_evaluations) { return P_EVALUATION_LIMIT; }
+            ++evaluations; ++cnt.prop;
+            RPhase prop;
+            if (side_provider(m, p, t, s.comp[ph], sides[ph], prop) != NONE) { return P_PROPERTY; }
+            if (!prop.act.smooth || !std::isfinite(prop.z) || !(prop.z > 0)) { return P_PROPERTY; }
+            s.act[ph] = prop.act;
+            s.z[ph] = prop.z;
+        }
+        s.rr_res = bal.residual; s.mass_abs = bal.mass_abs; s.mass_rel = bal.mass_rel;
+        double gc = 0, mag = 1, mc = 0;
+        s.norm = 0; s.gibbs = 0;
+        for (int i = 0; i < n; ++i) {
+            s.lk1[i] = s.lk2[i] = s.r1[i] = s.r2[i] = s.common[i] = 0;
+            if (feed[i] == 0) { continue; }
+            const double l0 = std::log(s.comp[0][i]), ll1 = std::log(s.comp[1][i]), ll2 = std::log(s.comp[2][i]);
+            const double mu0 = l0 + s.act[0].lnphi[i], mu1 = ll1 + s.act[1].lnphi[i], mu2 = ll2 + s.act[2].lnphi[i];
+            s.lk1[i] = ll1 - l0;
+            s.lk2[i] = ll2 - l0;
+            s.r1[i] = mu0 - mu1;
+            s.r2[i] = mu0 - mu2;
+            s.common[i] = (mu0 + mu1 + mu2) / 3.0;
+            s.norm = std::max({s.norm, std::fabs(s.r1[i]), std::fabs(s.r2[i])});
+            for (int ph = 0; ph < 3; ++ph) {
+                const double lx = std::log(s.comp[ph][i]);
+                const double mu = lx + s.act[ph].lnphi[i];
+                kadd(s.frac[ph] * s.comp[ph][i] * mu, s.gibbs, gc);
+                kadd(s.frac[ph] * s.comp[ph][i] * (std::fabs(lx) + std::fabs(s.act[ph].lnphi[i])), mag, mc);
+            }
+        }
+        s.guard = 256.0 * EPS * mag;
+        if (!std::isfinite(s.norm) || !std::isfinite(s.gibbs) || !std::isfinite(s.guard)) { return P_PROPERTY; }
+        return -1;
+    };
+    {
+        const int e = evaluate(lk1_0, lk2_0, b0, R.pt);
+        if (e != -1) { R.status = e; return; }
+        R.has_point = true;
+    }
+    int iterations = 0;
+    for (;;) {
+        P3State& pt = R.pt;
+        if (pt.norm <= o.mu_tol) {
+            if (pt.rr_res > o.bal_tol || pt.mass_abs > o.mass_abs || pt.mass_rel > o.mass_rel) {
+                R.status = P_BAL_FAILURE; return;
+            }
+            for (int ph = 0; ph < 3; ++ph) {
+                if (pt.frac[ph] <= o.min_pf) { R.status = P_DISAPPEARANCE; R.disappearing = ph; return; }
+            }
+            const double d01 = log_distance(n, pt.comp[0], pt.comp[1], feed);
+            const double d02 = log_distance(n, pt.comp[0], pt.comp[2], feed);
+            const double d12 = log_distance(n, pt.comp[1], pt.comp[2], feed);
+            if (!(d01 > o.lsep) || !(d02 > o.lsep) || !(d12 > o.lsep)) { R.status = P_INDISTINGUISHABLE; return; }
+            R.status = P_CONVERGED;
+            return;
+        }
+        if (iterations >= o.max_iterations) { R.status = P_ITERATION_LIMIT; return; }
+        const double scale = std::max(1.0, pt.norm / o.max_log_step);
+        bool acc = false;
+        double alpha = 1.0;
+        for (int bt = 0; bt < o.max_backtracks; ++bt, alpha *= 0.5) {
+            double n1[NCMAX], n2[NCMAX];
+            for (int i = 0; i < n; ++i) {
+                n1[i] = pt.lk1[i] + (alpha / scale) * pt.r1[i];
+                n2[i] = pt.lk2[i] + (alpha / scale) * pt.r2[i];
+            }
+            const double seed[2] = {pt.frac[1], pt.frac[2]};
+            P3State nx;
+            const int e = evaluate(n1, n2, seed, nx);
+            if (e == -1) {
+                const double req = o.res_dec * (alpha / scale) * pt.norm;
+                const bool prog = nx.norm <= o.mu_tol || pt.norm - nx.norm > req;
+                const bool gok = nx.gibbs <= pt.gibbs + pt.guard + nx.guard;
+                if (prog && gok) { R.pt = nx; acc = true; break; }
+            } else if (e == P_EVALUATION_LIMIT) {
+                R.status = P_EVALUATION_LIMIT; return;
+            }
+        }
+        if (!acc) { R.status = P_LINE_SEARCH_FAILED; return; }
+        ++iterations;
+    }
+}
+
+// ---------------------------------------------------------------- max3
+enum Max3Status { M_SINGLE = 0, M_TWO, M_THREE, M_BOUNDARY, M_HIGHER, M_INDETERMINATE };
+struct Max3Result {
+    int status;
+    VleResult base;
+    bool use_neighbor;
+    VleResult neighbor;
+    P3State three;
+};
+
+// pr76_witness_root_sides, pr76_three_phase.hpp:273-305. Bits: 1 = lower, 2 = upper.
+int witness_sides(const Mixture& m, double p, double t, const double* w, int branch) {
+    PhaseCtx ctx;
+    if (prepare(m, p, t, w, ctx) != NONE) { return 0; }
+    int adm[3], na = 0;
+    for (int k = 0; k < ctx.rs.count; ++k) { if (ctx.rs.r[k].slope_sign > 0) { adm[na++] = k; } }
+    bool found = false;
+    for (int k = 0; k < na; ++k) { found = found || adm[k] == branch; }
+    if (na == 0 || !found) { return 0; }
+    if (na == 1) { return 3; }
+    if (branch == adm[0]) { return 1; }
+    if (branch == adm[na - 1]) { return 2; }
+    return 0;
+}
+
+bool log_ratio(int n, const double* num, const double* den, const double* z, double* out) { // :248-271
+    for (int i = 0; i < n; ++i) {
+        out[i] = 0;
+        if (z[i] == 0) { if (num[i] != 0 || den[i] != 0) { return false; } continue; }
+        if (!(num[i] > 0) || !(den[i] > 0)) { return false; }
+        out[i] = std::log(num[i]) - std::log(den[i]);
+        if (!std::isfinite(out[i])) { return false; }
+    }
+    return true;
+}
+
+// solve_pr76_pt_max3, pr76_three_phase.hpp:328-561 (no caller continuation hints).
+void solve_max3(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
+                Max3Result& M, Counters& cnt) {
+    const int n = m.n;
+    (void)check_composition(feed, n);
+    M.use_neighbor = false;
+    solve_vle(m, p, t, feed, so, nullptr, 0, M.base, cnt);
+    switch (M.base.status) {
+    case V_SINGLE: M.status = M_SINGLE; return;
+    case V_TWO: M.status = M_TWO; return;
+    case V_INDETERMINATE: M.status = M_INDETERMINATE; return;
+    default: break;
+    }
+    if (!M.base.has_candidate || !M.base.has_final) { M.status = M_INDETERMINATE; return; }
+    const StabResult& fin = M.base.fin;
+    const SplitState& pair = M.base.cand.pt;
+    const double* z = M.base.init.feed;
+    bool saw_higher = false, saw_boundary = false;
+    int automatic = 0;
+    bool auto_limit = false;
+    P3Options po;
+    // returns true when an attempt closes the solve
+    auto run_attempt = [&](const double* l1, const double* l2, const double* fr, const int* sides) -> bool {
+        if (automatic >= 16) { auto_limit = true; return false; }
+        ++automatic;
+        P3Result E;
+        iterate_three(m, p, t, z, l1, l2, fr, sides, po, E, cnt);
+        if (E.status == P_DISAPPEARANCE) {
+            saw_boundary = true;
+            if (E.has_point && E.disappearing >= 0) {
+                const double* st[2];
+                int ns = 0;
+                for (int ph = 0; ph < 3; ++ph) { if (ph != E.disappearing) { st[ns++] = E.pt.comp[ph]; } }
+                solve_vle(m, p, t, z, so, st, 2, M.neighbor, cnt);
+                if (M.neighbor.status == V_TWO) { M.status = M_TWO; M.use_neighbor = true; return true; }
+            }
+            return false;
+        }
+        if (E.status != P_CONVERGED) { return false; }
+        const P3State& c = E.pt;
+        if (!(c.norm <= po.mu_tol && c.rr_res <= po.bal_tol && c.mass_abs <= po.mass_abs &&
+              c.mass_rel <= po.mass_rel)) { return false; }
+        StabOptions fo;
+        fo.tpd_tol += c.norm;
+        const double* st[3] = {c.comp[0], c.comp[1], c.comp[2]};
+        StabResult* F = new StabResult;
+        stability(m, p, t, z, c.common, st, 3, fo, *F, cnt);
+        const int fs = F->status;
+        delete F;
+        if (fs == S_UNSTABLE) { saw_higher = true; }
+        else if (fs == S_NO_INSTABILITY) { M.status = M_THREE; M.three = c; return true; }
+        return false;
+    };
+    double src1[NCMAX];
+    if (log_ratio(n, pair.fr.y, pair.fr.x, z, src1)) {
+        for (int k = 0; k < fin.ntrials && !auto_limit; ++k) {
+            const Trial& w = fin.trials[k];
+            if (!(w.status == T_NEGATIVE && w.has_point && negative(w.pt, fin.opt))) { continue; }
+            double l2[NCMAX];
+            if (!log_ratio(n, w.pt.w, pair.fr.x, z, l2)) { continue; }
+            const int sides = witness_sides(m, p, t, w.pt.w, w.pt.branch);
+            for (int s = 0; s < 2; ++s) {
+                if (!(sides & (1 << s))) { continue; }
+                const double bv = pair.fr.beta;
+                const double fr[2] = {(1.0 - 0.1) * bv, 0.1};
+                const int sd[3] = {0, 1, s};
+                if (run_attempt(src1, l2, fr, sd)) { return; }
+                if (auto_limit) { break; }
+            }
+        }
+    }
+    M.status = saw_higher ? M_HIGHER : (saw_boundary ? M_BOUNDARY : M_INDETERMINATE);
+}
+
+// ---------------------------------------------------------------- publication
+struct Out {
+    int32_t* status;
+    int32_t* phase_count;
+    double* beta;
+    double* zfactor;
+    int32_t* branch;
+    double* x;
+    double* ln_phi;
+    uint32_t* counters;
+};
+
+struct Phase {
+    double beta;
+    const double* x;
+    const SPhase* act;
+    double z;
+};
+
+void write_cell(const Out& o, int n, int64_t N, int64_t c, int set, int max3, int split, int np, const Phase* ph,
+                const Counters& cnt, int flags) {
+    o.status[c] = set | (max3 << 4) | (split << 8) | flags;
+    o.phase_count[c] = np;
+    for (int k = 0; k < 3; ++k) {
+        const bool have = k < np;
+        o.beta[k * N + c] = have ? ph[k].beta : NaN;
+        o.zfactor[k * N + c] = have ? ph[k].z : NaN;
+        o.branch[k * N + c] = have ? (ph[k].act->branch | (ph[k].act->smooth ? 0x100 : 0)) : -1;
+        for (int i = 0; i < n; ++i) {
+            o.x[(k * n + i) * N + c] = have ? ph[k].x[i] : NaN;
+            o.ln_phi[(k * n + i) * N + c] = have ? ph[k].act->lnphi[i] : NaN;
+        }
+    }
+    if (o.counters) {
+        o.counters[0 * N + c] = static_cast<uint32_t>(cnt.prop);
+        o.counters[1 * N + c] = static_cast<uint32_t>(cnt.tpd);
+        o.counters[2 * N + c] = static_cast<uint32_t>(cnt.rr);
+        o.counters[3 * N + c] = static_cast<uint32_t>(cnt.rr3);
+    }
+}
+
+// pt_vle_phase_set.hpp:169-258: pair = [liquid (1-beta_V), vapor (beta_V)].
+int pair_phases(const VleResult& V, Phase* ph) {
+    const SplitState& s = V.cand.pt;
+    ph[0] = {1.0 - s.fr.beta, s.fr.x, &s.liq.act, s.liq.z};
+    ph[1] = {s.fr.beta, s.fr.y, &s.vap.act, s.vap.z};
+    return 2;
+}
+
+void publish_vle(const Out& o, int n, int64_t N, int64_t c, const VleResult& V, int max3_code, const Counters& cnt) {
+    Phase ph[3];
+    switch (V.status) {
+    case V_SINGLE:
+        if (!V.init.has_ref) { write_cell(o, n, N, c, 2, max3_code, V.status, 0, ph, cnt, 0); return; }
+        ph[0] = {1.0, V.init.feed, &V.init.ref, NaN};
+        write_cell(o, n, N, c, 0, max3_code, V.status, 1, ph, cnt, 0);
+        return;
+    case V_TWO:
+        write_cell(o, n, N, c, 0, max3_code, V.status, pair_phases(V, ph), ph, cnt, 0);
+        return;
+    case V_UNSTABLE:
+        write_cell(o, n, N, c, 1, max3_code, V.status, V.has_candidate ? pair_phases(V, ph) : 0, ph, cnt, 0);
+        return;
+    default:
+        write_cell(o, n, N, c, 2, max3_code, V.status, V.has_candidate ? pair_phases(V, ph) : 0, ph, cnt, 0);
+        return;
+    }
+}
+
+// pt_phase_set.hpp:90-150 (project_max3_solution).
+void publish_max3(const Out& o, int n, int64_t N, int64_t c, const Max3Result& M, const Counters& cnt) {
+    Phase ph[3];
+    const int split = M.base.status;
+    switch (M.status) {
+    case M_SINGLE:
+        publish_vle(o, n, N, c, M.base, M.status, cnt);
+        return;
+    case M_TWO: {
+        const VleResult& V = M.use_neighbor ? M.neighbor : M.base;
+        write_cell(o, n, N, c, 0, M.status, split, pair_phases(V, ph), ph, cnt, 0);
+        return;
+    }
+    case M_THREE:
+        for (int k = 0; k < 3; ++k) { ph[k] = {M.three.frac[k], M.three.comp[k], &M.three.act[k], M.three.z[k]}; }
+        write_cell(o, n, N, c, 0, M.status, split, 3, ph, cnt, 0);
+        return;
+    case M_HIGHER:
+        write_cell(o, n, N, c, 1, M.status, split, M.base.has_candidate ? pair_phases(M.base, ph) : 0, ph, cnt, 0);
+        return;
+    default:
+        write_cell(o, n, N, c, 2, M.status, split, 0, ph, cnt, 0);
+        return;
+    }
+}
+
+template <typename Body>
+void parallel_cells(int64_t ncell, int threads, Body body) {
+    if (threads <= 0) { threads = static_cast<int>(std::max(1U, std::thread::hardware_concurrency())); }
+    std::atomic<int64_t> next{0};
+    constexpr int64_t chunk = 16;
+    auto worker = [&]() {
+        for (;;) {
+            const int64_t begin = next.fetch_add(chunk);
+            if (begin >= ncell) { return; }
+            const int64_t end = std::min(ncell, begin + chunk);
+            for (int64_t c = begin; c < end; ++c) { body(c); }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) { pool.emplace_back(worker); }
+    worker();
+    for (auto& th : pool) { th.join(); }
+}
+
+} // namespace
+} // namespace cpuref
+
+using namespace cpuref;
+
+extern "C" int cpuref_flash_batch(int nc, const double* tc, const double* pc, const double* omega,
+                                  const double* kij, int64_t ncell, const double* p, const double* t,
+                                  const double* z, double fugacity_tolerance, int max_phases, int threads,
+                                  int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
+                                  int32_t* branch, double* x, double* ln_phi, uint32_t* counters) {
+    if (nc < 1 || nc > NCMAX || ncell < 0 || (max_phases != 2 && max_phases != 3)) { return 1; }
+    const Mixture m = make_mixture(nc, tc, pc, omega, kij);
+    const Out o{status, phase_count, beta, zfactor, branch, x, ln_phi, counters};
+    SplitOptions so;
+    so.fug_tol = fugacity_tolerance;
+    parallel_cells(ncell, threads, [&](int64_t c) {
+        double feed[NCMAX];
+        for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
+        Counters cnt;
+        auto M = std::make_unique<Max3Result>();
+        try {
+            if (max_phases == 2) {
+                solve_vle(m, p[c], t[c], feed, so, nullptr, 0, M->base, cnt);
+                publish_vle(o, nc, ncell, c, M->base, 0xF, cnt);
+            } else {
+                solve_max3(m, p[c], t[c], feed, so, *M, cnt);
+                publish_max3(o, nc, ncell, c, *M, cnt);
+            }
+        } catch (const Rejected&) {
+            Phase ph[3];
+            write_cell(o, nc, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+        }
+    });
+    return 0;
+}
+
+extern "C" int cpuref_phase_batch(int nc, const double* tc, const double* pc, const double* omega,
+                                  const double* kij, int64_t ncell, const double* p, const double* t,
+                                  const double* x, int32_t* root_status, int32_t* root_count, double* zroot,
+                                  int32_t* root_flags, double* ln_phi) {
+    if (nc < 1 || nc > NCMAX || ncell < 0) { return 1; }
+    const Mixture m = make_mixture(nc, tc, pc, omega, kij);
+    for (int64_t c = 0; c < ncell; ++c) {
+        double w[NCMAX];
+        for (int i = 0; i < nc; ++i) { w[i] = x[i * ncell + c]; }
+        root_count[c] = 0;
+        for (int k = 0; k < 3; ++k) {
+            zroot[k * ncell + c] = NaN;
+            root_flags[k * ncell + c] = 0;
+            for (int i = 0; i < nc; ++i) { ln_phi[(k * nc + i) * ncell + c] = NaN; }
+        }
+        PhaseCtx ctx;
+        try {
+            const int e = prepare(m, p[c], t[c], w, ctx);
+            if (e == ROOT_RANGE && ctx.rs.status == RS_SUCCESS) { root_status[c] = RS_UNREPRESENTABLE; continue; }
+            root_status[c] = ctx.rs.status;
+            if (e != NONE) { continue; }
+        } catch (const Rejected&) {
+            return 2;
+        }
+        root_count[c] = ctx.rs.count;
+        for (int k = 0; k < ctx.rs.count; ++k) {
+            const Root& r = ctx.rs.r[k];
+            zroot[k * ncell + c] = r.z;
+            root_flags[k * ncell + c] = (r.slope_sign + 1) | (r.dvalid ? 0x10 : 0);
+            double lp[NCMAX];
+            if (lnphi(m, p[c], t[c], ctx, k, lp, nullptr) == NONE) {
+                for (int i = 0; i < nc; ++i) { ln_phi[(k * nc + i) * ncell + c] = lp[i]; }
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" int cpuref_rachford_rice(int nc, const double* z, const double* log_k, double* beta, double* x,
+                                    double* y, int32_t* iterations) {
+    if (nc < 1 || nc > NCMAX) { return -1; }
+    RRResult r;
+    Counters cnt;
+    try {
+        rachford_rice(nc, z, log_k, r, cnt);
+    } catch (const Rejected&) {
+        return -2;
+    }
+    *beta = r.beta;
+    for (int i = 0; i < nc; ++i) { x[i] = r.x[i]; y[i] = r.y[i]; }
+    *iterations = r.iterations;
+    return r.status;
+}

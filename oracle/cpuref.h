@@ -1,0 +1,37 @@
+/*
+ * cpuref — the oracle: an FP64 CPU restatement of the reference's PR76 flash path.
+ *
+ * Test infrastructure only. Loaded by tests/ and by bench.py's CPU-baseline leg;
+ * the product library never links it. Result layout is the host-memory twin of
+ * include/mpmc_hnu_amd.h (structure of arrays, cell index fastest).
+ */
+#ifndef MPMC_CPUREF_H
+#define MPMC_CPUREF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* max_phases: 2 = VLE path (solve_pr76_pt_vle), 3 = max3 path (solve_pr76_pt_max3).
+ * threads <= 0: all hardware threads. Returns 0 on success. */
+int cpuref_flash_batch(int nc, const double* tc, const double* pc, const double* omega,
+                       const double* kij, int64_t ncell, const double* p, const double* t,
+                       const double* z, double fugacity_tolerance, int max_phases, int threads,
+                       int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
+                       int32_t* branch, double* x, double* ln_phi, uint32_t* counters);
+
+int cpuref_phase_batch(int nc, const double* tc, const double* pc, const double* omega,
+                       const double* kij, int64_t ncell, const double* p, const double* t,
+                       const double* x, int32_t* root_status, int32_t* root_count, double* zroot,
+                       int32_t* root_flags, double* ln_phi);
+
+/* Rachford-Rice at fixed log K for one feed: returns the RachfordRiceStatus code and
+ * writes beta, x[nc], y[nc], iterations. */
+int cpuref_rachford_rice(int nc, const double* z, const double* log_k, double* beta, double* x,
+                         double* y, int32_t* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,241 @@
+// Test-infrastructure driver around the reference's own header-only PR76 flash.
+//
+// Built by oracle/Makefile from the headers where they lie under the reference
+// checkout; nothing of the reference is copied into this repository. The result is
+// oracle/_ref/libmpmc_ref.so, loaded only by tests and by bench.py's CPU-baseline leg.
+// It exposes the reference path behind the same SoA result layout as the HIP library
+// (include/mpmc_hnu_amd.h) so that both can be compared array by array.
+//
+// Reference call chain reproduced here, per cell:
+//   max_phases == 3: solve_pr76_pt_max3 + project_pr76_pt_max3_phase_set
+//                    (modules/flash/include/mpmc/flash/pr76_pt_flash_backend.hpp:204-213)
+//   max_phases == 2: solve_pr76_pt_vle + project_pr76_pt_phase_set
+//                    (modules/flash/include/mpmc/flash/pr76_phase_set.hpp:42-52)
+#include <mpmc/flash/pr76_max3_phase_set.hpp>
+#include <mpmc/flash/pr76_phase_set.hpp>
+#include <mpmc/flash/pr76_three_phase.hpp>
+#include <mpmc/thermodynamics/pr76_phase.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace th = mpmc::thermodynamics;
+namespace fl = mpmc::flash;
+
+namespace {
+
+th::Provenance synthetic(const std::string& locator) {
+    return {th::SourceKind::synthetic_test, "mpmc-hnu-amd://oracle", "v1", locator,
+            "Synthetic mixture for parity vectors", "generated", "none"};
+}
+
+th::SourcedScalar datum(double value, th::Unit unit, const std::string& locator) {
+    return {value, unit, synthetic(locator), "SI", "identity"};
+}
+
+th::PrParameterSet make_parameters(int nc, const double* tc, const double* pc,
+                                   const double* omega, const double* kij) {
+    std::vector<th::Component> catalog;
+    th::PrParameterInput input;
+    input.model_id = std::string(th::pr76_profile);
+    input.dataset_id = "mpmc-hnu-amd-synthetic";
+    input.revision = "v1";
+    input.applicability = {std::nullopt, std::nullopt, synthetic("bounds")};
+    std::vector<std::string> order;
+    for (int i = 0; i < nc; ++i) {
+        const std::string id = "c" + std::to_string(i);
+        order.push_back(id);
+        catalog.push_back({id, id, th::ComponentKind::pure, synthetic(id), {}});
+        input.pure.push_back({id, datum(tc[i], th::Unit::kelvin, id + ".tc"),
+                              datum(pc[i], th::Unit::pascal, id + ".pc"),
+                              datum(omega[i], th::Unit::dimensionless, id + ".omega")});
+        for (int j = 0; j < i; ++j) {
+            input.binary.push_back({id, "c" + std::to_string(j),
+                                    datum(kij[i * nc + j], th::Unit::dimensionless, id + ".kij")});
+        }
+    }
+    return th::PrParameterSet::create(catalog, order, input, th::DataPolicy::allow_synthetic_tests);
+}
+
+struct Outputs {
+    int32_t* status;
+    int32_t* phase_count;
+    double* beta;
+    double* zfactor;
+    int32_t* branch;
+    double* x;
+    double* ln_phi;
+    uint32_t* counters;
+};
+
+// Provider-call and TPD-evaluation counts, read back from the reference's own
+// result records (StabilityResult::evaluations counts the reference phase too).
+struct Counts {
+    std::size_t prop = 0, tpd = 0;
+    void stability(const fl::StabilityResult& s) {
+        prop += s.evaluations;
+        tpd += s.evaluations - (s.reference ? 1U : 0U);
+    }
+    void split(const fl::PtSplitResult& r) {
+        stability(r.initial_stability);
+        prop += r.split_evaluations;
+        if (r.final_stability) { stability(*r.final_stability); }
+    }
+};
+
+void publish(const fl::PtPhaseSetResult& set, int nc, int64_t ncell, int64_t c, int max3,
+             int split, const Counts& counts, const Outputs& out) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    out.status[c] = static_cast<int32_t>(set.status) | (max3 << 4) | (split << 8);
+    const auto* phases = set.candidate_phase_set ? &set.candidate_phase_set->phases : nullptr;
+    out.phase_count[c] = phases ? static_cast<int32_t>(phases->size()) : 0;
+    for (int ph = 0; ph < 3; ++ph) {
+        const bool have = phases && ph < static_cast<int>(phases->size());
+        out.beta[ph * ncell + c] = have ? (*phases)[ph].mole_phase_fraction : nan;
+        out.zfactor[ph * ncell + c] =
+            have && (*phases)[ph].compressibility_factor ? *(*phases)[ph].compressibility_factor : nan;
+        out.branch[ph * ncell + c] =
+            have ? static_cast<int32_t>((*phases)[ph].activity.branch) |
+                       ((*phases)[ph].activity.smooth ? 0x100 : 0)
+                 : -1;
+        for (int i = 0; i < nc; ++i) {
+            out.x[(ph * nc + i) * ncell + c] = have ? (*phases)[ph].composition[i] : nan;
+            out.ln_phi[(ph * nc + i) * ncell + c] = have ? (*phases)[ph].activity.ln_phi[i] : nan;
+        }
+    }
+    if (out.counters) {
+        out.counters[0 * ncell + c] = static_cast<uint32_t>(counts.prop);
+        out.counters[1 * ncell + c] = static_cast<uint32_t>(counts.tpd);
+        out.counters[2 * ncell + c] = 0;
+        out.counters[3 * ncell + c] = 0;
+    }
+}
+
+void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evaluator, int nc,
+                int64_t ncell, int64_t c, const double* p, const double* t, const double* z,
+                double fugacity_tolerance, int max_phases, const Outputs& out) {
+    std::vector<double> feed(nc);
+    for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
+    Counts counts;
+    try {
+        if (max_phases == 2) {
+            fl::PtSplitOptions options;
+            options.iteration.fugacity_tolerance = fugacity_tolerance;
+            const auto legacy = fl::solve_pr76_pt_vle(p[c], t[c], feed, evaluator, options);
+            counts.split(legacy.solution);
+            const auto published = fl::project_pr76_pt_phase_set(legacy);
+            publish(published.solution, nc, ncell, c, 0xF, static_cast<int>(legacy.solution.status),
+                    counts, out);
+        } else {
+            fl::Pr76PtMax3Options options;
+            options.two_phase.iteration.fugacity_tolerance = fugacity_tolerance;
+            const auto source = fl::solve_pr76_pt_max3(p[c], t[c], feed, evaluator, options);
+            counts.split(source.base.solution);
+            for (const auto& attempt : source.attempts) {
+                counts.prop += attempt.equilibrium.evaluations;
+                if (attempt.final_stability) { counts.stability(*attempt.final_stability); }
+                if (attempt.boundary_neighbor) { counts.split(attempt.boundary_neighbor->solution); }
+            }
+            const auto published = fl::project_pr76_pt_max3_phase_set(source);
+            publish(published.solution, nc, ncell, c, static_cast<int>(source.status),
+                    static_cast<int>(source.base.solution.status), counts, out);
+        }
+    } catch (const std::exception&) {
+        fl::PtPhaseSetResult empty;
+        publish(empty, nc, ncell, c, 0xF, 0xF, counts, out);
+        out.status[c] |= 0x2000; // input rejected by the reference (exception)
+    }
+    (void)model;
+}
+
+template <typename Body>
+void parallel_cells(int64_t ncell, int threads, Body body) {
+    if (threads <= 0) { threads = static_cast<int>(std::max(1U, std::thread::hardware_concurrency())); }
+    std::atomic<int64_t> next{0};
+    constexpr int64_t chunk = 16;
+    auto worker = [&]() {
+        body([&](auto&& per_cell) {
+            for (;;) {
+                const int64_t begin = next.fetch_add(chunk);
+                if (begin >= ncell) { return; }
+                const int64_t end = std::min(ncell, begin + chunk);
+                for (int64_t c = begin; c < end; ++c) { per_cell(c); }
+            }
+        });
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) { pool.emplace_back(worker); }
+    worker();
+    for (auto& thread : pool) { thread.join(); }
+}
+
+} // namespace
+
+extern "C" {
+
+int ref_flash_batch(int nc, const double* tc, const double* pc, const double* omega,
+                    const double* kij, int64_t ncell, const double* p, const double* t,
+                    const double* z, double fugacity_tolerance, int max_phases, int threads,
+                    int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
+                    int32_t* branch, double* x, double* ln_phi, uint32_t* counters) {
+    try {
+        const auto parameters = make_parameters(nc, tc, pc, omega, kij);
+        const auto model = th::Pr76Phase<double>::from_parameters(parameters);
+        const Outputs out{status, phase_count, beta, zfactor, branch, x, ln_phi, counters};
+        parallel_cells(ncell, threads, [&](auto&& run) {
+            fl::Pr76VleEvaluator evaluator(model); // one evaluator per thread
+            run([&](int64_t c) {
+                solve_cell(model, evaluator, nc, ncell, c, p, t, z, fugacity_tolerance, max_phases, out);
+            });
+        });
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+int ref_phase_batch(int nc, const double* tc, const double* pc, const double* omega,
+                    const double* kij, int64_t ncell, const double* p, const double* t,
+                    const double* x, int32_t* root_status, int32_t* root_count, double* zroot,
+                    int32_t* root_flags, double* ln_phi) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    try {
+        const auto parameters = make_parameters(nc, tc, pc, omega, kij);
+        const auto model = th::Pr76Phase<double>::from_parameters(parameters);
+        th::Pr76PhaseWorkspace<double> workspace;
+        std::vector<double> w(nc);
+        for (int64_t c = 0; c < ncell; ++c) {
+            for (int i = 0; i < nc; ++i) { w[i] = x[i * ncell + c]; }
+            root_count[c] = 0;
+            for (int k = 0; k < 3; ++k) {
+                zroot[k * ncell + c] = nan;
+                root_flags[k * ncell + c] = 0;
+                for (int i = 0; i < nc; ++i) { ln_phi[(k * nc + i) * ncell + c] = nan; }
+            }
+            const auto roots = model.roots_full(p[c], t[c], w, workspace);
+            root_status[c] = static_cast<int32_t>(roots.status);
+            if (roots.status != th::Pr76RootStatus::success) { continue; }
+            root_count[c] = static_cast<int32_t>(roots.count);
+            for (std::size_t k = 0; k < roots.count; ++k) {
+                const auto& root = roots.roots[k];
+                zroot[k * ncell + c] = root.z;
+                root_flags[k * ncell + c] = (root.slope_sign + 1) | (root.derivative_valid ? 0x10 : 0);
+                const auto values = model.evaluate_full(p[c], t[c], std::span<const double>(w), k, workspace);
+                for (int i = 0; i < nc; ++i) { ln_phi[(k * nc + i) * ncell + c] = values.ln_phi[i]; }
+            }
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+} // extern "C"

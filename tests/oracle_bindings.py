@@ -1,0 +1,118 @@
+"""ctypes bindings of the checkers: the oracle (oracle/libcpuref.so) and, where it
+was built, the reference compiled from its own headers (oracle/_ref/libmpmc_ref.so).
+
+Both return results in the same structure-of-arrays layout as the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+CPUREF = ROOT / "oracle" / "libcpuref.so"
+REFLIB = ROOT / "oracle" / "_ref" / "libmpmc_ref.so"
+
+_P = ctypes.c_void_p
+_FLASH_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_double, ctypes.c_int,
+               ctypes.c_int] + [_P] * 8
+_PHASE_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P] + [_P] * 5
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_P)
+
+
+def _load(path: Path, prefix: str):
+    if not path.exists():
+        return None
+    lib = ctypes.CDLL(str(path))
+    getattr(lib, prefix + "_flash_batch").argtypes = _FLASH_ARGS
+    getattr(lib, prefix + "_phase_batch").argtypes = _PHASE_ARGS
+    return lib
+
+
+_libs: dict = {}
+
+
+def lib(which: str = "cpuref"):
+    """which: 'cpuref' (the oracle) or 'ref' (reference build; None when absent)."""
+    if which not in _libs:
+        _libs[which] = _load(CPUREF, "cpuref") if which == "cpuref" else _load(REFLIB, "ref")
+    return _libs[which]
+
+
+def have_ref() -> bool:
+    return lib("ref") is not None
+
+
+def _mix_arrays(mix):
+    return (np.ascontiguousarray(mix["tc"], dtype=np.float64), np.ascontiguousarray(mix["pc"], dtype=np.float64),
+            np.ascontiguousarray(mix["omega"], dtype=np.float64),
+            np.ascontiguousarray(mix["kij"], dtype=np.float64))
+
+
+def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which="cpuref"):
+    """Run the checker flash on host arrays; returns a dict of SoA arrays."""
+    L = lib(which)
+    if L is None:
+        raise RuntimeError(f"checker library '{which}' is not built")
+    tc, pc, om, kij = _mix_arrays(mix)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    nc, n = z.shape
+    out = {
+        "status": np.zeros(n, np.int32),
+        "phase_count": np.zeros(n, np.int32),
+        "beta": np.zeros((3, n)),
+        "zfactor": np.zeros((3, n)),
+        "branch": np.zeros((3, n), np.int32),
+        "x": np.zeros((3, nc, n)),
+        "ln_phi": np.zeros((3, nc, n)),
+        "counters": np.zeros((4, n), np.uint32),
+    }
+    fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch")
+    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), float(fugacity_tolerance),
+            int(max_phases), int(threads), _ptr(out["status"]), _ptr(out["phase_count"]), _ptr(out["beta"]),
+            _ptr(out["zfactor"]), _ptr(out["branch"]), _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]))
+    if rc != 0:
+        raise RuntimeError(f"{which}_flash_batch failed rc={rc}")
+    return out
+
+
+def phase(mix, p, t, x, which="cpuref"):
+    L = lib(which)
+    tc, pc, om, kij = _mix_arrays(mix)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nc, n = x.shape
+    out = {
+        "root_status": np.zeros(n, np.int32),
+        "root_count": np.zeros(n, np.int32),
+        "zroot": np.zeros((3, n)),
+        "root_flags": np.zeros((3, n), np.int32),
+        "ln_phi": np.zeros((3, nc, n)),
+    }
+    fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_phase_batch")
+    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(x), _ptr(out["root_status"]),
+            _ptr(out["root_count"]), _ptr(out["zroot"]), _ptr(out["root_flags"]), _ptr(out["ln_phi"]))
+    if rc != 0:
+        raise RuntimeError(f"{which}_phase_batch failed rc={rc}")
+    return out
+
+
+def rachford_rice(z, log_k):
+    L = lib("cpuref")
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    lk = np.ascontiguousarray(log_k, dtype=np.float64)
+    n = z.size
+    beta = ctypes.c_double()
+    x = np.zeros(n)
+    y = np.zeros(n)
+    it = ctypes.c_int32()
+    st = L.cpuref_rachford_rice(n, _ptr(z), _ptr(lk), ctypes.byref(beta), _ptr(x), _ptr(y), ctypes.byref(it))
+    return st, beta.value, x, y, it.value
