@@ -85,9 +85,12 @@ typedef struct mpmc_flash_options {
     /* 2: solve_pr76_pt_phase_set (VLE, pr76_phase_set.hpp:42-52)
      * 3: Pr76PtFlashBackend::solve (max3, pr76_pt_flash_backend.hpp:190-237) */
     int32_t max_phases;
-    /* 0: every evaluation in FP64. 1: mixed precision (see DESIGN). */
+    /* 0: every evaluation in FP64. */
     int32_t precision;
-    int32_t reserved[4];
+    /* 0: staged batch (one lane per stability trial / split attempt, persistent kernels)
+     * 1: serial (one lane per cell runs the whole search) */
+    int32_t schedule;
+    int32_t reserved[3];
 } mpmc_flash_options;
 
 typedef struct mpmc_flash_outputs {
@@ -112,6 +115,19 @@ void mpmc_flash_default_options(mpmc_flash_options* options);
 int mpmc_flash_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
                      const double* z, const mpmc_flash_options* options,
                      const mpmc_flash_outputs* out, void* stream);
+
+/* Device workspace bytes the staged schedule needs for ncell cells of an nc-component
+ * mixture (0 when nc is out of range). */
+size_t mpmc_flash_workspace_bytes(int nc, int64_t ncell);
+
+/* mpmc_flash_batch with a caller-owned device workspace of at least
+ * mpmc_flash_workspace_bytes(nc, ncell) bytes. mpmc_flash_batch itself keeps one
+ * workspace per mixture handle (grown on demand), so concurrent calls on one handle
+ * from different streams must use this entry point with separate workspaces. */
+int mpmc_flash_batch_ws(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                        const double* z, const mpmc_flash_options* options,
+                        const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes,
+                        void* stream);
 
 /*
  * Candidate-phase PR76 evaluation, one composition per cell: all roots, then ln(phi) of

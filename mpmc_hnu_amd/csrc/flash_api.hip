@@ -1,7 +1,7 @@
 // C ABI of include/mpmc_hnu_amd.h: mixture residency, batched flash and batched
 // phase evaluation. Argument checks mirror the reference's input contracts; numerical
 // outcomes are per-cell statuses written by the kernels.
-#include "flash_kernels.hpp"
+#include "flash_pipeline.hpp"
 
 #include "../../include/mpmc_hnu_amd.h"
 
@@ -15,6 +15,7 @@
 
 namespace mpmc::dev {
 MPMC_FOR_EACH_NC(MPMC_DECLARE_LAUNCH)
+MPMC_FOR_EACH_NC(MPMC_DECLARE_PIPELINE)
 } // namespace mpmc::dev
 
 namespace {
@@ -47,7 +48,22 @@ struct mpmc_mixture {
     int nc;
     int device;
     double* dev_constants; // sqrt_tc, ac, b, kappa [nc each], factor [nc*nc], r_gas
+    void* workspace;       // staged-schedule workspace used by mpmc_flash_batch
+    size_t workspace_bytes;
 };
+
+namespace {
+
+size_t workspace_bytes_for(int nc, int64_t ncell) {
+    switch (nc) {
+#define MPMC_CASE(N) case N: return mpmc::dev::pipeline_bytes<N>(ncell);
+        MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+    default: return 0;
+    }
+}
+
+} // namespace
 
 extern "C" {
 
@@ -59,6 +75,7 @@ void mpmc_flash_default_options(mpmc_flash_options* o) {
     o->fugacity_tolerance = 1e-11;
     o->max_phases = 3;
     o->precision = 0;
+    o->schedule = 0;
 }
 
 int mpmc_mixture_create(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
@@ -100,7 +117,7 @@ int mpmc_mixture_create(int nc, const double* tc, const double* pc, const double
     }
     for (int k = 0; k < nc * nc; ++k) { factor[k] = 1.0 - kij[k]; }
     h[4 * nc + nc * nc] = r_gas;
-    auto* m = new (std::nothrow) mpmc_mixture{nc, 0, nullptr};
+    auto* m = new (std::nothrow) mpmc_mixture{nc, 0, nullptr, nullptr, 0};
     if (!m) { return fail(MPMC_E_NO_MEMORY, "mixture_create: host allocation"); }
     hipError_t e = hipGetDevice(&m->device);
     if (e == hipSuccess) { e = hipMalloc(&m->dev_constants, h.size() * sizeof(double)); }
@@ -116,17 +133,27 @@ int mpmc_mixture_create(int nc, const double* tc, const double* pc, const double
 
 int mpmc_mixture_destroy(mpmc_mixture_t m) {
     if (!m) { return MPMC_OK; }
-    const hipError_t e = m->dev_constants ? hipFree(m->dev_constants) : hipSuccess;
+    hipError_t e = m->dev_constants ? hipFree(m->dev_constants) : hipSuccess;
+    if (m->workspace) {
+        const hipError_t e2 = hipFree(m->workspace);
+        if (e == hipSuccess) { e = e2; }
+    }
     delete m;
     return e == hipSuccess ? MPMC_OK : hip_fail(e, "mixture_destroy");
 }
 
 int mpmc_mixture_components(mpmc_mixture_t m) { return m ? m->nc : 0; }
 
-int mpmc_flash_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
-                     const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* stream) {
+size_t mpmc_flash_workspace_bytes(int nc, int64_t ncell) {
+    if (nc < 1 || nc > MPMC_MAX_COMPONENTS || ncell < 0) { return 0; }
+    return workspace_bytes_for(nc, ncell);
+}
+
+static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                            const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
+                            size_t workspace_bytes, bool own_workspace, void* stream) {
     if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: null mixture or outputs"); }
-    if (ncell < 0 || ncell > (int64_t{1} << 40)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
+    if (ncell < 0 || ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
     mpmc_flash_options o;
     mpmc_flash_default_options(&o);
     if (options) { o = *options; }
@@ -137,6 +164,7 @@ int mpmc_flash_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const dou
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: fugacity tolerance must be finite and positive");
     }
     if (o.precision != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: precision mode not available"); }
+    if (o.schedule != 0 && o.schedule != 1) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: schedule must be 0 or 1"); }
     if (ncell == 0) { return MPMC_OK; }
     const void* ptrs[] = {p, t, z, out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
                           out->ln_phi};
@@ -150,16 +178,67 @@ int mpmc_flash_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const dou
                                  out->ln_phi, out->counters};
     const hipStream_t s = static_cast<hipStream_t>(stream);
     hipError_t e = hipErrorInvalidValue;
-    switch (m->nc) {
-#define MPMC_CASE(N)                                                                                              \
-    case N:                                                                                                       \
+    if (o.schedule == 1) {
+        switch (m->nc) {
+#define MPMC_CASE(N)                                                                                                 \
+    case N:                                                                                                          \
         e = mpmc::dev::launch_flash<N>(m->dev_constants, ncell, p, t, z, o.max_phases, o.fugacity_tolerance, fo, s); \
+        break;
+            MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+        default: return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: unsupported nc");
+        }
+        return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_batch launch");
+    }
+    const size_t need = workspace_bytes_for(m->nc, ncell);
+    if (own_workspace) {
+        if (m->workspace_bytes < need) {
+            if (m->workspace) {
+                // the previous workspace may still be read by work queued on this stream
+                e = hipStreamSynchronize(s);
+                if (e != hipSuccess) { return hip_fail(e, "flash_batch workspace sync"); }
+                (void)hipFree(m->workspace);
+                m->workspace = nullptr;
+                m->workspace_bytes = 0;
+            }
+            e = hipMalloc(&m->workspace, need);
+            if (e != hipSuccess) {
+                m->workspace = nullptr;
+                return fail(MPMC_E_NO_MEMORY, std::string("flash_batch: workspace allocation of ") +
+                                                  std::to_string(need) + " bytes failed");
+            }
+            m->workspace_bytes = need;
+        }
+        workspace = m->workspace;
+        workspace_bytes = m->workspace_bytes;
+    } else {
+        if (!workspace || workspace_bytes < need || !is_device_ptr(workspace)) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_ws: workspace must be device memory of "
+                                                 "mpmc_flash_workspace_bytes(nc, ncell) bytes");
+        }
+    }
+    switch (m->nc) {
+#define MPMC_CASE(N)                                                                                             \
+    case N:                                                                                                      \
+        e = mpmc::dev::launch_pipeline<N>(m->dev_constants, workspace, workspace_bytes, ncell, p, t, z,          \
+                                          o.max_phases, o.fugacity_tolerance, fo, s);                            \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE
     default: return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: unsupported nc");
     }
     return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_batch launch");
+}
+
+int mpmc_flash_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                     const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* stream) {
+    return flash_batch_impl(m, ncell, p, t, z, options, out, nullptr, 0, true, stream);
+}
+
+int mpmc_flash_batch_ws(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                        const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, false, stream);
 }
 
 int mpmc_phase_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* x,

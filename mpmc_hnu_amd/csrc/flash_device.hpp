@@ -937,43 +937,39 @@ __device__ __forceinline__ bool log_ratio(const double* num, const double* den, 
 
 enum Max3Status : int { M_SINGLE = 0, M_TWO, M_THREE, M_BOUNDARY, M_HIGHER, M_INDETERMINATE };
 
+// Output of the max3 orchestration after an unstable two-phase review: the accepted
+// three-phase state, or the fresh two-phase neighbour of a disappearance boundary.
 template <int NC>
-struct Max3Out {
+struct Max3Tail {
     int status;
-    VleOut<NC> base;
     bool use_neighbor;
-    VleOut<NC> neighbor;
+    SplitState<NC> neighbor;
     P3State<NC> three;
 };
 
-// solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), without caller continuation hints.
-template <int NC>
-__device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
-                                        Max3Out<NC>& M, Counters& cnt, bool& rejected) {
+// The part of solve_pr76_pt_max3 (pr76_three_phase.hpp:360-561) that runs once the
+// two-phase common-tangent review found lower-Gibbs evidence: negative final-review
+// witnesses seed three-phase attempts (witness sides from pr76_witness_root_sides),
+// each followed by the three-phase final review or, on disappearance, a fresh
+// two-phase neighbour solve. `witness(i, w, branch)` yields the i-th negative witness
+// of the review in trial order. No caller continuation hints.
+template <int NC, typename WitnessFn>
+__device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, const double* z, const double* px,
+                                       const double* py, double pbeta, int nwit, WitnessFn witness, double fug_tol,
+                                       Max3Tail<NC>& M, Counters& cnt, bool& rejected) {
     M.use_neighbor = false;
-    (void)check_composition<NC>(feed, rejected);
-    if (rejected) { return; }
-    solve_vle<NC>(m, p, t, feed, fug_tol, nullptr, 0, M.base, cnt, rejected);
-    if (rejected) { return; }
-    switch (M.base.status) {
-    case V_SINGLE: M.status = M_SINGLE; return;
-    case V_TWO: M.status = M_TWO; return;
-    case V_INDETERMINATE: M.status = M_INDETERMINATE; return;
-    default: break;
-    }
-    if (!M.base.has_candidate || !M.base.has_final) { M.status = M_INDETERMINATE; return; }
-    const SplitState<NC>& pair = M.base.cand;
-    const double* z = M.base.init.feed;
     bool saw_higher = false, saw_boundary = false;
     int automatic = 0;
     bool auto_limit = false;
     double src1[NC];
-    if (log_ratio<NC>(pair.fr.y, pair.fr.x, z, src1)) {
-        for (int k = 0; k < M.base.fin.nwit && !auto_limit; ++k) {
-            const Witness<NC>& w = M.base.fin.wit[k];
+    if (log_ratio<NC>(py, px, z, src1)) {
+        for (int k = 0; k < nwit && !auto_limit; ++k) {
+            double ww[NC];
+            int wbranch;
+            witness(k, ww, wbranch);
             double l2[NC];
-            if (!log_ratio<NC>(w.w, pair.fr.x, z, l2)) { continue; }
-            const int sides = witness_sides<NC>(m, p, t, w.w, w.branch);
+            if (!log_ratio<NC>(ww, px, z, l2)) { continue; }
+            const int sides = witness_sides<NC>(m, p, t, ww, wbranch);
             for (int s = 0; s < 2; ++s) {
                 if (!(sides & (1 << s))) { continue; }
                 if (automatic >= 16) { auto_limit = true; break; }
@@ -981,8 +977,8 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
                 const int sd[3] = {0, 1, s};
                 int disappearing;
                 P3State<NC> E;
-                const int st = iterate_three<NC>(m, p, t, z, src1, l2, (1.0 - 0.1) * pair.fr.beta, 0.1, sd, E,
-                                                 disappearing, cnt, rejected);
+                const int st = iterate_three<NC>(m, p, t, z, src1, l2, (1.0 - 0.1) * pbeta, 0.1, sd, E, disappearing,
+                                                 cnt, rejected);
                 if (rejected) { return; }
                 if (st == P_DISAPPEARANCE) {
                     saw_boundary = true;
@@ -990,9 +986,15 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
                         const double* stv[2];
                         int ns = 0;
                         for (int ph = 0; ph < 3; ++ph) { if (ph != disappearing) { stv[ns++] = E.comp[ph]; } }
-                        solve_vle<NC>(m, p, t, z, fug_tol, stv, 2, M.neighbor, cnt, rejected);
+                        VleOut<NC> nb;
+                        solve_vle<NC>(m, p, t, z, fug_tol, stv, 2, nb, cnt, rejected);
                         if (rejected) { return; }
-                        if (M.neighbor.status == V_TWO) { M.status = M_TWO; M.use_neighbor = true; return; }
+                        if (nb.status == V_TWO) {
+                            M.status = M_TWO;
+                            M.use_neighbor = true;
+                            M.neighbor = nb.cand;
+                            return;
+                        }
                     }
                     continue;
                 }
@@ -1008,6 +1010,41 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
         }
     }
     M.status = saw_higher ? M_HIGHER : (saw_boundary ? M_BOUNDARY : M_INDETERMINATE);
+}
+
+template <int NC>
+struct Max3Out {
+    int status;
+    VleOut<NC> base;
+    Max3Tail<NC> tail;
+};
+
+// solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), without caller continuation hints,
+// serial in one lane.
+template <int NC>
+__device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
+                                        Max3Out<NC>& M, Counters& cnt, bool& rejected) {
+    M.tail.use_neighbor = false;
+    (void)check_composition<NC>(feed, rejected);
+    if (rejected) { return; }
+    solve_vle<NC>(m, p, t, feed, fug_tol, nullptr, 0, M.base, cnt, rejected);
+    if (rejected) { return; }
+    switch (M.base.status) {
+    case V_SINGLE: M.status = M_SINGLE; return;
+    case V_TWO: M.status = M_TWO; return;
+    case V_INDETERMINATE: M.status = M_INDETERMINATE; return;
+    default: break;
+    }
+    if (!M.base.has_candidate || !M.base.has_final) { M.status = M_INDETERMINATE; return; }
+    const StabOut<NC>& fin = M.base.fin;
+    max3_tail<NC>(m, p, t, M.base.init.feed, M.base.cand.fr.x, M.base.cand.fr.y, M.base.cand.fr.beta, fin.nwit,
+                  [&](int i, double* w, int& branch) {
+#pragma unroll
+                      for (int c = 0; c < NC; ++c) { w[c] = fin.wit[i].w[c]; }
+                      branch = fin.wit[i].branch;
+                  },
+                  fug_tol, M.tail, cnt, rejected);
+    M.status = M.tail.status;
 }
 
 } // namespace mpmc::dev

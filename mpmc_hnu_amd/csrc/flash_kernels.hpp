@@ -104,12 +104,12 @@ __device__ void publish_max3(const FlashOut& o, int64_t N, int64_t c, const Max3
         publish_vle<NC>(o, N, c, M.base, M.status, cnt);
         return;
     case M_TWO:
-        write_cell<NC>(o, N, c, 0, M.status, split, pair_view<NC>(M.use_neighbor ? M.neighbor.cand : M.base.cand, ph),
+        write_cell<NC>(o, N, c, 0, M.status, split, pair_view<NC>(M.tail.use_neighbor ? M.tail.neighbor : M.base.cand, ph),
                        ph, cnt, 0);
         return;
     case M_THREE:
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { ph[k] = {M.three.frac[k], M.three.comp[k], &M.three.act[k], M.three.z[k]}; }
+        for (int k = 0; k < 3; ++k) { ph[k] = {M.tail.three.frac[k], M.tail.three.comp[k], &M.tail.three.act[k], M.tail.three.z[k]}; }
         write_cell<NC>(o, N, c, 0, M.status, split, 3, ph, cnt, 0);
         return;
     case M_HIGHER:

@@ -101,10 +101,15 @@ def allocate_outputs(nc: int, ncell: int, device, counters: bool = True) -> dict
 
 
 def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, max_phases: int = 3,
-                precision: int = 0, stream=None, out: dict | None = None, counters: bool = True) -> dict:
+                precision: int = 0, stream=None, out: dict | None = None, counters: bool = True,
+                schedule: int = 0, workspace=None) -> dict:
     """Flash every cell (p[ncell] Pa, t[ncell] K, z[nc, ncell]); returns device tensors.
 
-    Asynchronous on ``stream`` (default: torch's current stream)."""
+    schedule 0 runs the staged batch (one lane per stability trial / split attempt),
+    schedule 1 one lane per cell. ``workspace`` (a uint8 device tensor of at least
+    workspace_bytes(nc, ncell)) makes the call use caller-owned scratch; otherwise the
+    mixture handle's own workspace is used. Asynchronous on ``stream`` (default:
+    torch's current stream)."""
     torch = _torch()
     lib = native.library()
     device = torch.device("cuda", torch.cuda.current_device())
@@ -121,15 +126,26 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
     opts.fugacity_tolerance = float(fugacity_tolerance)
     opts.max_phases = int(max_phases)
     opts.precision = int(precision)
+    opts.schedule = int(schedule)
     o = native.FlashOutputs(
         out["status"].data_ptr(), out["phase_count"].data_ptr(), out["beta"].data_ptr(), out["zfactor"].data_ptr(),
         out["branch"].data_ptr(), out["x"].data_ptr(), out["ln_phi"].data_ptr(),
         out["counters"].data_ptr() if out.get("counters") is not None else None)
-    native.check(lib.mpmc_flash_batch(mixture.handle, n, ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(t.data_ptr()),
-                                      ctypes.c_void_p(z.data_ptr()), ctypes.byref(opts), ctypes.byref(o),
-                                      _stream_ptr(stream)), "flash_batch")
+    V = ctypes.c_void_p
+    if workspace is None:
+        rc = lib.mpmc_flash_batch(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
+                                  ctypes.byref(opts), ctypes.byref(o), _stream_ptr(stream))
+    else:
+        rc = lib.mpmc_flash_batch_ws(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
+                                     ctypes.byref(opts), ctypes.byref(o), V(workspace.data_ptr()),
+                                     workspace.numel() * workspace.element_size(), _stream_ptr(stream))
+    native.check(rc, "flash_batch")
     out["_inputs"] = (p, t, z)  # keep inputs alive until the stream consumes them
     return out
+
+
+def workspace_bytes(nc: int, ncell: int) -> int:
+    return int(native.library().mpmc_flash_workspace_bytes(nc, ncell))
 
 
 def phase_batch(mixture: Mixture, p, t, x, stream=None) -> dict:

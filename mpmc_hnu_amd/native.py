@@ -19,6 +19,8 @@ EXPORTED = (
     "mpmc_mixture_components",
     "mpmc_flash_default_options",
     "mpmc_flash_batch",
+    "mpmc_flash_batch_ws",
+    "mpmc_flash_workspace_bytes",
     "mpmc_phase_batch",
     "mpmc_last_error",
 )
@@ -29,7 +31,8 @@ class FlashOptions(ctypes.Structure):
         ("fugacity_tolerance", ctypes.c_double),
         ("max_phases", ctypes.c_int32),
         ("precision", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("schedule", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -70,6 +73,10 @@ def library() -> ctypes.CDLL:
     lib.mpmc_flash_default_options.restype = None
     lib.mpmc_flash_batch.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOptions),
                                      ctypes.POINTER(FlashOutputs), P]
+    lib.mpmc_flash_batch_ws.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOptions),
+                                        ctypes.POINTER(FlashOutputs), P, ctypes.c_size_t, P]
+    lib.mpmc_flash_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64]
+    lib.mpmc_flash_workspace_bytes.restype = ctypes.c_size_t
     lib.mpmc_phase_batch.argtypes = [P, ctypes.c_int64, P, P, P, P, P, P, P, P, P]
     lib.mpmc_last_error.restype = ctypes.c_char_p
     _lib = lib

@@ -109,3 +109,39 @@ def test_phase_batch_roots(gpu):
     assert np.array_equal(g["zroot"][fin], o["zroot"][fin])
     fl = np.isfinite(o["ln_phi"])
     assert np.allclose(g["ln_phi"][fl], o["ln_phi"][fl], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("maxph", [2, 3])
+def test_staged_and_serial_schedules_agree(gpu, maxph):
+    """The staged batch (lane per trial / attempt) and the serial one-lane-per-cell path
+    run the same searches: identical decisions and values."""
+    from mpmc_hnu_amd import flash
+
+    mix = W.mixture_linspaced(5)
+    p, t, z = W.cells_uniform(5, 2048, seed=77)
+    m = flash.Mixture.from_dict(mix)
+    a = flash.to_host(flash.flash_batch(m, p, t, z, max_phases=maxph, schedule=0))
+    b = flash.to_host(flash.flash_batch(m, p, t, z, max_phases=maxph, schedule=1))
+    m.close()
+    rep, bad = parity.compare(a, b, rel=0.0, abs_lnphi=0.0)
+    print("staged vs serial", rep, flush=True)
+    assert not bad.any(), rep
+    assert np.array_equal(a["counters"], b["counters"])
+
+
+@pytest.mark.gpu
+def test_caller_workspace(gpu):
+    import torch
+
+    from mpmc_hnu_amd import flash
+
+    mix = W.mixture_linspaced(4)
+    p, t, z = W.cells_uniform(4, 1000, seed=5)
+    m = flash.Mixture.from_dict(mix)
+    ws = torch.empty(flash.workspace_bytes(4, 1000), dtype=torch.uint8, device=gpu)
+    a = flash.to_host(flash.flash_batch(m, p, t, z, workspace=ws))
+    b = flash.to_host(flash.flash_batch(m, p, t, z))
+    m.close()
+    rep, bad = parity.compare(a, b, rel=0.0, abs_lnphi=0.0)
+    assert not bad.any(), rep
