@@ -144,6 +144,20 @@ int mpmc_phase_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, con
                      const double* x, int32_t* root_status, int32_t* root_count, double* zroot,
                      int32_t* root_flags, double* ln_phi, void* stream);
 
+/*
+ * Rachford-Rice at fixed log K, one feed per cell: solve_rachford_rice
+ * (modules/flash/include/mpmc/flash/rachford_rice.hpp:70-147).
+ *   z[nc][n], log_k[nc][n]; guess[n] (optional, NULL = 0.5) only steers the root enclosure
+ *   that lets the bisection skip evaluations whose sign is certified, and never changes the
+ *   result; a negative guess runs the plain bisection.
+ *   status[n]: RachfordRiceStatus (interior=0, no_resolved_interior_root=1, degenerate=2,
+ *              iteration_limit=3, unrepresentable=4; -2 = input rejected)
+ *   beta[n]; x[nc][n], y[nc][n] (NaN unless interior); iterations[n] bisection steps
+ */
+int mpmc_rr_batch(int nc, int64_t n, const double* z, const double* log_k, const double* guess,
+                  int32_t* status, double* beta, double* x, double* y, int32_t* iterations,
+                  void* stream);
+
 /* Last error text for this thread (never NULL). */
 const char* mpmc_last_error(void);
 

@@ -266,4 +266,26 @@ int mpmc_phase_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const dou
     return e == hipSuccess ? MPMC_OK : hip_fail(e, "phase_batch launch");
 }
 
+int mpmc_rr_batch(int nc, int64_t n, const double* z, const double* log_k, const double* guess, int32_t* status,
+                  double* beta, double* x, double* y, int32_t* iterations, void* stream) {
+    if (nc < 1 || nc > MPMC_MAX_COMPONENTS || n < 0 || n > (int64_t{1} << 31)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "rr_batch: nc outside [1, 12] or bad n");
+    }
+    if (n == 0) { return MPMC_OK; }
+    const void* ptrs[] = {z, log_k, status, beta, x, y, iterations};
+    for (const void* ptr : ptrs) {
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "rr_batch: buffers must be device memory"); }
+    }
+    if (guess && !is_device_ptr(guess)) { return fail(MPMC_E_INVALID_ARGUMENT, "rr_batch: guess must be device memory"); }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipErrorInvalidValue;
+    switch (nc) {
+#define MPMC_CASE(N) case N: e = mpmc::dev::launch_rr<N>(n, z, log_k, guess, status, beta, x, y, iterations, s); break;
+        MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+    default: break;
+    }
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "rr_batch launch");
+}
+
 } // extern "C"

@@ -75,7 +75,7 @@ struct StabOut {
 
 // tangent_plane_distance (pt_stability.hpp:333-375)
 template <int NC>
-__device__ __noinline__ int tpd_point(const double* w, const double* z, const Activity<NC>& trial,
+MPMC_HOT int tpd_point(const double* w, const double* z, const Activity<NC>& trial,
                                       const Activity<NC>& ref, TpdPoint<NC>& r, bool& rejected) {
     const double sum = check_composition<NC>(w, rejected);
     (void)check_composition<NC>(z, rejected);
@@ -303,6 +303,78 @@ struct RRState {
     double mass_abs, mass_rel;
 };
 
+// rr_value in the reference's arithmetic order (rachford_rice.hpp:53-63), with the
+// per-component exp(-|lk|) and expm1 numerators precomputed (same values).
+template <int NC>
+__device__ __forceinline__ double rr_value_exact(double beta, const double* z, const double* lk, const double* e,
+                                                 const double* num) {
+    double v = 0, c = 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        if (z[i] == 0) { continue; }
+        const double den = lk[i] >= 0 ? beta + (1 - beta) * e[i] : (1 - beta) + beta * e[i];
+        kadd((z[i] / den) * num[i], v, c);
+    }
+    return v;
+}
+
+struct RRBracket {
+    bool ok;
+    double lo, hi, a, s2;
+};
+
+// Enclosure of the exact root of the RR function used to skip bisection evaluations.
+template <int NC>
+__device__ __forceinline__ RRBracket rr_enclose(const double* z, const double* lk, const double* e,
+                                                const double* num, double guess) {
+    RRBracket br{false, 0, 1, 0, 0};
+    if (!(guess >= 0)) { return br; } // negative or NaN guess: plain bisection
+    double a = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        if (z[i] > 0) {
+            a += z[i] * __builtin_fabs(num[i]);
+            s2 += z[i] * num[i] * num[i];
+        }
+    }
+    if (!(a > 1e-200) || !(s2 > 1e-200) || !finite(a) || !finite(s2)) { return br; }
+    br.a = a * (1.0 + 64 * kEps);  // upper bound on the rounded sum
+    br.s2 = s2 * (1.0 - 64 * kEps); // lower bound
+    double b = (guess > 0 && guess < 1) ? guess : 0.5, lo = 0, hi = 1;
+    for (int it = 0; it < 48; ++it) {
+        double f = 0, fp = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            if (z[i] == 0) { continue; }
+            const double den = lk[i] >= 0 ? b + (1 - b) * e[i] : (1 - b) + b * e[i];
+            const double q = num[i] / den;
+            f += z[i] * q;
+            fp -= z[i] * q * q;
+        }
+        if (!finite(f) || !finite(fp) || !(fp < 0)) { return br; }
+        if (f > 0) { lo = b; } else if (f < 0) { hi = b; } else { break; }
+        double bn = b - f / fp;
+        if (!(bn > lo && bn < hi)) { bn = 0.5 * (lo + hi); }
+        const double step = __builtin_fabs(bn - b);
+        b = bn;
+        if (step <= 2 * kEps * smin(b, 1 - b)) { break; }
+    }
+    const double dm = smin(b, 1 - b);
+    if (!(dm > 1e-150)) { return br; }
+    const double delta = 4.0 * (32.0 * kEps * br.a / (0.5 * dm)) / br.s2 + 4 * kEps * dm;
+    const double blo = b - delta, bhi = b + delta;
+    if (!(blo > 0.5 * b) || !(bhi < b + 0.5 * (1 - b))) { return br; }
+    const double flo = rr_value_exact<NC>(blo, z, lk, e, num);
+    const double fhi = rr_value_exact<NC>(bhi, z, lk, e, num);
+    const double elo = 32.0 * kEps * br.a / smin(blo, 1 - blo);
+    const double ehi = 32.0 * kEps * br.a / smin(bhi, 1 - bhi);
+    if (!(flo > elo) || !(fhi < -ehi)) { return br; }
+    br.ok = true;
+    br.lo = blo;
+    br.hi = bhi;
+    return br;
+}
+
 template <int NC>
 __device__ __forceinline__ double rr_log_sum(const double* z, const double* lk, double sign) {
     double largest = -kInf;
@@ -317,8 +389,8 @@ __device__ __forceinline__ double rr_log_sum(const double* z, const double* lk, 
 // solve_rachford_rice (rachford_rice.hpp:70-147): beta bisection on (0,1) in the
 // overflow-safe log-K form, then explicit roundoff normalisation and balance check.
 template <int NC>
-__device__ __noinline__ int rachford_rice(const double* z, const double* lk, RRState<NC>& r, Counters& cnt,
-                                          bool& rejected) {
+MPMC_HOT int rachford_rice(const double* z, const double* lk, RRState<NC>& r, Counters& cnt,
+                                          bool& rejected, double guess = 0.5) {
     (void)check_composition<NC>(z, rejected);
     double contrast = 0;
     // the log-K split used by rr_value: e_i = exp(-|lk_i|), numerator via expm1
@@ -337,22 +409,33 @@ __device__ __noinline__ int rachford_rice(const double* z, const double* lk, RRS
     const double ls = rr_log_sum<NC>(z, lk, 1.0), rs = rr_log_sum<NC>(z, lk, -1.0);
     if (!finite(ls) || !finite(rs)) { return RR_UNREPRESENTABLE; }
     if (ls <= 64 * kEps || rs <= 64 * kEps) { return RR_NO_INTERIOR; }
+    // Certified signs. With e_i, num_i fixed, the exact F(b) = sum z_i num_i / den_i(b) is
+    // strictly decreasing on (0,1) with |F'| >= S2 = sum z_i num_i^2 (den_i <= 1), and the
+    // reference's computed value differs from F(b) by at most E(b) = 32 eps A / min(b,1-b)
+    // (A = sum z_i |num_i|; each term carries <= 6 roundings, the compensated sum <= 2).
+    // A safeguarded Newton estimate b0 is enclosed by [blo, bhi], checked with two exact
+    // evaluations; a midpoint b outside the enclosure with S2 dist(b, [blo,bhi]) > E(b)
+    // has a computed value of known, nonzero sign and is not evaluated. The visited
+    // midpoints, the decisions and the final beta are those of the plain bisection.
+    RRBracket cert = rr_enclose<NC>(z, lk, e, num, guess);
     double left = 0, right = 1, beta = 0.5, value = 0;
     bool resolved = false;
     for (int it = 0; it < 192; ++it) {
         ++cnt.rr;
         beta = (left + right) / 2;
         if (!(beta > 0 && beta < 1)) { break; }
-        double v = 0, c = 0;
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            if (z[i] == 0) { continue; }
-            const double den = lk[i] >= 0 ? beta + (1 - beta) * e[i] : (1 - beta) + beta * e[i];
-            kadd((z[i] / den) * num[i], v, c);
+        const bool term = beta == left || beta == right || right - left <= 8 * kEps * smin(beta, 1 - beta);
+        if (cert.ok && !term) {
+            const double dm = smin(beta, 1 - beta);
+            if (dm > 1e-200) {
+                const double eb = 32.0 * kEps * cert.a / dm;
+                if (beta < cert.lo && cert.s2 * (cert.lo - beta) > eb) { left = beta; continue; }
+                if (beta > cert.hi && cert.s2 * (beta - cert.hi) > eb) { right = beta; continue; }
+            }
         }
-        value = v;
+        value = rr_value_exact<NC>(beta, z, lk, e, num);
         if (!finite(value)) { return RR_UNREPRESENTABLE; }
-        if (value == 0 || beta == left || beta == right || right - left <= 8 * kEps * smin(beta, 1 - beta)) {
+        if (value == 0 || term) {
             resolved = true;
             break;
         }
@@ -411,10 +494,10 @@ struct SplitState {
 // candidates, fugacity residual, reduced Gibbs energy. Returns -1 on success, an
 // AttemptStatus for RR/budget failures, or A_PROPERTY_FAILURE.
 template <int NC>
-__device__ __noinline__ int split_evaluate(const Mix<NC>& m, double p, double t, const double* z, const double* lk,
+MPMC_HOT int split_evaluate(const Mix<NC>& m, double p, double t, const double* z, const double* lk,
                                            uint32_t& evaluations, uint32_t max_eval, SplitState<NC>& s,
-                                           Counters& cnt, bool& rejected) {
-    const int rs = rachford_rice<NC>(z, lk, s.fr, cnt, rejected);
+                                           Counters& cnt, bool& rejected, double beta_guess = 0.5) {
+    const int rs = rachford_rice<NC>(z, lk, s.fr, cnt, rejected, beta_guess);
     if (rejected) { return A_PROPERTY_FAILURE; }
     if (rs != RR_INTERIOR) {
         return rs == RR_DEGENERATE ? A_INDISTINGUISHABLE : (rs == RR_NO_INTERIOR ? A_NO_INTERIOR : A_RR_FAILURE);
@@ -513,7 +596,7 @@ __device__ __noinline__ int iterate_split(const Mix<NC>& m, double p, double t, 
 #pragma unroll
             for (int i = 0; i < NC; ++i) { nlk[i] = pt.lk[i] + (alpha / scale) * pt.r[i]; }
             SplitState<NC> nx;
-            const int e = split_evaluate<NC>(m, p, t, z, nlk, evaluations, max_eval, nx, cnt, rejected);
+            const int e = split_evaluate<NC>(m, p, t, z, nlk, evaluations, max_eval, nx, cnt, rejected, pt.fr.beta);
             if (rejected) { return A_PROPERTY_FAILURE; }
             if (e == -1) {
                 const double pred = -alpha * slope;

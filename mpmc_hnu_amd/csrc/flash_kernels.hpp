@@ -209,6 +209,35 @@ __global__ __launch_bounds__(64) void phase_kernel(const double* __restrict__ mi
     }
 }
 
+// Batched Rachford-Rice at fixed log K (solve_rachford_rice, rachford_rice.hpp:70-147).
+template <int NC>
+__global__ __launch_bounds__(64) void rr_kernel(int64_t n, const double* __restrict__ z,
+                                                const double* __restrict__ log_k, const double* __restrict__ guess,
+                                                int32_t* status, double* beta, double* x, double* y,
+                                                int32_t* iterations) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n) { return; }
+    double zz[NC], lk[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        zz[i] = z[i * n + c];
+        lk[i] = log_k[i * n + c];
+    }
+    RRState<NC> r;
+    r.beta = __builtin_nan("");
+    Counters cnt{0, 0, 0, 0};
+    bool rejected = false;
+    const int st = rachford_rice<NC>(zz, lk, r, cnt, rejected, guess ? guess[c] : 0.5);
+    status[c] = rejected ? -2 : st;
+    beta[c] = r.beta;
+    iterations[c] = static_cast<int32_t>(cnt.rr);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        x[i * n + c] = st == RR_INTERIOR ? r.x[i] : __builtin_nan("");
+        y[i * n + c] = st == RR_INTERIOR ? r.y[i] : __builtin_nan("");
+    }
+}
+
 } // namespace mpmc::dev
 
 
@@ -236,6 +265,14 @@ hipError_t launch_phase(const double* constants, int64_t ncell, const double* p,
     return hipGetLastError();
 }
 
+template <int NC>
+hipError_t launch_rr(int64_t n, const double* z, const double* lk, const double* guess, int32_t* st, double* beta,
+                     double* x, double* y, int32_t* it, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>((n + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(rr_kernel<NC>, grid, dim3(kBlock), 0, s, n, z, lk, guess, st, beta, x, y, it);
+    return hipGetLastError();
+}
+
 #define MPMC_FOR_EACH_NC(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 #define MPMC_DECLARE_LAUNCH(N)                                                                                   \
@@ -243,11 +280,15 @@ hipError_t launch_phase(const double* constants, int64_t ncell, const double* p,
                                                const double*, int, double, const FlashOut&, hipStream_t);       \
     extern template hipError_t launch_phase<N>(const double*, int64_t, const double*, const double*,            \
                                                const double*, int32_t*, int32_t*, double*, int32_t*, double*,   \
-                                               hipStream_t);
+                                               hipStream_t);                                                    \
+    extern template hipError_t launch_rr<N>(int64_t, const double*, const double*, const double*, int32_t*,     \
+                                            double*, double*, double*, int32_t*, hipStream_t);
 #define MPMC_INSTANTIATE_LAUNCH(N)                                                                               \
     template hipError_t launch_flash<N>(const double*, int64_t, const double*, const double*, const double*,     \
                                         int, double, const FlashOut&, hipStream_t);                              \
     template hipError_t launch_phase<N>(const double*, int64_t, const double*, const double*, const double*,     \
-                                        int32_t*, int32_t*, double*, int32_t*, double*, hipStream_t);
+                                        int32_t*, int32_t*, double*, int32_t*, double*, hipStream_t);    \
+    template hipError_t launch_rr<N>(int64_t, const double*, const double*, const double*, int32_t*, double*,   \
+                                     double*, double*, int32_t*, hipStream_t);
 
 } // namespace mpmc::dev

@@ -39,6 +39,17 @@
 
 namespace mpmc::dev {
 
+// Occupancy hint for the persistent search kernels (waves per SIMD the register
+// allocation must allow); 0 leaves the choice to the compiler.
+#ifndef MPMC_SEARCH_WAVES_PER_EU
+#define MPMC_SEARCH_WAVES_PER_EU 0
+#endif
+#if MPMC_SEARCH_WAVES_PER_EU > 0
+#define MPMC_SEARCH_ATTR __attribute__((amdgpu_waves_per_eu(MPMC_SEARCH_WAVES_PER_EU)))
+#else
+#define MPMC_SEARCH_ATTR
+#endif
+
 constexpr int kAttempts = 16;   // PtSplitOptions::max_split_attempts (pt_split.hpp:49)
 constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
@@ -324,7 +335,7 @@ __device__ __forceinline__ int tpd_check(TpdLane<NC>& L) {
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
+__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
     const int tk = stage == 0 ? NC + 2 : NC + 4;
@@ -562,7 +573,7 @@ __device__ __forceinline__ int split_check(SplitLane<NC>& L, double fug_tol) {
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k_split(Pipe<NC> P, const double* __restrict__ mixc) {
+__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
@@ -586,7 +597,8 @@ __global__ __launch_bounds__(64) void k_split(Pipe<NC> P, const double* __restri
         for (int i = 0; i < NC; ++i) { lk[i] = L.mode == TM_START ? L.lk0[i] : L.pt.lk[i] + (L.alpha / L.scale) * L.pt.r[i]; }
         SplitState<NC> nx;
         bool rejected = false;
-        const int e = split_evaluate<NC>(smix, L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected);
+        const int e = split_evaluate<NC>(smix, L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
+                                         L.mode == TM_START ? 0.5 : L.pt.fr.beta);
         if (rejected) {
             atomicOr(&P.flags[L.cell], CF_REJECTED);
             split_finish<NC>(P, L, A_PROPERTY_FAILURE);

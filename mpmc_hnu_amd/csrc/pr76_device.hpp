@@ -13,6 +13,18 @@
 #include <cfloat>
 #include <cstdint>
 
+// Functions on the per-evaluation hot path are inlined into their kernels by default: a
+// real call saves and restores the caller's registers through scratch on every
+// evaluation. MPMC_HOT_INLINE=0 builds them as calls (smaller code, fewer registers).
+#ifndef MPMC_HOT_INLINE
+#define MPMC_HOT_INLINE 1
+#endif
+#if MPMC_HOT_INLINE
+#define MPMC_HOT __device__ __forceinline__
+#else
+#define MPMC_HOT __device__ __noinline__
+#endif
+
 namespace mpmc::dev {
 
 constexpr double kEps = DBL_EPSILON;
@@ -71,7 +83,7 @@ __device__ __forceinline__ double poly_scale(double y, const Cubic& c) {
 
 // All roots with Z > B of the PR76 cubic in y = (Z-B)/s: stationary-point brackets,
 // bisection to 4 eps relative width (pr76_roots.hpp:77-173).
-__device__ __noinline__ RootSet pr76_roots(double a, double b) {
+MPMC_HOT RootSet pr76_roots(double a, double b) {
     RootSet rs;
     rs.count = 0;
     rs.status = RS_UNREPRESENTABLE;
@@ -183,7 +195,7 @@ __device__ __forceinline__ int check_fractions(const double* w) {
 }
 
 template <int NC>
-__device__ __noinline__ int phase_prepare(const Mix<NC>& m, double p, double t, const double* w, PhaseCtx<NC>& ctx) {
+MPMC_HOT int phase_prepare(const Mix<NC>& m, double p, double t, const double* w, PhaseCtx<NC>& ctx) {
     ctx.rs.status = RS_UNREPRESENTABLE;
     ctx.rs.count = 0;
     if (!finite(p) || !(p > 0) || !finite(t) || !(t > 0)) { return REJECTED; }
@@ -250,7 +262,7 @@ __device__ __forceinline__ double log1p_over_x(double u) {
 
 // ln(phi) of root k (pr76_phase.hpp:193-258, double arithmetic).
 template <int NC>
-__device__ __noinline__ int phase_lnphi(const Mix<NC>& m, double p, double t, const PhaseCtx<NC>& ctx, int k,
+MPMC_HOT int phase_lnphi(const Mix<NC>& m, double p, double t, const PhaseCtx<NC>& ctx, int k,
                                         double* out, double* zout) {
 #pragma unroll
     for (int i = 0; i < NC; ++i) { if (!finite(ctx.S[i])) { return ROOT_RANGE; } }
@@ -293,7 +305,7 @@ struct Activity {
 // Stability provider: the least-Gibbs mechanically admissible root at w, with the
 // near-tie smoothness flag (pr76_stability.hpp:36-119).
 template <int NC>
-__device__ __noinline__ int stability_provider(const Mix<NC>& m, double p, double t, const double* w,
+MPMC_HOT int stability_provider(const Mix<NC>& m, double p, double t, const double* w,
                                                Activity<NC>& out) {
     PhaseCtx<NC> ctx;
     const int pi = phase_prepare<NC>(m, p, t, w, ctx);
@@ -341,7 +353,7 @@ __device__ __noinline__ int stability_provider(const Mix<NC>& m, double p, doubl
 // candidate roles (pr76_split.hpp:188-234) and the three-phase root sides
 // (pr76_three_phase.hpp:39-102).
 template <int NC>
-__device__ __noinline__ int side_provider(const Mix<NC>& m, double p, double t, const double* w, int upper,
+MPMC_HOT int side_provider(const Mix<NC>& m, double p, double t, const double* w, int upper,
                                           Activity<NC>& out, double& zout) {
     PhaseCtx<NC> ctx;
     const int pi = phase_prepare<NC>(m, p, t, w, ctx);

@@ -174,6 +174,31 @@ def phase_batch(mixture: Mixture, p, t, x, stream=None) -> dict:
     return out
 
 
+def rachford_rice_batch(z, log_k, guess=None, stream=None) -> dict:
+    """Rachford-Rice at fixed log K for z[nc, n], log_k[nc, n] (device tensors out)."""
+    torch = _torch()
+    lib = native.library()
+    device = torch.device("cuda", torch.cuda.current_device())
+    z = _dev(z, torch.float64, device)
+    lk = _dev(log_k, torch.float64, device)
+    nc, n = z.shape
+    g = _dev(guess, torch.float64, device) if guess is not None else None
+    out = {
+        "status": torch.empty(n, dtype=torch.int32, device=device),
+        "beta": torch.empty(n, dtype=torch.float64, device=device),
+        "x": torch.empty((nc, n), dtype=torch.float64, device=device),
+        "y": torch.empty((nc, n), dtype=torch.float64, device=device),
+        "iterations": torch.empty(n, dtype=torch.int32, device=device),
+    }
+    V = ctypes.c_void_p
+    native.check(lib.mpmc_rr_batch(nc, n, V(z.data_ptr()), V(lk.data_ptr()), V(g.data_ptr()) if g is not None else None,
+                                   V(out["status"].data_ptr()), V(out["beta"].data_ptr()), V(out["x"].data_ptr()),
+                                   V(out["y"].data_ptr()), V(out["iterations"].data_ptr()), _stream_ptr(stream)),
+                 "rr_batch")
+    out["_inputs"] = (z, lk, g)
+    return out
+
+
 def to_host(res: dict) -> dict:
     """Synchronise and copy a result dict to numpy (counters as uint32)."""
     torch = _torch()

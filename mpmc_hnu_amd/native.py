@@ -6,9 +6,12 @@ no fallback: when it is missing, every entry point raises.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libmpmc_hnu_amd.so"
+# MPMC_HNU_AMD_LIBRARY may name another build of the same library (A/B timing of
+# compile variants); by default the in-tree build is loaded.
+LIB_PATH = Path(os.environ.get("MPMC_HNU_AMD_LIBRARY", Path(__file__).resolve().parent / "libmpmc_hnu_amd.so"))
 
 MAX_COMPONENTS = 12
 
@@ -22,6 +25,7 @@ EXPORTED = (
     "mpmc_flash_batch_ws",
     "mpmc_flash_workspace_bytes",
     "mpmc_phase_batch",
+    "mpmc_rr_batch",
     "mpmc_last_error",
 )
 
@@ -78,6 +82,7 @@ def library() -> ctypes.CDLL:
     lib.mpmc_flash_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64]
     lib.mpmc_flash_workspace_bytes.restype = ctypes.c_size_t
     lib.mpmc_phase_batch.argtypes = [P, ctypes.c_int64, P, P, P, P, P, P, P, P, P]
+    lib.mpmc_rr_batch.argtypes = [ctypes.c_int, ctypes.c_int64] + [P] * 9
     lib.mpmc_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
