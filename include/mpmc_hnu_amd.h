@@ -85,7 +85,12 @@ typedef struct mpmc_flash_options {
     /* 2: solve_pr76_pt_phase_set (VLE, pr76_phase_set.hpp:42-52)
      * 3: Pr76PtFlashBackend::solve (max3, pr76_pt_flash_backend.hpp:190-237) */
     int32_t max_phases;
-    /* 0: every evaluation in FP64. */
+    /* 0: every evaluation in FP64 with the reference's bisection roots.
+     * 1: mixed precision. TPD descents start with mixing, ln(phi) and the TPD in FP32 and
+     *    switch to FP64 before any termination test; cubic brackets, the near-multiple test
+     *    and root selection stay FP64 (roots by safeguarded Newton inside the same
+     *    brackets); Rachford-Rice, splits, Gibbs comparisons and reviews are FP64.
+     *    Staged schedule only. */
     int32_t precision;
     /* 0: staged batch (one lane per stability trial / split attempt, persistent kernels)
      * 1: serial (one lane per cell runs the whole search) */

@@ -163,8 +163,11 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
     if (!std::isfinite(o.fugacity_tolerance) || !(o.fugacity_tolerance > 0)) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: fugacity tolerance must be finite and positive");
     }
-    if (o.precision != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: precision mode not available"); }
+    if (o.precision != 0 && o.precision != 1) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: precision must be 0 or 1"); }
     if (o.schedule != 0 && o.schedule != 1) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: schedule must be 0 or 1"); }
+    if (o.schedule == 1 && o.precision != 0) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: the serial schedule runs FP64 only");
+    }
     if (ncell == 0) { return MPMC_OK; }
     const void* ptrs[] = {p, t, z, out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
                           out->ln_phi};
@@ -221,7 +224,7 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
 #define MPMC_CASE(N)                                                                                             \
     case N:                                                                                                      \
         e = mpmc::dev::launch_pipeline<N>(m->dev_constants, workspace, workspace_bytes, ncell, p, t, z,          \
-                                          o.max_phases, o.fugacity_tolerance, fo, s);                            \
+                                          o.max_phases, o.fugacity_tolerance, o.precision, fo, s);               \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE

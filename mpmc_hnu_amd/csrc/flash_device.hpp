@@ -116,11 +116,47 @@ MPMC_HOT int tpd_point(const double* w, const double* z, const Activity<NC>& tri
 
 __device__ __forceinline__ bool tpd_negative(double value, double guard, double tol) { return value < -tol - guard; }
 
+// Tangent-plane distance from an FP32 branch evaluation (mixed-precision descent only):
+// logarithms in FP32, sums in FP64, and a roundoff guard at FP32 resolution.
+template <int NC>
+__device__ __forceinline__ void tpd_point_f32(const double* w, const double* z, const float* lnphi, int branch,
+                                              const Activity<NC>& ref, TpdPoint<NC>& r) {
+    double sum = 0, value = 0, mag = 1;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { sum += w[i]; }
+    r.branch = branch;
+    r.smooth = true;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        r.w[i] = w[i];
+        r.res[i] = 0;
+        if (z[i] == 0 || w[i] == 0) { continue; }
+        const float wf = static_cast<float>(w[i]), zf = static_cast<float>(z[i]);
+        const float lw = logf(wf), lz = logf(zf);
+        const float lr = __builtin_fabsf(wf - zf) < 0.5f * zf ? log1pf((wf - zf) / zf) : lw - lz;
+        const float q = lr + (lnphi[i] - static_cast<float>(ref.lnphi[i]));
+        r.res[i] = q;
+        value += w[i] * static_cast<double>(q);
+        mag += w[i] * static_cast<double>(__builtin_fabsf(lw) + __builtin_fabsf(lz) + __builtin_fabsf(lnphi[i]) +
+                                          __builtin_fabsf(static_cast<float>(ref.lnphi[i])));
+    }
+    r.value = value;
+    const double mean = value / sum;
+    r.stat = 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        if (z[i] == 0) { continue; }
+        r.res[i] = w[i] == 0 ? -kInf : r.res[i] - mean;
+        r.stat = smax(r.stat, __builtin_fabs(r.res[i]));
+    }
+    r.guard = 64.0 * 1.1920928955078125e-07 * mag;
+}
+
 // Multistart TPD search (detail::test_pt_stability_impl, pt_stability.hpp:383-616). With
 // imposed == nullptr the reference phase is the least-Gibbs feed phase; otherwise the
 // imposed common log-activities define it (test_pt_stability_against, :637-644).
 // Negative-TPD witnesses are kept, in trial order, for the split and max3 seeds.
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ void stability(const Mix<NC>& m, double p, double t, const double* feed_in,
                                        const double* imposed, const double* const* extra, int nextra,
                                        double tpd_tol, StabOut<NC>& R, Counters& cnt, bool& rejected) {
@@ -160,7 +196,7 @@ __device__ __noinline__ void stability(const Mix<NC>& m, double p, double t, con
     Activity<NC> imp;
     if (!imposed) {
         ++evaluations; ++cnt.prop;
-        const int e = stability_provider<NC>(m, p, t, R.feed, R.ref);
+        const int e = stability_provider<NC, kN>(m, p, t, R.feed, R.ref);
         if (e == REJECTED) { rejected = true; return; }
         if (e != ISSUE_NONE) { return; } // every trial: property_failure
         R.has_ref = true;
@@ -214,7 +250,7 @@ __device__ __noinline__ void stability(const Mix<NC>& m, double p, double t, con
         {
             ++evaluations; ++cnt.prop; ++cnt.tpd;
             Activity<NC> ph;
-            int e = stability_provider<NC>(m, p, t, w0, ph);
+            int e = stability_provider<NC, kN>(m, p, t, w0, ph);
             if (e == ISSUE_NONE) { e = tpd_point<NC>(w0, R.feed, ph, dref, pt, rejected); }
             if (e == REJECTED || rejected) { rejected = true; return; }
             if (e != ISSUE_NONE) { all_stat = false; continue; }
@@ -265,7 +301,7 @@ __device__ __noinline__ void stability(const Mix<NC>& m, double p, double t, con
                 TpdPoint<NC> nx;
                 ++evaluations; ++cnt.prop; ++cnt.tpd;
                 Activity<NC> ph;
-                int e = stability_provider<NC>(m, p, t, cand, ph);
+                int e = stability_provider<NC, kN>(m, p, t, cand, ph);
                 if (e == ISSUE_NONE) { e = tpd_point<NC>(cand, R.feed, ph, dref, nx, rejected); }
                 if (e == REJECTED || rejected) { rejected = true; return; }
                 if (e != ISSUE_NONE) { continue; }
@@ -493,7 +529,7 @@ struct SplitState {
 // evaluate() of iterate_pt_split (pt_split.hpp:300-341): RR, liquid and vapor
 // candidates, fugacity residual, reduced Gibbs energy. Returns -1 on success, an
 // AttemptStatus for RR/budget failures, or A_PROPERTY_FAILURE.
-template <int NC>
+template <int NC, bool kN = false>
 MPMC_HOT int split_evaluate(const Mix<NC>& m, double p, double t, const double* z, const double* lk,
                                            uint32_t& evaluations, uint32_t max_eval, SplitState<NC>& s,
                                            Counters& cnt, bool& rejected, double beta_guess = 0.5) {
@@ -505,8 +541,8 @@ MPMC_HOT int split_evaluate(const Mix<NC>& m, double p, double t, const double* 
     for (int role = 0; role < 2; ++role) {
         if (evaluations >= max_eval) { return A_EVALUATION_LIMIT; }
         ++evaluations; ++cnt.prop;
-        const int e = role == 0 ? side_provider<NC>(m, p, t, s.fr.x, 0, s.liq, s.zl)
-                                : side_provider<NC>(m, p, t, s.fr.y, 1, s.vap, s.zv);
+        const int e = role == 0 ? side_provider<NC, kN>(m, p, t, s.fr.x, 0, s.liq, s.zl)
+                                : side_provider<NC, kN>(m, p, t, s.fr.y, 1, s.vap, s.zv);
         if (e == REJECTED) { rejected = true; return A_PROPERTY_FAILURE; }
         if (e != ISSUE_NONE) { return A_PROPERTY_FAILURE; }
     }
@@ -539,7 +575,7 @@ MPMC_HOT int split_evaluate(const Mix<NC>& m, double p, double t, const double* 
 
 // iterate_pt_split (pt_split.hpp:274-408): logK <- logK + (alpha/scale) r, accepted by the
 // Gibbs Armijo rule with the closed-form slope, else by residual progress.
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ int iterate_split(const Mix<NC>& m, double p, double t, const double* z, const double* lk0,
                                           double fug_tol, uint32_t max_eval, uint32_t& evaluations,
                                           SplitState<NC>& pt, Counters& cnt, bool& rejected) {
@@ -549,7 +585,7 @@ __device__ __noinline__ int iterate_split(const Mix<NC>& m, double p, double t, 
     (void)check_composition<NC>(z, rejected);
     if (rejected) { return A_PROPERTY_FAILURE; }
     {
-        const int e = split_evaluate<NC>(m, p, t, z, lk0, evaluations, max_eval, pt, cnt, rejected);
+        const int e = split_evaluate<NC, kN>(m, p, t, z, lk0, evaluations, max_eval, pt, cnt, rejected);
         if (e != -1) { return e; }
     }
     int iterations = 0;
@@ -596,7 +632,7 @@ __device__ __noinline__ int iterate_split(const Mix<NC>& m, double p, double t, 
 #pragma unroll
             for (int i = 0; i < NC; ++i) { nlk[i] = pt.lk[i] + (alpha / scale) * pt.r[i]; }
             SplitState<NC> nx;
-            const int e = split_evaluate<NC>(m, p, t, z, nlk, evaluations, max_eval, nx, cnt, rejected, pt.fr.beta);
+            const int e = split_evaluate<NC, kN>(m, p, t, z, nlk, evaluations, max_eval, nx, cnt, rejected, pt.fr.beta);
             if (rejected) { return A_PROPERTY_FAILURE; }
             if (e == -1) {
                 const double pred = -alpha * slope;
@@ -660,7 +696,7 @@ struct VleOut {
 };
 
 // solve_pt_vle (pt_split.hpp:414-525) with the PR76 evaluator as both providers.
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
                                        const double* const* starts, int nstarts, VleOut<NC>& V, Counters& cnt,
                                        bool& rejected) {
@@ -677,7 +713,7 @@ __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, con
         for (int i = 0; i < NC; ++i) { if ((feed[i] > 0) != (starts[e][i] > 0)) { rejected = true; } }
     }
     if (rejected) { return; }
-    stability<NC>(m, p, t, feed, nullptr, starts, nstarts, 1e-10, V.init, cnt, rejected);
+    stability<NC, kN>(m, p, t, feed, nullptr, starts, nstarts, 1e-10, V.init, cnt, rejected);
     if (rejected) { return; }
     if (V.init.status == S_NO_INSTABILITY) { V.status = V_SINGLE; return; }
     if (V.init.status != S_UNSTABLE) { return; }
@@ -693,7 +729,7 @@ __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, con
             uint32_t evals = 0;
             SplitState<NC> st;
             if (split_seed<NC>(z, V.init.wit[k].w, side == 0, lk)) {
-                status = iterate_split<NC>(m, p, t, z, lk, fug_tol, kMaxSplitEval - split_evals, evals, st, cnt,
+                status = iterate_split<NC, kN>(m, p, t, z, lk, fug_tol, kMaxSplitEval - split_evals, evals, st, cnt,
                                            rejected);
                 if (rejected) { return; }
             }
@@ -725,7 +761,7 @@ __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, con
     for (int e = 0; e < nstarts; ++e) { fs[nf++] = starts[e]; }
     fs[nf++] = c.fr.x;
     fs[nf++] = c.fr.y;
-    stability<NC>(m, p, t, z, c.common, fs, nf, ftol, V.fin, cnt, rejected);
+    stability<NC, kN>(m, p, t, z, c.common, fs, nf, ftol, V.fin, cnt, rejected);
     if (rejected) { return; }
     V.has_final = true;
     if (V.fin.status == S_UNSTABLE) { V.status = V_UNSTABLE; }
@@ -868,7 +904,7 @@ __device__ __forceinline__ double log_distance(const double* a, const double* b,
 }
 
 // evaluate_state of iterate_pt_three_phase (pt_three_phase.hpp:463-555)
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ int p3_evaluate(const Mix<NC>& m, double p, double t, const double* feed, const double* l1,
                                         const double* l2, double s1, double s2, const int* sides,
                                         uint32_t& evaluations, P3State<NC>& s, Counters& cnt, bool& rejected) {
@@ -888,7 +924,7 @@ __device__ __noinline__ int p3_evaluate(const Mix<NC>& m, double p, double t, co
     for (int ph = 0; ph < 3; ++ph) {
         if (evaluations >= kMaxEval) { return P_EVALUATION_LIMIT; }
         ++evaluations; ++cnt.prop;
-        const int e = side_provider<NC>(m, p, t, s.comp[ph], sides[ph], s.act[ph], s.z[ph]);
+        const int e = side_provider<NC, kN>(m, p, t, s.comp[ph], sides[ph], s.act[ph], s.z[ph]);
         if (e == REJECTED) { rejected = true; return P_PROPERTY; }
         if (e != ISSUE_NONE) { return P_PROPERTY; }
         if (!finite(s.z[ph]) || !(s.z[ph] > 0)) { return P_PROPERTY; }
@@ -921,7 +957,7 @@ __device__ __noinline__ int p3_evaluate(const Mix<NC>& m, double p, double t, co
 }
 
 // iterate_pt_three_phase (pt_three_phase.hpp:402-674)
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ int iterate_three(const Mix<NC>& m, double p, double t, const double* feed_in,
                                           const double* l1_0, const double* l2_0, double b1, double b2,
                                           const int* sides, P3State<NC>& pt, int& disappearing, Counters& cnt,
@@ -943,7 +979,7 @@ __device__ __noinline__ int iterate_three(const Mix<NC>& m, double p, double t, 
     if (lost) { return P_BAL_FAILURE; }
     uint32_t evaluations = 0;
     {
-        const int e = p3_evaluate<NC>(m, p, t, feed, l1_0, l2_0, b1, b2, sides, evaluations, pt, cnt, rejected);
+        const int e = p3_evaluate<NC, kN>(m, p, t, feed, l1_0, l2_0, b1, b2, sides, evaluations, pt, cnt, rejected);
         if (e != -1) { return e; }
     }
     int iterations = 0;
@@ -971,7 +1007,7 @@ __device__ __noinline__ int iterate_three(const Mix<NC>& m, double p, double t, 
                 n2[i] = pt.lk2[i] + (alpha / scale) * pt.r2[i];
             }
             P3State<NC> nx;
-            const int e = p3_evaluate<NC>(m, p, t, feed, n1, n2, pt.frac[1], pt.frac[2], sides, evaluations, nx, cnt,
+            const int e = p3_evaluate<NC, kN>(m, p, t, feed, n1, n2, pt.frac[1], pt.frac[2], sides, evaluations, nx, cnt,
                                           rejected);
             if (rejected) { return P_PROPERTY; }
             if (e == -1) {
@@ -989,10 +1025,10 @@ __device__ __noinline__ int iterate_three(const Mix<NC>& m, double p, double t, 
 }
 
 // pr76_witness_root_sides (pr76_three_phase.hpp:273-305); bit 0 = lower, bit 1 = upper.
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ int witness_sides(const Mix<NC>& m, double p, double t, const double* w, int branch) {
     PhaseCtx<NC> ctx;
-    if (phase_prepare<NC>(m, p, t, w, ctx) != ISSUE_NONE) { return 0; }
+    if (phase_prepare<NC, kN>(m, p, t, w, ctx) != ISSUE_NONE) { return 0; }
     int adm[3], na = 0;
     for (int k = 0; k < ctx.rs.count; ++k) { if (ctx.rs.r[k].slope_sign > 0) { adm[na++] = k; } }
     bool found = false;
@@ -1036,7 +1072,7 @@ struct Max3Tail {
 // each followed by the three-phase final review or, on disappearance, a fresh
 // two-phase neighbour solve. `witness(i, w, branch)` yields the i-th negative witness
 // of the review in trial order. No caller continuation hints.
-template <int NC, typename WitnessFn>
+template <int NC, bool kN, typename WitnessFn>
 __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, const double* z, const double* px,
                                        const double* py, double pbeta, int nwit, WitnessFn witness, double fug_tol,
                                        Max3Tail<NC>& M, Counters& cnt, bool& rejected) {
@@ -1052,7 +1088,7 @@ __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, con
             witness(k, ww, wbranch);
             double l2[NC];
             if (!log_ratio<NC>(ww, px, z, l2)) { continue; }
-            const int sides = witness_sides<NC>(m, p, t, ww, wbranch);
+            const int sides = witness_sides<NC, kN>(m, p, t, ww, wbranch);
             for (int s = 0; s < 2; ++s) {
                 if (!(sides & (1 << s))) { continue; }
                 if (automatic >= 16) { auto_limit = true; break; }
@@ -1060,7 +1096,7 @@ __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, con
                 const int sd[3] = {0, 1, s};
                 int disappearing;
                 P3State<NC> E;
-                const int st = iterate_three<NC>(m, p, t, z, src1, l2, (1.0 - 0.1) * pbeta, 0.1, sd, E, disappearing,
+                const int st = iterate_three<NC, kN>(m, p, t, z, src1, l2, (1.0 - 0.1) * pbeta, 0.1, sd, E, disappearing,
                                                  cnt, rejected);
                 if (rejected) { return; }
                 if (st == P_DISAPPEARANCE) {
@@ -1070,7 +1106,7 @@ __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, con
                         int ns = 0;
                         for (int ph = 0; ph < 3; ++ph) { if (ph != disappearing) { stv[ns++] = E.comp[ph]; } }
                         VleOut<NC> nb;
-                        solve_vle<NC>(m, p, t, z, fug_tol, stv, 2, nb, cnt, rejected);
+                        solve_vle<NC, kN>(m, p, t, z, fug_tol, stv, 2, nb, cnt, rejected);
                         if (rejected) { return; }
                         if (nb.status == V_TWO) {
                             M.status = M_TWO;
@@ -1085,7 +1121,7 @@ __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, con
                 if (!(E.norm <= 1e-11 && E.rr_res <= 2e-13 && E.mass_abs <= 1e-12 && E.mass_rel <= 1e-10)) { continue; }
                 const double* stv[3] = {E.comp[0], E.comp[1], E.comp[2]};
                 StabOut<NC> F;
-                stability<NC>(m, p, t, z, E.common, stv, 3, 1e-10 + E.norm, F, cnt, rejected);
+                stability<NC, kN>(m, p, t, z, E.common, stv, 3, 1e-10 + E.norm, F, cnt, rejected);
                 if (rejected) { return; }
                 if (F.status == S_UNSTABLE) { saw_higher = true; }
                 else if (F.status == S_NO_INSTABILITY) { M.status = M_THREE; M.three = E; return; }
@@ -1104,13 +1140,13 @@ struct Max3Out {
 
 // solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), without caller continuation hints,
 // serial in one lane.
-template <int NC>
+template <int NC, bool kN = false>
 __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
                                         Max3Out<NC>& M, Counters& cnt, bool& rejected) {
     M.tail.use_neighbor = false;
     (void)check_composition<NC>(feed, rejected);
     if (rejected) { return; }
-    solve_vle<NC>(m, p, t, feed, fug_tol, nullptr, 0, M.base, cnt, rejected);
+    solve_vle<NC, kN>(m, p, t, feed, fug_tol, nullptr, 0, M.base, cnt, rejected);
     if (rejected) { return; }
     switch (M.base.status) {
     case V_SINGLE: M.status = M_SINGLE; return;
@@ -1120,7 +1156,7 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
     }
     if (!M.base.has_candidate || !M.base.has_final) { M.status = M_INDETERMINATE; return; }
     const StabOut<NC>& fin = M.base.fin;
-    max3_tail<NC>(m, p, t, M.base.init.feed, M.base.cand.fr.x, M.base.cand.fr.y, M.base.cand.fr.beta, fin.nwit,
+    max3_tail<NC, kN>(m, p, t, M.base.init.feed, M.base.cand.fr.x, M.base.cand.fr.y, M.base.cand.fr.beta, fin.nwit,
                   [&](int i, double* w, int& branch) {
 #pragma unroll
                       for (int c = 0; c < NC; ++c) { w[c] = fin.wit[i].w[c]; }

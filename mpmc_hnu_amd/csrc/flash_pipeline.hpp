@@ -55,7 +55,11 @@ constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
 enum CellFlag : int32_t { CF_REJECTED = 1, CF_FALLBACK = 2 };
 enum GlobalSlot : int { G_NU = 0, G_NFALLBACK = 1, G_TASK_INIT = 2, G_TASK_SPLIT = 3, G_TASK_REVIEW = 4,
-                        G_TASK_MAX3 = 5, G_TASK_FALLBACK = 6, G_SLOTS = 16 };
+                        G_TASK_MAX3 = 5, G_TASK_FALLBACK = 6, G_TASK_INIT64 = 7, G_TASK_REVIEW64 = 8, G_SLOTS = 16 };
+
+// Trial slot status of a descent parked by the FP32 phase (mixed precision); the FP64
+// continuation resumes it from the stored point. Iteration count in bits 16..
+constexpr int32_t kParked = 0xFE;
 
 // Device workspace carved from one allocation (pipe_carve). n = cells; arrays
 // indexed by the compact unstable index u use capacity n.
@@ -149,7 +153,7 @@ static __global__ void k_reset(unsigned long long* g) {
     if (threadIdx.x < G_SLOTS) { g[threadIdx.x] = 0; }
 }
 
-template <int NC>
+template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restrict__ mixc) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
         Activity<NC> ref;
         if (!rejected && !lost) {
             ++prop;
-            const int e = stability_provider<NC>(smix, p, t, feed, ref);
+            const int e = stability_provider<NC, kN>(smix, p, t, feed, ref);
             if (e == REJECTED) { rejected = true; }
             else if (e == ISSUE_NONE) { info |= ref.branch | (ref.smooth ? 0x100 : 0) | 0x200; }
         }
@@ -213,14 +217,18 @@ struct TpdLane {
     TpdPoint<NC> pt;
     double dir[NC], lw[NC], cand[NC];
     double slope, scale, alpha;
+    uint32_t evals0;   // evaluations already counted by the FP32 phase
+    CellF32<NC> cf;
 };
 
 // stage 0: initial feed test; stage 1: common-tangent review of the selected pair.
-template <int NC>
-__device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, TpdLane<NC>& L) {
+// kResume: FP64 continuation of a descent parked by the FP32 phase (mixed precision).
+template <int NC, bool kF32, bool kResume>
+__device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>& smix, int stage, int k, int64_t slot, TpdLane<NC>& L) {
     const int64_t n = P.n;
     int64_t c;
     int32_t* st_arr = stage == 0 ? P.tst : P.fst;
+    if (kResume && (st_arr[k * n + slot] & 0xFF) != kParked) { return false; }
     if (stage == 0) {
         c = slot;
         if (P.flags[c] & CF_REJECTED) { return false; }
@@ -266,6 +274,16 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, Tpd
         L.ref.smooth = true;
         if (!fin) { st_arr[k * n + slot] = T_PROPERTY_FAILURE; P.fev[k * n + slot] = 0; return false; }
     }
+    if (kResume) {
+        const double* w = stage == 0 ? P.tw : P.fw;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { L.cand[i] = w[(k * NC + i) * n + slot]; }
+        L.it = st_arr[k * n + slot] >> 16;
+        L.evals = (stage == 0 ? P.tev : P.fev)[k * n + slot];
+        L.evals0 = L.evals;
+        L.mode = TM_START;
+        return true;
+    }
     if (k < gen) {
         trial_start<NC>(L.z, active, k, L.cand);
     } else {
@@ -282,8 +300,10 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, Tpd
         lost = lost || (L.z[i] > 0 && L.cand[i] == 0);
     }
     L.evals = 0;
+    L.evals0 = 0;
     L.it = 0;
     L.mode = TM_START;
+    if (kF32) { cell_f32<NC>(smix, L.t, L.cf); }
     if (lost) {
         st_arr[k * n + slot] = T_UNREPRESENTABLE;
         (stage == 0 ? P.tev : P.fev)[k * n + slot] = 0;
@@ -303,6 +323,20 @@ __device__ void tpd_finish(const Pipe<NC>& P, int stage, TpdLane<NC>& L, int sta
 #pragma unroll
         for (int i = 0; i < NC; ++i) { w[(L.k * NC + i) * n + L.slot] = L.pt.w[i]; }
     }
+    atomicAdd(&P.cnt[0 * n + L.cell], L.evals - L.evals0);
+    atomicAdd(&P.cnt[1 * n + L.cell], L.evals - L.evals0);
+}
+
+// Park an FP32-phase descent at its current point for the FP64 continuation.
+template <int NC>
+__device__ void tpd_park(const Pipe<NC>& P, int stage, TpdLane<NC>& L, const double* w) {
+    const int64_t n = P.n;
+    const int64_t idx = L.k * n + L.slot;
+    (stage == 0 ? P.tst : P.fst)[idx] = kParked | (L.it << 16);
+    (stage == 0 ? P.tev : P.fev)[idx] = L.evals;
+    double* dst = stage == 0 ? P.tw : P.fw;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { dst[(L.k * NC + i) * n + L.slot] = w[i]; }
     atomicAdd(&P.cnt[0 * n + L.cell], L.evals);
     atomicAdd(&P.cnt[1 * n + L.cell], L.evals);
 }
@@ -334,14 +368,29 @@ __device__ __forceinline__ int tpd_check(TpdLane<NC>& L) {
     return -1;
 }
 
+// Mixed precision: leave the FP32 phase near any termination test of the reference
+// (TPD clearly negative, stationarity within 1e-4, a nonsmooth branch, a long search or
+// a failed FP32 line search) and re-evaluate the current point in FP64, after which the
+// search continues under the FP64 rules above.
+constexpr int kSwitch64 = -2;
 template <int NC>
+__device__ __forceinline__ int tpd_check_f32(TpdLane<NC>& L) {
+    if (L.pt.value < -1e-3 || L.pt.stat <= 1e-4 || !L.pt.smooth || L.it >= 256) { return kSwitch64; }
+    const int r = tpd_check<NC>(L);
+    return r == -1 ? -1 : kSwitch64;
+}
+
+// kF32: the FP32 descent phase of mixed precision (parks every search for kResume);
+// kResume: its FP64 continuation; neither: the FP64 search from the trial start.
+template <int NC, bool kN, bool kF32, bool kResume>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
     const int tk = stage == 0 ? NC + 2 : NC + 4;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
-    unsigned long long* counter = &P.g[stage == 0 ? G_TASK_INIT : G_TASK_REVIEW];
+    unsigned long long* counter =
+        &P.g[stage == 0 ? (kResume ? G_TASK_INIT64 : G_TASK_INIT) : (kResume ? G_TASK_REVIEW64 : G_TASK_REVIEW)];
     TpdLane<NC> L;
     bool active = false, exhausted = false;
     for (;;) {
@@ -349,7 +398,9 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         const int64_t task = fetch_task(counter, want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else { active = tpd_begin<NC>(P, stage, static_cast<int>(task / nslot), task % nslot, L); }
+            else {
+                active = tpd_begin<NC, kF32, kResume>(P, smix, stage, static_cast<int>(task / nslot), task % nslot, L);
+            }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
@@ -381,14 +432,29 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         TpdPoint<NC> nx;
         int e = ISSUE_NONE;
         bool rejected = false;
+        constexpr bool f32 = kF32;
         if (do_eval) {
             ++L.evals;
-            Activity<NC> ph;
-            e = stability_provider<NC>(smix, L.p, L.t, L.cand, ph);
-            if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
+            if (f32) {
+                float lp[NC];
+                int br = 0;
+                e = stability_provider_f32<NC>(smix, L.cf, L.p, L.t, L.cand, lp, br);
+                if (e == ISSUE_NONE) { tpd_point_f32<NC>(L.cand, L.z, lp, br, L.ref, nx); }
+            } else {
+                Activity<NC> ph;
+                e = stability_provider<NC, kN>(smix, L.p, L.t, L.cand, ph);
+                if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
+            }
             if (e == REJECTED || rejected) {
                 atomicOr(&P.flags[L.cell], CF_REJECTED);
                 tpd_finish<NC>(P, stage, L, T_PROPERTY_FAILURE);
+                active = false;
+                continue;
+            }
+            if (f32 && e != ISSUE_NONE) {
+                // FP32 could not decide this evaluation: the FP64 continuation redoes the
+                // start point, or resumes from the current accepted point
+                tpd_park<NC>(P, stage, L, L.mode == TM_BT ? L.pt.w : L.cand);
                 active = false;
                 continue;
             }
@@ -396,7 +462,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         int done = -1;
         if (L.mode == TM_START) {
             if (e != ISSUE_NONE) { done = T_PROPERTY_FAILURE; }
-            else { L.pt = nx; done = tpd_check<NC>(L); }
+            else { L.pt = nx; done = f32 ? tpd_check_f32<NC>(L) : tpd_check<NC>(L); }
         } else {
             bool ok = false;
             if (do_eval && e == ISSUE_NONE) {
@@ -412,12 +478,17 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
             if (ok) {
                 L.pt = nx;
                 ++L.it;
-                done = tpd_check<NC>(L);
+                done = f32 ? tpd_check_f32<NC>(L) : tpd_check<NC>(L);
             } else {
                 ++L.bt;
                 L.alpha *= 0.5;
-                if (L.bt >= 32) { done = T_LINE_SEARCH_FAILED; }
+                if (L.bt >= 32) { done = f32 ? kSwitch64 : T_LINE_SEARCH_FAILED; }
             }
+        }
+        if (done == kSwitch64) {
+            tpd_park<NC>(P, stage, L, L.pt.w);
+            active = false;
+            continue;
         }
         if (done >= 0) {
             tpd_finish<NC>(P, stage, L, done);
@@ -572,7 +643,7 @@ __device__ __forceinline__ int split_check(SplitLane<NC>& L, double fug_tol) {
     return -1;
 }
 
-template <int NC>
+template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
@@ -597,7 +668,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const
         for (int i = 0; i < NC; ++i) { lk[i] = L.mode == TM_START ? L.lk0[i] : L.pt.lk[i] + (L.alpha / L.scale) * L.pt.r[i]; }
         SplitState<NC> nx;
         bool rejected = false;
-        const int e = split_evaluate<NC>(smix, L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
+        const int e = split_evaluate<NC, kN>(smix, L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
                                          L.mode == TM_START ? 0.5 : L.pt.fr.beta);
         if (rejected) {
             atomicOr(&P.flags[L.cell], CF_REJECTED);
@@ -721,7 +792,7 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
 }
 
 // ------------------------------------------------------------------ max3 tail
-template <int NC>
+template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __restrict__ mixc) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
@@ -750,7 +821,7 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
         Counters cnt{0, 0, 0, 0};
         bool rejected = false;
         Max3Tail<NC> M;
-        max3_tail<NC>(smix, P.p[c], P.t[c], z, x, y, beta, nwit,
+        max3_tail<NC, kN>(smix, P.p[c], P.t[c], z, x, y, beta, nwit,
                       [&](int i, double* w, int& branch) {
                           int seen = -1, k = 0;
                           for (; k < ntr; ++k) {
@@ -918,7 +989,7 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
 }
 
 // Cells whose searches a shared evaluation budget would have cut: the serial path.
-template <int NC>
+template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __restrict__ mixc) {
     __shared__ Mix<NC> smix;
     stage_mix<NC>(mixc, smix);
@@ -937,12 +1008,12 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
         PhaseView<NC> ph[3];
         if (P.max_phases == 2) {
             VleOut<NC> V;
-            solve_vle<NC>(smix, P.p[c], P.t[c], feed, P.fug_tol, nullptr, 0, V, cnt, rejected);
+            solve_vle<NC, kN>(smix, P.p[c], P.t[c], feed, P.fug_tol, nullptr, 0, V, cnt, rejected);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_vle<NC>(P.out, n, c, V, 0xF, cnt); }
         } else {
             Max3Out<NC> M;
-            solve_max3<NC>(smix, P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected);
+            solve_max3<NC, kN>(smix, P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_max3<NC>(P.out, n, c, M, cnt); }
         }
@@ -1014,10 +1085,42 @@ inline unsigned persistent_blocks(K kernel) {
     return static_cast<unsigned>(cus * per_cu);
 }
 
+template <int NC, bool kN>
+hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, hipStream_t s) {
+    static const unsigned b_tpd = persistent_blocks(k_tpd<NC, kN, false, kN>);
+    static const unsigned b_split = persistent_blocks(k_split<NC, kN>);
+    static const unsigned b_tail = persistent_blocks(k_max3_tail<NC, kN>);
+    static const unsigned b_fb = persistent_blocks(k_fallback<NC, kN>);
+    const int64_t n = P.n;
+    const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
+    hipLaunchKernelGGL((k_prep<NC, kN>), dim3(cell_blocks), dim3(64), 0, s, P, mixc);
+    if constexpr (kN) {
+        static const unsigned b_tpd32 = persistent_blocks(k_tpd<NC, true, true, false>);
+        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b_tpd32), dim3(64), 0, s, P, mixc, 0);
+    }
+    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b_tpd), dim3(64), 0, s, P, mixc, 0);
+    hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_split<NC, kN>), dim3(b_split), dim3(64), 0, s, P, mixc);
+    hipLaunchKernelGGL(k_split_select<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    if constexpr (kN) {
+        static const unsigned b_tpd32 = persistent_blocks(k_tpd<NC, true, true, false>);
+        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b_tpd32), dim3(64), 0, s, P, mixc, 1);
+    }
+    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b_tpd), dim3(64), 0, s, P, mixc, 1);
+    hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, kN>), dim3(b_tail), dim3(64), 0, s, P, mixc); }
+    hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_fallback<NC, kN>), dim3(b_fb), dim3(64), 0, s, P, mixc);
+    return hipGetLastError();
+}
+
+// precision 0: FP64 evaluations with the reference's bisection roots (bit-level
+// parity); 1: mixed precision (FP32 descent phase, Newton roots, FP64 decisions).
 template <int NC>
 hipError_t launch_pipeline(const double* mixc, void* workspace, size_t workspace_bytes, int64_t n, const double* p,
-                           const double* t, const double* z, int max_phases, double fug_tol, const FlashOut& out,
-                           hipStream_t s) {
+                           const double* t, const double* z, int max_phases, double fug_tol, int precision,
+                           const FlashOut& out, hipStream_t s) {
     Pipe<NC> P{};
     const size_t need = pipe_carve<NC>(P, static_cast<char*>(workspace), n);
     if (!workspace || need > workspace_bytes) { return hipErrorInvalidValue; }
@@ -1028,33 +1131,18 @@ hipError_t launch_pipeline(const double* mixc, void* workspace, size_t workspace
     P.t = t;
     P.zin = z;
     P.out = out;
-    static const unsigned b_tpd = persistent_blocks(k_tpd<NC>);
-    static const unsigned b_split = persistent_blocks(k_split<NC>);
-    static const unsigned b_tail = persistent_blocks(k_max3_tail<NC>);
-    static const unsigned b_fb = persistent_blocks(k_fallback<NC>);
-    const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
-    hipLaunchKernelGGL(k_prep<NC>, dim3(cell_blocks), dim3(64), 0, s, P, mixc);
-    hipLaunchKernelGGL(k_tpd<NC>, dim3(b_tpd), dim3(64), 0, s, P, mixc, 0);
-    hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL(k_split<NC>, dim3(b_split), dim3(64), 0, s, P, mixc);
-    hipLaunchKernelGGL(k_split_select<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL(k_tpd<NC>, dim3(b_tpd), dim3(64), 0, s, P, mixc, 1);
-    hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    if (max_phases == 3) { hipLaunchKernelGGL(k_max3_tail<NC>, dim3(b_tail), dim3(64), 0, s, P, mixc); }
-    hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL(k_fallback<NC>, dim3(b_fb), dim3(64), 0, s, P, mixc);
-    return hipGetLastError();
+    return precision == 1 ? launch_pipeline_prec<NC, true>(P, mixc, max_phases, s)
+                          : launch_pipeline_prec<NC, false>(P, mixc, max_phases, s);
 }
 
 #define MPMC_DECLARE_PIPELINE(N)                                                                                  \
     extern template size_t pipeline_bytes<N>(int64_t);                                                            \
     extern template hipError_t launch_pipeline<N>(const double*, void*, size_t, int64_t, const double*,           \
-                                                  const double*, const double*, int, double, const FlashOut&,     \
+                                                  const double*, const double*, int, double, int, const FlashOut&,\
                                                   hipStream_t);
 #define MPMC_INSTANTIATE_PIPELINE(N)                                                                              \
     template size_t pipeline_bytes<N>(int64_t);                                                                   \
     template hipError_t launch_pipeline<N>(const double*, void*, size_t, int64_t, const double*, const double*,   \
-                                           const double*, int, double, const FlashOut&, hipStream_t);
+                                           const double*, int, double, int, const FlashOut&, hipStream_t);
 
 } // namespace mpmc::dev
