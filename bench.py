@@ -116,10 +116,29 @@ def main(argv=None):
         host = flash.to_host(out)
         print(f"[bench] precision={precision}: {gpu_ms:.3f} ms/flash (events), wall {wall * 1e3 / a.steps:.3f} ms",
               file=sys.stderr, flush=True)
-        return gpu_ms, wall * 1e3 / a.steps, host
+        return gpu_ms, wall * 1e3 / a.steps, host, out
 
-    fp64_ms, fp64_wall, h64 = run(0)
-    mixed_ms, mixed_wall, hmx = run(1)
+    def run_closure(fl):
+        """Dual closure of every two-phase cell of a flash result (the same cells)."""
+        res = flash.closure_batch(m, pd, td, zd, fl, stream=stream)
+        torch.cuda.synchronize()
+        e0 = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
+        e1 = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
+        for k in range(a.steps):
+            e0[k].record(stream)
+            flash.closure_batch(m, pd, td, zd, fl, stream=stream, out=res)
+            e1[k].record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(a.steps)]))
+        st = flash.to_host(res)["status"]
+        lin = int(np.sum(st == 0))
+        print(f"[bench] closure: {ms:.3f} ms for {ncell} cells, {lin} linearized", file=sys.stderr, flush=True)
+        return ms, lin
+
+    fp64_ms, fp64_wall, h64, d64 = run(0)
+    closure_ms, closure_lin = run_closure(d64)
+    del d64
+    mixed_ms, mixed_wall, hmx, _ = run(1)
     status_diff = int(np.sum((h64["status"] & 0xFFF) != (hmx["status"] & 0xFFF)))
     count_diff = int(np.sum(h64["phase_count"] != hmx["phase_count"]))
     same = ((h64["status"] & 0xFFF) == (hmx["status"] & 0xFFF)) & (h64["phase_count"] == hmx["phase_count"])
@@ -173,6 +192,9 @@ def main(argv=None):
         "mixed_vs_fp64": {"status_mismatches": status_diff, "phase_count_mismatches": count_diff,
                           "max_rel_diff_beta_x": rel},
         "accepted_phase_counts": counts,
+        "closure": {"ms": closure_ms, "cells": ncell, "linearized_cells": closure_lin,
+                    "what": "Dual<nc+1> closure + implicit sensitivity of every accepted two-phase cell "
+                            "(primal Z, rho; d(beta, x, y, Z, rho)/d(p, T, z_red)) on the FP64 flash result"},
         "roofline": {
             "bound": "mfma",
             "pipe": "fp64 vector ALU (no MFMA: no dense contraction in the flash)",

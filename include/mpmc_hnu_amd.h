@@ -163,6 +163,48 @@ int mpmc_rr_batch(int nc, int64_t n, const double* z, const double* log_k, const
                   int32_t* status, double* beta, double* x, double* y, int32_t* iterations,
                   void* stream);
 
+/* ---------------------------------------------------------------- closure
+ * Thermodynamic closure of accepted two-phase flash results: build_pr76_pt_vle_closure
+ * (modules/physics/include/mpmc/physics/pr76_thermodynamic_closure.hpp:148-381) with its
+ * implicit sensitivity differentiate_pr76_pt_vle (modules/flash/include/mpmc/flash/
+ * pr76_sensitivity.hpp:51-390). Columns q = (p, T, z_0..z_{nc-2}), nq = nc + 1.
+ *
+ * Per-cell status word: primal | linearization << 4 | reason << 8, with
+ *   primal        ThermodynamicClosurePrimalStatus (valid=0, phase_regime_not_implemented=1,
+ *                 indeterminate=2), thermodynamic_closure.hpp:18-22
+ *   linearization ThermodynamicClosureLinearizationStatus (available=0, unavailable=1)
+ *   reason        ThermodynamicClosureLinearizationReason (none=0 .. solution_not_accepted=7),
+ *                 thermodynamic_closure.hpp:26-35
+ *   0x2000        the cell is one the reference rejects with an exception
+ * Two-phase sets (VLE status two-phase, or max3 status two-phase) are closed; single-phase
+ * and three-phase sets are phase_regime_not_implemented; others are indeterminate. */
+typedef struct mpmc_closure_options {
+    double minimum_derivative_phase_fraction; /* PtSensitivityOptions (pt_sensitivity.hpp:26-34), 2^-26 */
+    double minimum_reciprocal_condition;      /* default 0 */
+    double fugacity_tolerance;                /* tolerance of the flash that produced the inputs, 1e-11 */
+    int32_t reserved[2];
+} mpmc_closure_options;
+
+typedef struct mpmc_closure_outputs {
+    int32_t* status;   /* [ncell] status word */
+    double* primal;    /* [4][ncell] Z_L, Z_V, rho_L, rho_V (mol/m^3); NaN unless the primal is valid */
+    double* jac;       /* [2nc+5][nq][ncell] d/dq of beta_V, x_L[nc], y_V[nc], Z_L, Z_V, rho_L, rho_V,
+                        * jac[(row*nq + col)*ncell + c]; NaN unless the linearization is available */
+    double* log_k_jac; /* [nc][nq][ncell] dlogK/dq, or NULL */
+    double* diag;      /* [3][ncell] equilibrium Jacobian rcond, solve backward error, residual norm,
+                        * or NULL */
+} mpmc_closure_outputs;
+
+void mpmc_closure_default_options(mpmc_closure_options* options);
+
+/* Closure of every cell of a flash result (the mpmc_flash_batch outputs status, phase_count,
+ * beta, zfactor, branch and x on the same p, t, z). nc >= 2. Uses a workspace held by the
+ * mixture handle (one closure at a time per handle). */
+int mpmc_closure_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                       const double* z, const mpmc_flash_outputs* flash,
+                       const mpmc_closure_options* options, const mpmc_closure_outputs* out,
+                       void* stream);
+
 /* Last error text for this thread (never NULL). */
 const char* mpmc_last_error(void);
 

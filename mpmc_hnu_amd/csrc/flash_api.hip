@@ -1,6 +1,7 @@
 // C ABI of include/mpmc_hnu_amd.h: mixture residency, batched flash and batched
 // phase evaluation. Argument checks mirror the reference's input contracts; numerical
 // outcomes are per-cell statuses written by the kernels.
+#include "closure_device.hpp"
 #include "flash_pipeline.hpp"
 
 #include "../../include/mpmc_hnu_amd.h"
@@ -16,6 +17,7 @@
 namespace mpmc::dev {
 MPMC_FOR_EACH_NC(MPMC_DECLARE_LAUNCH)
 MPMC_FOR_EACH_NC(MPMC_DECLARE_PIPELINE)
+MPMC_FOR_EACH_NC2(MPMC_DECLARE_CLOSURE)
 } // namespace mpmc::dev
 
 namespace {
@@ -50,6 +52,8 @@ struct mpmc_mixture {
     double* dev_constants; // sqrt_tc, ac, b, kappa [nc each], factor [nc*nc], r_gas
     void* workspace;       // staged-schedule workspace used by mpmc_flash_batch
     size_t workspace_bytes;
+    void* closure_ws;      // eligible-cell list of mpmc_closure_batch
+    size_t closure_ws_bytes;
 };
 
 namespace {
@@ -117,7 +121,7 @@ int mpmc_mixture_create(int nc, const double* tc, const double* pc, const double
     }
     for (int k = 0; k < nc * nc; ++k) { factor[k] = 1.0 - kij[k]; }
     h[4 * nc + nc * nc] = r_gas;
-    auto* m = new (std::nothrow) mpmc_mixture{nc, 0, nullptr, nullptr, 0};
+    auto* m = new (std::nothrow) mpmc_mixture{nc, 0, nullptr, nullptr, 0, nullptr, 0};
     if (!m) { return fail(MPMC_E_NO_MEMORY, "mixture_create: host allocation"); }
     hipError_t e = hipGetDevice(&m->device);
     if (e == hipSuccess) { e = hipMalloc(&m->dev_constants, h.size() * sizeof(double)); }
@@ -134,9 +138,11 @@ int mpmc_mixture_create(int nc, const double* tc, const double* pc, const double
 int mpmc_mixture_destroy(mpmc_mixture_t m) {
     if (!m) { return MPMC_OK; }
     hipError_t e = m->dev_constants ? hipFree(m->dev_constants) : hipSuccess;
-    if (m->workspace) {
-        const hipError_t e2 = hipFree(m->workspace);
-        if (e == hipSuccess) { e = e2; }
+    for (void* ws : {m->workspace, m->closure_ws}) {
+        if (ws) {
+            const hipError_t e2 = hipFree(ws);
+            if (e == hipSuccess) { e = e2; }
+        }
     }
     delete m;
     return e == hipSuccess ? MPMC_OK : hip_fail(e, "mixture_destroy");
@@ -289,6 +295,78 @@ int mpmc_rr_batch(int nc, int64_t n, const double* z, const double* log_k, const
     default: break;
     }
     return e == hipSuccess ? MPMC_OK : hip_fail(e, "rr_batch launch");
+}
+
+void mpmc_closure_default_options(mpmc_closure_options* o) {
+    if (!o) { return; }
+    std::memset(o, 0, sizeof(*o));
+    o->minimum_derivative_phase_fraction = 1.490116119384765625e-8;
+    o->minimum_reciprocal_condition = 0.0;
+    o->fugacity_tolerance = 1e-11;
+}
+
+int mpmc_closure_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                       const mpmc_flash_outputs* flash, const mpmc_closure_options* options,
+                       const mpmc_closure_outputs* out, void* stream) {
+    if (!m || !flash || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: null mixture, flash or outputs"); }
+    if (m->nc < 2) { return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: the closure needs nc >= 2"); }
+    if (ncell < 0 || ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: bad ncell"); }
+    mpmc_closure_options o;
+    mpmc_closure_default_options(&o);
+    if (options) { o = *options; }
+    // PtSensitivityOptions validation (pr76_sensitivity.hpp:56-65)
+    if (!std::isfinite(o.minimum_derivative_phase_fraction) || !(o.minimum_derivative_phase_fraction > 0.0) ||
+        !(o.minimum_derivative_phase_fraction < 0.5) || !std::isfinite(o.minimum_reciprocal_condition) ||
+        o.minimum_reciprocal_condition < 0.0 || o.minimum_reciprocal_condition >= 1.0) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: invalid sensitivity options");
+    }
+    if (!std::isfinite(o.fugacity_tolerance) || !(o.fugacity_tolerance > 0)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: fugacity tolerance must be finite and positive");
+    }
+    if (ncell == 0) { return MPMC_OK; }
+    const void* ptrs[] = {p, t, z, flash->status, flash->phase_count, flash->beta, flash->zfactor, flash->branch,
+                          flash->x, out->status, out->primal, out->jac};
+    for (const void* ptr : ptrs) {
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: buffers must be device memory"); }
+    }
+    if ((out->log_k_jac && !is_device_ptr(out->log_k_jac)) || (out->diag && !is_device_ptr(out->diag))) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: optional outputs must be device memory");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t need = mpmc::dev::closure_workspace_bytes(ncell);
+    hipError_t e = hipSuccess;
+    if (m->closure_ws_bytes < need) {
+        if (m->closure_ws) {
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) { return hip_fail(e, "closure_batch workspace sync"); }
+            (void)hipFree(m->closure_ws);
+            m->closure_ws = nullptr;
+            m->closure_ws_bytes = 0;
+        }
+        e = hipMalloc(&m->closure_ws, need);
+        if (e != hipSuccess) {
+            m->closure_ws = nullptr;
+            return fail(MPMC_E_NO_MEMORY, "closure_batch: workspace allocation failed");
+        }
+        m->closure_ws_bytes = need;
+    }
+    const mpmc::dev::ClosureIn in{flash->status, flash->phase_count, flash->beta, flash->zfactor, flash->branch,
+                                  flash->x};
+    const mpmc::dev::ClosureOut co{out->status, out->primal, out->jac, out->log_k_jac, out->diag};
+    const mpmc::dev::ClosureOpt opt{o.minimum_derivative_phase_fraction, o.minimum_reciprocal_condition,
+                                    o.fugacity_tolerance};
+    e = hipErrorInvalidValue;
+    switch (m->nc) {
+#define MPMC_CASE(N)                                                                                            \
+    case N:                                                                                                     \
+        e = mpmc::dev::launch_closure<N>(m->dev_constants, m->closure_ws, m->closure_ws_bytes, ncell, p, t, z, \
+                                         in, co, opt, s);                                                       \
+        break;
+        MPMC_FOR_EACH_NC2(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: unsupported nc");
+    }
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "closure_batch launch");
 }
 
 } // extern "C"

@@ -199,6 +199,59 @@ def rachford_rice_batch(z, log_k, guess=None, stream=None) -> dict:
     return out
 
 
+def _flash_struct(fl: dict) -> native.FlashOutputs:
+    return native.FlashOutputs(
+        fl["status"].data_ptr(), fl["phase_count"].data_ptr(), fl["beta"].data_ptr(), fl["zfactor"].data_ptr(),
+        fl["branch"].data_ptr(), fl["x"].data_ptr(), fl["ln_phi"].data_ptr() if fl.get("ln_phi") is not None else None,
+        None)
+
+
+def closure_batch(mixture: Mixture, p, t, z, flash: dict, fugacity_tolerance: float = 1e-11,
+                  min_fraction: float = 1.490116119384765625e-8, min_rcond: float = 0.0, log_k: bool = True,
+                  diag: bool = True, stream=None, out: dict | None = None) -> dict:
+    """Thermodynamic closure (build_pr76_pt_vle_closure, pr76_thermodynamic_closure.hpp:148-381)
+    of every cell of a flash result ``flash`` computed on the same p, t, z.
+
+    Returns status [n] (primal | linearization << 4 | reason << 8), primal [4, n]
+    (Z_L, Z_V, rho_L, rho_V), jac [2nc+5, nc+1, n] over q = (p, T, z_0..z_{nc-2}) with
+    rows beta_V, x_L[nc], y_V[nc], Z_L, Z_V, rho_L, rho_V, and optionally log_k_jac
+    [nc, nc+1, n] and diag [3, n] (rcond, backward error, residual norm)."""
+    torch = _torch()
+    lib = native.library()
+    device = torch.device("cuda", torch.cuda.current_device())
+    p = _dev(p, torch.float64, device)
+    t = _dev(t, torch.float64, device)
+    z = _dev(z, torch.float64, device)
+    nc, n = mixture.nc, p.numel()
+    if z.shape != (nc, n) or t.numel() != n or flash["status"].numel() != n:
+        raise ValueError("closure_batch: expected p[n], t[n], z[nc, n] and a flash result of n cells")
+    q = nc + 1
+    if out is None:
+        f64 = torch.float64
+        out = {
+            "status": torch.empty(n, dtype=torch.int32, device=device),
+            "primal": torch.empty((4, n), dtype=f64, device=device),
+            "jac": torch.empty((2 * nc + 5, q, n), dtype=f64, device=device),
+            "log_k_jac": torch.empty((nc, q, n), dtype=f64, device=device) if log_k else None,
+            "diag": torch.empty((3, n), dtype=f64, device=device) if diag else None,
+        }
+    opts = native.ClosureOptions()
+    lib.mpmc_closure_default_options(ctypes.byref(opts))
+    opts.minimum_derivative_phase_fraction = float(min_fraction)
+    opts.minimum_reciprocal_condition = float(min_rcond)
+    opts.fugacity_tolerance = float(fugacity_tolerance)
+    o = native.ClosureOutputs(out["status"].data_ptr(), out["primal"].data_ptr(), out["jac"].data_ptr(),
+                              out["log_k_jac"].data_ptr() if out.get("log_k_jac") is not None else None,
+                              out["diag"].data_ptr() if out.get("diag") is not None else None)
+    fo = _flash_struct(flash)
+    V = ctypes.c_void_p
+    native.check(lib.mpmc_closure_batch(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
+                                        ctypes.byref(fo), ctypes.byref(opts), ctypes.byref(o), _stream_ptr(stream)),
+                 "closure_batch")
+    out["_inputs"] = (p, t, z, flash)
+    return out
+
+
 def to_host(res: dict) -> dict:
     """Synchronise and copy a result dict to numpy (counters as uint32)."""
     torch = _torch()

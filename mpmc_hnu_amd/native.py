@@ -26,6 +26,8 @@ EXPORTED = (
     "mpmc_flash_workspace_bytes",
     "mpmc_phase_batch",
     "mpmc_rr_batch",
+    "mpmc_closure_default_options",
+    "mpmc_closure_batch",
     "mpmc_last_error",
 )
 
@@ -50,6 +52,25 @@ class FlashOutputs(ctypes.Structure):
         ("x", ctypes.c_void_p),
         ("ln_phi", ctypes.c_void_p),
         ("counters", ctypes.c_void_p),
+    ]
+
+
+class ClosureOptions(ctypes.Structure):
+    _fields_ = [
+        ("minimum_derivative_phase_fraction", ctypes.c_double),
+        ("minimum_reciprocal_condition", ctypes.c_double),
+        ("fugacity_tolerance", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 2),
+    ]
+
+
+class ClosureOutputs(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_void_p),
+        ("primal", ctypes.c_void_p),
+        ("jac", ctypes.c_void_p),
+        ("log_k_jac", ctypes.c_void_p),
+        ("diag", ctypes.c_void_p),
     ]
 
 
@@ -83,6 +104,10 @@ def library() -> ctypes.CDLL:
     lib.mpmc_flash_workspace_bytes.restype = ctypes.c_size_t
     lib.mpmc_phase_batch.argtypes = [P, ctypes.c_int64, P, P, P, P, P, P, P, P, P]
     lib.mpmc_rr_batch.argtypes = [ctypes.c_int, ctypes.c_int64] + [P] * 9
+    lib.mpmc_closure_default_options.argtypes = [ctypes.POINTER(ClosureOptions)]
+    lib.mpmc_closure_default_options.restype = None
+    lib.mpmc_closure_batch.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOutputs),
+                                       ctypes.POINTER(ClosureOptions), ctypes.POINTER(ClosureOutputs), P]
     lib.mpmc_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib

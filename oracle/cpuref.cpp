@@ -27,6 +27,7 @@
 #include <limits>
 #include <memory>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace cpuref {
@@ -1363,6 +1364,564 @@ void publish_max3(const Out& o, int n, int64_t N, int64_t c, const Max3Result& M
     }
 }
 
+// ---------------------------------------------------------------- forward-mode AD
+// Dual number of ad/dual.hpp:78-345 with the derivative rules of ad/math.hpp
+// (log :200-214, log1p :258-272, sqrt :277-291): value plus N directional derivatives.
+template <int N>
+struct Dual {
+    double v = 0;
+    double d[N] = {};
+    Dual() = default;
+    Dual(double x) : v(x) {}
+};
+struct AdDomain {};   // std::domain_error (AD math domain, division by zero, mixing domain)
+struct RangeFail {};  // std::range_error
+struct PhaseFail {};  // Pr76PhaseError
+struct IndexFail {};  // std::out_of_range (escapes the sensitivity's handlers)
+
+inline double val(double x) { return x; }
+template <int N> inline double val(const Dual<N>& x) { return x.v; }
+inline bool fin(double x) { return std::isfinite(x); }
+template <int N> inline bool fin(const Dual<N>& x) {
+    if (!std::isfinite(x.v)) { return false; }
+    for (int k = 0; k < N; ++k) { if (!std::isfinite(x.d[k])) { return false; } }
+    return true;
+}
+
+template <int N> Dual<N> operator+(Dual<N> a, const Dual<N>& b) { a.v += b.v; for (int k = 0; k < N; ++k) { a.d[k] += b.d[k]; } return a; }
+template <int N> Dual<N> operator-(Dual<N> a, const Dual<N>& b) { a.v -= b.v; for (int k = 0; k < N; ++k) { a.d[k] -= b.d[k]; } return a; }
+template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r;
+    for (int k = 0; k < N; ++k) { r.d[k] = a.d[k] * b.v + a.v * b.d[k]; }
+    r.v = a.v * b.v;
+    return r;
+}
+template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+    if (b.v == 0) { throw AdDomain{}; }
+    Dual<N> r;
+    const double q = a.v / b.v;
+    for (int k = 0; k < N; ++k) { r.d[k] = (a.d[k] - q * b.d[k]) / b.v; }
+    r.v = q;
+    return r;
+}
+template <int N> Dual<N> operator+(Dual<N> a, double s) { a.v += s; return a; }
+template <int N> Dual<N> operator+(double s, const Dual<N>& a) { return Dual<N>(s) + a; }
+template <int N> Dual<N> operator-(Dual<N> a, double s) { a.v -= s; return a; }
+template <int N> Dual<N> operator-(double s, const Dual<N>& a) { return Dual<N>(s) - a; }
+template <int N> Dual<N> operator*(Dual<N> a, double s) { a.v *= s; for (int k = 0; k < N; ++k) { a.d[k] *= s; } return a; }
+template <int N> Dual<N> operator*(double s, Dual<N> a) { a.v = s * a.v; for (int k = 0; k < N; ++k) { a.d[k] = s * a.d[k]; } return a; }
+template <int N> Dual<N> operator/(Dual<N> a, double s) {
+    if (s == 0) { throw AdDomain{}; }
+    a.v /= s; for (int k = 0; k < N; ++k) { a.d[k] /= s; } return a;
+}
+template <int N> Dual<N> operator/(double s, const Dual<N>& a) { return Dual<N>(s) / a; }
+template <int N> Dual<N>& operator+=(Dual<N>& a, const Dual<N>& b) { a = a + b; return a; }
+template <int N> Dual<N>& operator-=(Dual<N>& a, const Dual<N>& b) { a = a - b; return a; }
+template <int N> Dual<N> dlog(const Dual<N>& x) {
+    if (!std::isfinite(x.v) || !(x.v > 0)) { throw AdDomain{}; }
+    Dual<N> r(std::log(x.v));
+    for (int k = 0; k < N; ++k) { r.d[k] = x.d[k] / x.v; }
+    return r;
+}
+template <int N> Dual<N> dlog1p(const Dual<N>& x) {
+    if (!std::isfinite(x.v) || !(x.v > -1)) { throw AdDomain{}; }
+    Dual<N> r(std::log1p(x.v));
+    for (int k = 0; k < N; ++k) { r.d[k] = x.d[k] / (1.0 + x.v); }
+    return r;
+}
+template <int N> Dual<N> dsqrt(const Dual<N>& x) {
+    if (!std::isfinite(x.v) || !(x.v > 0)) { throw AdDomain{}; }
+    const double s = std::sqrt(x.v);
+    Dual<N> r(s);
+    for (int k = 0; k < N; ++k) { r.d[k] = (x.d[k] / s) / 2.0; }
+    return r;
+}
+inline double dlog(double x) { return std::log(x); }
+inline double dlog1p(double x) { return std::log1p(x); }
+inline double dsqrt(double x) { return std::sqrt(x); }
+
+// log1p(u)/u in Number arithmetic, pr76_phase.hpp:67-79.
+template <typename Num>
+Num log1p_over_x_n(const Num& u) {
+    if (std::fabs(val(u)) <= 0.125) {
+        constexpr int degree = 53 / 3 + 2;
+        Num v((degree % 2 == 0 ? 1.0 : -1.0) / static_cast<double>(degree + 1));
+        for (int k = degree - 1; k >= 0; --k) { v = (k % 2 == 0 ? 1.0 : -1.0) / static_cast<double>(k + 1) + u * v; }
+        return v;
+    }
+    return dlog1p(u) / u;
+}
+
+// Pr76Phase::evaluate_reduced (pr76_phase.hpp:193-258) through Pr76Mixture::evaluate_impl
+// <Reduced=true, WithSums=true> (pr76_mixture.hpp:124-219) and Pr76Pure::evaluate
+// (pr76_pure.hpp:144-165): n-1 independent fractions, x_last = 1 - sum in Number
+// arithmetic; roots on the primal and the first-order IFT for the selected root.
+template <typename Num>
+void phase_reduced(const Mixture& m, const Num& p, const Num& t, const Num* xr, int root_index, Num& zout,
+                   Num* lnphi) {
+    const int n = m.n;
+    if (!fin(p) || !(val(p) > 0)) { throw AdDomain{}; }
+    Num last(1.0), isum(0.0);
+    for (int i = 0; i + 1 < n; ++i) {
+        if (!fin(xr[i]) || val(xr[i]) < 0 || val(xr[i]) > 1) { throw AdDomain{}; }
+        isum += xr[i];
+    }
+    last -= isum;
+    if (!fin(last) || val(last) < 0 || val(last) > 1) { throw AdDomain{}; }
+    auto x = [&](int i) -> const Num& { return i == n - 1 ? last : xr[i]; };
+    if (!fin(t) || !(val(t) > 0)) { throw AdDomain{}; }
+    Num roots_a[NCMAX], sums[NCMAX];
+    Num att(0.0), cov(0.0);
+    for (int i = 0; i < n; ++i) {
+        const Num rr = dsqrt(t) / m.sqrt_tc[i];
+        const Num br = 1.0 + m.kappa[i] * (1.0 - rr);
+        if (!fin(rr) || !fin(br)) { throw RangeFail{}; }
+        const Num a = (br * m.ac[i]) * br;
+        if (!fin(a) || (val(a) == 0 && val(br) != 0)) { throw RangeFail{}; }
+        att += (x(i) * a) * x(i);
+        cov += x(i) * Num(m.b[i]);
+        sums[i] = x(i) * a;
+        if (n > 1) {
+            if (!(val(a) > 0)) { throw AdDomain{}; }
+            roots_a[i] = dsqrt(a);
+            if (!fin(roots_a[i])) { throw RangeFail{}; }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        for (int j = i + 1; j < n; ++j) {
+            const double f = m.factor[i * n + j];
+            const Num cross = ((x(i) * roots_a[i]) * (x(j) * roots_a[j])) * f;
+            att += cross;
+            att += cross;
+            const Num aij = (roots_a[i] * f) * roots_a[j];
+            sums[i] += x(j) * aij;
+            sums[j] += x(i) * aij;
+        }
+    }
+    if (!fin(att) || !fin(cov) || !(val(cov) > 0)) { throw RangeFail{}; }
+    for (int i = 0; i < n; ++i) { if (!fin(sums[i])) { throw RangeFail{}; } }
+    // Pr76Phase::coefficients, pr76_phase.hpp:176-191
+    const Num rt = R_GAS * t;
+    if (!fin(rt) || !(val(rt) > 0)) { throw RangeFail{}; }
+    const Num prt = p / rt;
+    const Num A = (att / rt) * prt;
+    const Num B = cov * prt;
+    if (!fin(A) || !fin(B) || !(val(B) > 0) || (val(A) == 0 && val(att) != 0)) { throw RangeFail{}; }
+    const RootSet rs = roots(val(A), val(B));
+    if (rs.status != RS_SUCCESS) { throw PhaseFail{}; }
+    if (root_index < 0 || root_index >= rs.count) { throw IndexFail{}; }
+    const Root& r = rs.r[root_index];
+    Num fv(r.x);
+    if constexpr (!std::is_same_v<Num, double>) {
+        if (!r.dvalid) { throw PhaseFail{}; }
+        const double y0 = r.x / rs.scale;
+        // pr76_cubic_coefficients in Number arithmetic (pr76_roots.hpp:50-58)
+        const Num bs = B / rs.scale;
+        const double inv = 1.0 / rs.scale;
+        const Num c2 = 4.0 * bs - inv;
+        const Num c1 = (A / rs.scale) / rs.scale + 2.0 * bs * bs - 4.0 * bs * inv;
+        const Num c0 = ((-2.0 * bs) * bs) * inv;
+        const Num resid = ((Num(y0) + c2) * y0 + c1) * y0 + c0;
+        const double dh = (3.0 * y0 + 2.0 * val(c2)) * y0 + val(c1);
+        fv -= ((resid - Num(val(resid))) / dh) * rs.scale;
+    }
+    const Num z = fv + B;
+    const double s2 = std::sqrt(2.0);
+    const Num den = fv + (2.0 - s2) * B;
+    const Num u = (2.0 * s2) * (B / den);
+    const Num lf = log1p_over_x_n(u) / den;
+    Num zm1 = z - 1.0;
+    Num lfree = dlog(fv);
+    if (std::fabs(val(zm1)) < 0.25 && val(B) < 0.125) {
+        const Num bz = B / z;
+        zm1 = B / fv - (A / z) / (1.0 + 2.0 * bz - bz * bz);
+        lfree = dlog1p(zm1 - B);
+    }
+    const Num rt2 = R_GAS * t;
+    const Num prt2 = p / rt2;
+    for (int i = 0; i < n; ++i) {
+        const Num ratio = Num(m.b[i]) / cov;
+        const Num coef = ((2.0 * sums[i] - att * ratio) / rt2) * prt2;
+        lnphi[i] = ratio * zm1 - lfree - coef * lf;
+        if (!fin(lnphi[i])) { throw RangeFail{}; }
+    }
+    if (!fin(z)) { throw RangeFail{}; }
+    zout = z;
+}
+
+// ---------------------------------------------------------------- closure
+// Statuses: ThermodynamicClosurePrimalStatus / LinearizationStatus / LinearizationReason
+// (thermodynamic_closure.hpp:18-35) and PtSensitivityStatus (pt_sensitivity.hpp).
+enum ClosurePrimal { CP_VALID = 0, CP_NOT_IMPLEMENTED = 1, CP_INDETERMINATE = 2 };
+enum ClosureReason { CR_NONE = 0, CR_NOT_IMPLEMENTED, CR_PHASE_BOUNDARY, CR_ILL_CONDITIONED, CR_FEED_SUPPORT,
+                     CR_PROPERTY_FAILURE, CR_ARITHMETIC_FAILURE, CR_NOT_ACCEPTED };
+struct SensFail { int reason; };
+
+struct ClosureResult {
+    int primal = CP_INDETERMINATE, lin = 1, reason = CR_NONE;
+    bool thrown = false;
+    double zl = NaN, zv = NaN, rhol = NaN, rhov = NaN;
+    double jac[(2 * NCMAX + 5) * (NCMAX + 1)];
+    double dlk[NCMAX * (NCMAX + 1)];
+    bool have_jac = false, have_dlk = false;
+    double rcond = NaN, backward = NaN, resid = NaN;
+};
+
+// pt_sensitivity_rr_linearization, pt_sensitivity_detail.hpp:45-143
+template <int K>
+void rr_linearization(int n, const Dual<K>* feed, const Dual<K>* lk, const double* bx, const double* by,
+                      double bbeta, Dual<K>& beta, Dual<K>* liq, Dual<K>* vap) {
+    double x[NCMAX], y[NCMAX], xs = 0, ys = 0;
+    for (int i = 0; i + 1 < n; ++i) { x[i] = bx[i]; y[i] = by[i]; xs += x[i]; ys += y[i]; }
+    x[n - 1] = 1.0 - xs;
+    y[n - 1] = 1.0 - ys;
+    for (int i = 0; i < n; ++i) {
+        if (!(feed[i].v > 0) || !(x[i] > 0) || !(y[i] > 0) || !std::isfinite(x[i]) || !std::isfinite(y[i])) {
+            throw SensFail{CR_PHASE_BOUNDARY};
+        }
+    }
+    double h = 0;
+    for (int i = 0; i < n; ++i) { const double d = y[i] - x[i]; h += d * d / feed[i].v; }
+    if (!std::isfinite(h) || !(h > 0)) { throw SensFail{CR_ILL_CONDITIONED}; }
+    beta = Dual<K>(bbeta);
+    for (int l = 0; l < K; ++l) {
+        double num = 0;
+        for (int i = 0; i < n; ++i) {
+            const double z = feed[i].v;
+            num += (x[i] * y[i] / z) * lk[i].d[l] + ((y[i] - x[i]) / z) * feed[i].d[l];
+        }
+        beta.d[l] = num / h;
+        if (!std::isfinite(beta.d[l])) { throw SensFail{CR_ARITHMETIC_FAILURE}; }
+    }
+    Dual<K> ls(0.0), vs(0.0);
+    for (int i = 0; i + 1 < n; ++i) {
+        Dual<K> dx(x[i]), dy(y[i]);
+        const double z = feed[i].v;
+        for (int l = 0; l < K; ++l) {
+            const double dlx = feed[i].d[l] / z - ((y[i] - x[i]) / z) * beta.d[l] - bbeta * (y[i] / z) * lk[i].d[l];
+            dx.d[l] = x[i] * dlx;
+            dy.d[l] = y[i] * (dlx + lk[i].d[l]);
+            if (!std::isfinite(dx.d[l]) || !std::isfinite(dy.d[l])) { throw SensFail{CR_ARITHMETIC_FAILURE}; }
+        }
+        liq[i] = dx; vap[i] = dy;
+        ls += dx; vs += dy;
+    }
+    liq[n - 1] = 1.0 - ls;
+    vap[n - 1] = 1.0 - vs;
+    if (!(liq[n - 1].v > 0) || !(vap[n - 1].v > 0)) { throw SensFail{CR_PHASE_BOUNDARY}; }
+}
+
+// Blocked forward mode of value_and_jacobian_runtime (ad/runtime_differentiate.hpp:87-168),
+// direction width 4: ceil(inputs/4) passes, each re-running the whole primal; the
+// primal outputs must be identical in every pass.
+template <typename Fn>
+void jacobian_blocked(int nin, int nout, const double* inputs, Fn fn, double* values, double* J) {
+    constexpr int K = 4;
+    Dual<K> in[2 * NCMAX + 1], out[3 * NCMAX + 1];
+    for (int b0 = 0; b0 < nin; b0 += K) {
+        for (int i = 0; i < nin; ++i) {
+            in[i] = Dual<K>(inputs[i]);
+            if (i >= b0 && i < b0 + K) { in[i].d[i - b0] = 1.0; }
+        }
+        fn(in, out);
+        for (int o = 0; o < nout; ++o) {
+            if (b0 == 0) { values[o] = out[o].v; }
+            else if (!(values[o] == out[o].v) && !(std::isnan(values[o]) && std::isnan(out[o].v))) {
+                throw SensFail{CR_ARITHMETIC_FAILURE};
+            }
+            for (int k = 0; k < K && b0 + k < nin; ++k) { J[o * nin + b0 + k] = out[o].d[k]; }
+        }
+    }
+}
+
+// pt_sensitivity_factor / _solve / _rcond, pt_sensitivity_detail.hpp:152-259
+struct Lu { int n; double a[NCMAX * NCMAX]; int piv[NCMAX]; double norm; };
+bool lu_factor(const double* A, int n, Lu& f) {
+    f.n = n;
+    std::memcpy(f.a, A, sizeof(double) * n * n);
+    f.norm = 0;
+    for (int r = 0; r < n; ++r) {
+        double s = 0;
+        for (int c = 0; c < n; ++c) { s += std::fabs(A[r * n + c]); }
+        f.norm = std::max(f.norm, s);
+    }
+    if (!(f.norm > 0) || !std::isfinite(f.norm)) { return false; }
+    const double guard = 64.0 * EPS * n * f.norm;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        double mag = std::fabs(f.a[c * n + c]);
+        for (int r = c + 1; r < n; ++r) {
+            const double cand = std::fabs(f.a[r * n + c]);
+            if (cand > mag) { mag = cand; p = r; }
+        }
+        if (!std::isfinite(mag) || !(mag > guard)) { return false; }
+        f.piv[c] = p;
+        if (p != c) { for (int j = 0; j < n; ++j) { std::swap(f.a[c * n + j], f.a[p * n + j]); } }
+        const double dg = f.a[c * n + c];
+        for (int r = c + 1; r < n; ++r) {
+            const double mlt = f.a[r * n + c] / dg;
+            f.a[r * n + c] = mlt;
+            for (int j = c + 1; j < n; ++j) { f.a[r * n + j] -= mlt * f.a[c * n + j]; }
+        }
+    }
+    return true;
+}
+bool lu_solve(const Lu& f, double* rhs) {
+    const int n = f.n;
+    for (int c = 0; c < n; ++c) { if (f.piv[c] != c) { std::swap(rhs[c], rhs[f.piv[c]]); } }
+    for (int r = 0; r < n; ++r) { for (int c = 0; c < r; ++c) { rhs[r] -= f.a[r * n + c] * rhs[c]; } }
+    for (int r = n; r-- > 0;) {
+        for (int c = r + 1; c < n; ++c) { rhs[r] -= f.a[r * n + c] * rhs[c]; }
+        rhs[r] /= f.a[r * n + r];
+        if (!std::isfinite(rhs[r])) { return false; }
+    }
+    return true;
+}
+double lu_rcond(const Lu& f) {
+    const int n = f.n;
+    double rows[NCMAX] = {};
+    for (int c = 0; c < n; ++c) {
+        double rhs[NCMAX] = {};
+        rhs[c] = 1.0;
+        if (!lu_solve(f, rhs)) { return 0.0; }
+        for (int r = 0; r < n; ++r) { rows[r] += std::fabs(rhs[r]); }
+    }
+    double inv = rows[0];
+    for (int r = 1; r < n; ++r) { inv = std::max(inv, rows[r]); }
+    const double prod = f.norm * inv;
+    if (!(prod > 0) || !std::isfinite(prod)) { return 0.0; }
+    return 1.0 / prod;
+}
+
+// differentiate_pr76_pt_vle, pr76_sensitivity.hpp:51-390, for an accepted two-phase
+// candidate. Writes dlogK (n x q), d beta (q), dx, dy (n x q each). Throws SensFail.
+void differentiate(const Mixture& m, double p, double t, const double* feed, const SplitState& c,
+                   const SplitOptions& so, double min_pf, double min_rcond, ClosureResult& R,
+                   double* dbeta, double* dx, double* dy) {
+    const int n = m.n;
+    const int q = n + 1;
+    for (int i = 0; i < n; ++i) { if (!(feed[i] > 0)) { throw SensFail{CR_FEED_SUPPORT}; } }
+    double contrast = 0;
+    for (int i = 0; i < n; ++i) { contrast = std::max(contrast, std::fabs(c.lk[i])); }
+    if (!(contrast > so.lk_sep) || !(c.vap.z - c.liq.z > so.z_sep * std::max(c.vap.z, c.liq.z))) {
+        throw SensFail{CR_PHASE_BOUNDARY};
+    }
+    const double beta = c.fr.beta;
+    const double guard = std::max(min_pf, so.min_pf);
+    if (!(beta > guard) || !(1.0 - beta > guard)) { throw SensFail{CR_PHASE_BOUNDARY}; }
+    if (!(c.norm <= so.fug_tol)) { throw SensFail{CR_NOT_ACCEPTED}; }
+    for (int i = 0; i < n; ++i) {
+        if (!(c.fr.x[i] > 0) || !(c.fr.y[i] > 0) || !std::isfinite(c.lk[i])) { throw SensFail{CR_FEED_SUPPORT}; }
+        const double rec = std::fma(beta, c.fr.y[i], (1.0 - beta) * c.fr.x[i]);
+        const double err = std::fabs(rec - feed[i]);
+        if (err > so.mass_abs || err / feed[i] > so.mass_rel) { throw IndexFail{}; } // invalid_argument
+    }
+    // pr76_sensitivity_local_system, detail/pr76_sensitivity_local.hpp:29-109
+    const int nv = 2 * n + 1, no = 3 * n + 1;
+    double inputs[2 * NCMAX + 1];
+    for (int i = 0; i < n; ++i) { inputs[i] = c.lk[i]; }
+    inputs[n] = p;
+    inputs[n + 1] = t;
+    for (int i = 0; i + 1 < n; ++i) { inputs[n + 2 + i] = feed[i]; }
+    const int bl = c.liq.act.branch, bv = c.vap.act.branch;
+    auto eqs = [&](const Dual<4>* v, Dual<4>* out) {
+        Dual<4> lk[NCMAX], z[NCMAX], fs(0.0);
+        for (int i = 0; i < n; ++i) { lk[i] = v[i]; }
+        for (int i = 0; i + 1 < n; ++i) { z[i] = v[n + 2 + i]; fs += z[i]; }
+        z[n - 1] = 1.0 - fs;
+        Dual<4> b, xl[NCMAX], yv[NCMAX];
+        rr_linearization<4>(n, z, lk, c.fr.x, c.fr.y, c.fr.beta, b, xl, yv);
+        Dual<4> zl, zv, ll[NCMAX], lv[NCMAX];
+        phase_reduced<Dual<4>>(m, v[n], v[n + 1], xl, bl, zl, ll);
+        phase_reduced<Dual<4>>(m, v[n], v[n + 1], yv, bv, zv, lv);
+        for (int i = 0; i < n; ++i) { out[i] = dlog(xl[i]) - dlog(yv[i]) + ll[i] - lv[i]; }
+        out[n] = b;
+        for (int i = 0; i < n; ++i) { out[n + 1 + i] = xl[i]; out[2 * n + 1 + i] = yv[i]; }
+    };
+    double vals[3 * NCMAX + 1], J[(3 * NCMAX + 1) * (2 * NCMAX + 1)];
+    try {
+        jacobian_blocked(nv, no, inputs, eqs, vals, J);
+    } catch (const PhaseFail&) {
+        throw SensFail{CR_PROPERTY_FAILURE};
+    } catch (const RangeFail&) {
+        throw SensFail{CR_PROPERTY_FAILURE};
+    } catch (const AdDomain&) {
+        throw SensFail{CR_ARITHMETIC_FAILURE};
+    }
+    double rn = 0;
+    for (int i = 0; i < n; ++i) { rn = std::max(rn, std::fabs(vals[i])); }
+    R.resid = rn;
+    const double ascale = 8192.0 * EPS * n;
+    if (rn > so.fug_tol + ascale) { throw SensFail{CR_NOT_ACCEPTED}; }
+    double A[NCMAX * NCMAX], Bm[NCMAX * (NCMAX + 1)];
+    for (int r = 0; r < n; ++r) {
+        for (int col = 0; col < n; ++col) { A[r * n + col] = J[r * nv + col]; }
+        for (int col = 0; col < q; ++col) { Bm[r * q + col] = J[r * nv + n + col]; }
+    }
+    Lu f;
+    if (!lu_factor(A, n, f)) { throw SensFail{CR_ILL_CONDITIONED}; }
+    R.rcond = lu_rcond(f);
+    const double need = std::max(256.0 * EPS * n, min_rcond);
+    if (!std::isfinite(R.rcond) || !(R.rcond > need)) { throw SensFail{CR_ILL_CONDITIONED}; }
+    double* D = R.dlk;
+    for (int col = 0; col < q; ++col) {
+        double rhs[NCMAX];
+        for (int r = 0; r < n; ++r) { rhs[r] = -Bm[r * q + col]; }
+        if (!lu_solve(f, rhs)) { throw SensFail{CR_ILL_CONDITIONED}; }
+        for (int r = 0; r < n; ++r) { D[r * q + col] = rhs[r]; }
+    }
+    double be = 0;
+    for (int col = 0; col < q; ++col) {
+        for (int r = 0; r < n; ++r) {
+            double res = Bm[r * q + col];
+            double sc = std::fabs(res);
+            for (int j = 0; j < n; ++j) {
+                const double term = A[r * n + j] * D[j * q + col];
+                res += term;
+                sc += std::fabs(term);
+            }
+            be = std::max(be, std::fabs(res) / std::max(1.0, sc));
+        }
+    }
+    R.backward = be;
+    if (!std::isfinite(be) || be > ascale) { throw SensFail{CR_ARITHMETIC_FAILURE}; }
+    auto total = [&](int o, int col) {
+        double v = J[o * nv + n + col];
+        for (int j = 0; j < n; ++j) { v += J[o * nv + j] * D[j * q + col]; }
+        return v;
+    };
+    bool finite_all = true;
+    for (int col = 0; col < q; ++col) {
+        dbeta[col] = total(n, col);
+        finite_all = finite_all && std::isfinite(dbeta[col]);
+        for (int i = 0; i < n; ++i) {
+            dx[i * q + col] = total(n + 1 + i, col);
+            dy[i * q + col] = total(2 * n + 1 + i, col);
+            finite_all = finite_all && std::isfinite(dx[i * q + col]) && std::isfinite(dy[i * q + col]);
+        }
+    }
+    for (int k = 0; k < n * q; ++k) { finite_all = finite_all && std::isfinite(D[k]); }
+    if (!finite_all) { throw SensFail{CR_ARITHMETIC_FAILURE}; }
+    const double bb = vals[n];
+    for (int col = 0; col < q; ++col) {
+        double ls = 0, vs = 0;
+        for (int i = 0; i < n; ++i) { ls += dx[i * q + col]; vs += dy[i * q + col]; }
+        if (std::fabs(ls) > ascale || std::fabs(vs) > ascale) { throw SensFail{CR_ARITHMETIC_FAILURE}; }
+        for (int i = 0; i < n; ++i) {
+            const double dz = col < 2 ? 0.0 : (i == col - 2 ? 1.0 : (i + 1 == n ? -1.0 : 0.0));
+            const double xv = vals[n + 1 + i], yv = vals[2 * n + 1 + i];
+            const double der = (1.0 - bb) * dx[i * q + col] + bb * dy[i * q + col] + (yv - xv) * dbeta[col] - dz;
+            const double sc = std::max(1.0, std::fabs(dx[i * q + col]) + std::fabs(dy[i * q + col]) +
+                                                std::fabs(dbeta[col]) + std::fabs(dz));
+            if (std::fabs(der) > ascale * sc) { throw SensFail{CR_ARITHMETIC_FAILURE}; }
+        }
+    }
+    R.have_dlk = true;
+}
+
+// pr76_phase_z_partials, pr76_thermodynamic_closure.hpp:81-118: Z over (p, T, w_0..w_{n-2}).
+void z_partials(const Mixture& m, double p, double t, const double* w, int branch, double& zval, double* grad) {
+    const int n = m.n;
+    double inputs[NCMAX + 1];
+    inputs[0] = p;
+    inputs[1] = t;
+    for (int i = 0; i + 1 < n; ++i) { inputs[2 + i] = w[i]; }
+    auto fn = [&](const Dual<4>* v, Dual<4>* out) {
+        Dual<4> lp[NCMAX];
+        phase_reduced<Dual<4>>(m, v[0], v[1], v + 2, branch, out[0], lp);
+    };
+    jacobian_blocked(n + 1, 1, inputs, fn, &zval, grad);
+}
+
+inline bool same_z(double v, double acc) { // pr76_thermodynamic_closure.hpp:70-79
+    if (!std::isfinite(v) || !std::isfinite(acc)) { return false; }
+    return std::fabs(v - acc) <= 1024.0 * EPS * std::max(1.0, std::fabs(acc));
+}
+
+// build_pr76_pt_vle_closure, pr76_thermodynamic_closure.hpp:148-381, on the oracle's
+// own VLE result. Row order of R.jac: beta_V, x_L[n], y_V[n], Z_L, Z_V, rho_L, rho_V.
+void build_closure(const Mixture& m, double p, double t, const VleResult& V, const SplitOptions& so, double min_pf,
+                   double min_rcond, ClosureResult& R) {
+    const int n = m.n;
+    const int q = n + 1;
+    if (V.status == V_SINGLE) { R.primal = CP_NOT_IMPLEMENTED; return; }
+    if (V.status == V_UNSTABLE || V.status == V_INDETERMINATE) { R.primal = CP_INDETERMINATE; return; }
+    if (!V.has_candidate || !V.has_final || V.fin.status != S_NO_INSTABILITY) { R.primal = CP_INDETERMINATE; return; }
+    const SplitState& c = V.cand.pt;
+    const double beta = c.fr.beta;
+    if (!std::isfinite(beta) || !(beta > 0) || !(beta < 1) || !std::isfinite(c.liq.z) || !(c.liq.z > 0) ||
+        !std::isfinite(c.vap.z) || !(c.vap.z > 0)) {
+        R.primal = CP_INDETERMINATE;
+        return;
+    }
+    // primal reproduction at the full stored compositions
+    try {
+        PhaseCtx cl, cv;
+        double lp[NCMAX], zl2 = NaN, zv2 = NaN;
+        if (prepare(m, p, t, c.fr.x, cl) != NONE || c.liq.act.branch >= cl.rs.count ||
+            lnphi(m, p, t, cl, c.liq.act.branch, lp, &zl2) != NONE) { throw RangeFail{}; }
+        if (prepare(m, p, t, c.fr.y, cv) != NONE || c.vap.act.branch >= cv.rs.count ||
+            lnphi(m, p, t, cv, c.vap.act.branch, lp, &zv2) != NONE) { throw RangeFail{}; }
+        if (!same_z(zl2, c.liq.z) || !same_z(zv2, c.vap.z)) { throw RangeFail{}; }
+        const double rl = p / (c.liq.z * R_GAS * t), rv = p / (c.vap.z * R_GAS * t);
+        if (!std::isfinite(rl) || !(rl > 0) || !std::isfinite(rv) || !(rv > 0)) { throw RangeFail{}; }
+        R.zl = c.liq.z; R.zv = c.vap.z; R.rhol = rl; R.rhov = rv;
+        R.primal = CP_VALID;
+    } catch (...) {
+        R.primal = CP_INDETERMINATE;
+        return;
+    }
+    double dbeta[NCMAX + 1], dx[NCMAX * (NCMAX + 1)], dy[NCMAX * (NCMAX + 1)];
+    try {
+        differentiate(m, p, t, V.init.feed, c, so, min_pf, min_rcond, R, dbeta, dx, dy);
+    } catch (const SensFail& e) {
+        R.reason = e.reason;
+        if (e.reason == CR_NOT_ACCEPTED) {
+            R.primal = CP_INDETERMINATE;
+            R.zl = R.zv = R.rhol = R.rhov = NaN;
+        }
+        return;
+    }
+    try {
+        double zlv, zvv, gl[NCMAX + 1], gv[NCMAX + 1];
+        z_partials(m, p, t, c.fr.x, c.liq.act.branch, zlv, gl);
+        z_partials(m, p, t, c.fr.y, c.vap.act.branch, zvv, gv);
+        if (!same_z(zlv, R.zl) || !same_z(zvv, R.zv)) { throw AdDomain{}; }
+        double* J = R.jac;
+        bool ok = true;
+        for (int col = 0; col < q; ++col) {
+            double dzl = col == 0 ? gl[0] : (col == 1 ? gl[1] : 0.0);
+            double dzv = col == 0 ? gv[0] : (col == 1 ? gv[1] : 0.0);
+            for (int k = 0; k + 1 < n; ++k) {
+                dzl += gl[2 + k] * dx[k * q + col];
+                dzv += gv[2 + k] * dy[k * q + col];
+            }
+            const double dp = col == 0 ? 1.0 / p : 0.0;
+            const double dt = col == 1 ? 1.0 / t : 0.0;
+            J[0 * q + col] = dbeta[col];
+            for (int i = 0; i < n; ++i) {
+                J[(1 + i) * q + col] = dx[i * q + col];
+                J[(1 + n + i) * q + col] = dy[i * q + col];
+            }
+            J[(2 * n + 1) * q + col] = dzl;
+            J[(2 * n + 2) * q + col] = dzv;
+            J[(2 * n + 3) * q + col] = R.rhol * (dp - dzl / R.zl - dt);
+            J[(2 * n + 4) * q + col] = R.rhov * (dp - dzv / R.zv - dt);
+        }
+        for (int k = 0; k < (2 * n + 5) * q; ++k) { ok = ok && std::isfinite(J[k]); }
+        if (!ok) { throw AdDomain{}; }
+        R.have_jac = true;
+        R.lin = 0;
+        R.reason = CR_NONE;
+    } catch (const PhaseFail&) {
+        R.reason = CR_PROPERTY_FAILURE;
+    } catch (const RangeFail&) {
+        R.reason = CR_PROPERTY_FAILURE;
+    } catch (...) {
+        R.reason = CR_ARITHMETIC_FAILURE;
+    }
+}
+
 template <typename Body>
 void parallel_cells(int64_t ncell, int threads, Body body) {
     if (threads <= 0) { threads = static_cast<int>(std::max(1U, std::thread::hardware_concurrency())); }
@@ -1470,4 +2029,48 @@ extern "C" int cpuref_rachford_rice(int nc, const double* z, const double* log_k
     for (int i = 0; i < nc; ++i) { x[i] = r.x[i]; y[i] = r.y[i]; }
     *iterations = r.iterations;
     return r.status;
+}
+
+extern "C" int cpuref_closure_batch(int nc, const double* tc, const double* pc, const double* omega,
+                                    const double* kij, int64_t ncell, const double* p, const double* t,
+                                    const double* z, double fugacity_tolerance, double min_fraction,
+                                    double min_rcond, int threads, int32_t* status, double* primal, double* jac,
+                                    double* log_k_jac, double* diag) {
+    if (nc < 2 || nc > NCMAX || ncell < 0) { return 1; }
+    if (!std::isfinite(min_fraction) || !(min_fraction > 0) || !(min_fraction < 0.5) ||
+        !std::isfinite(min_rcond) || min_rcond < 0 || min_rcond >= 1) { return 1; }
+    const Mixture m = make_mixture(nc, tc, pc, omega, kij);
+    SplitOptions so;
+    so.fug_tol = fugacity_tolerance;
+    const int q = nc + 1;
+    const int rows = 2 * nc + 5;
+    parallel_cells(ncell, threads, [&](int64_t c) {
+        double feed[NCMAX];
+        for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
+        Counters cnt;
+        auto V = std::make_unique<VleResult>();
+        auto R = std::make_unique<ClosureResult>();
+        try {
+            solve_vle(m, p[c], t[c], feed, so, nullptr, 0, *V, cnt);
+            build_closure(m, p[c], t[c], *V, so, min_fraction, min_rcond, *R);
+        } catch (...) {
+            R = std::make_unique<ClosureResult>();
+            R->thrown = true;
+        }
+        status[c] = R->thrown ? (CP_INDETERMINATE | (1 << 4) | 0x2000) : (R->primal | (R->lin << 4) | (R->reason << 8));
+        primal[0 * ncell + c] = R->zl;
+        primal[1 * ncell + c] = R->zv;
+        primal[2 * ncell + c] = R->rhol;
+        primal[3 * ncell + c] = R->rhov;
+        for (int k = 0; k < rows * q; ++k) { jac[k * ncell + c] = R->have_jac ? R->jac[k] : NaN; }
+        if (log_k_jac) {
+            for (int k = 0; k < nc * q; ++k) { log_k_jac[k * ncell + c] = R->have_dlk ? R->dlk[k] : NaN; }
+        }
+        if (diag) {
+            diag[0 * ncell + c] = R->rcond;
+            diag[1 * ncell + c] = R->backward;
+            diag[2 * ncell + c] = R->resid;
+        }
+    });
+    return 0;
 }

@@ -31,6 +31,17 @@ int cpuref_phase_batch(int nc, const double* tc, const double* pc, const double*
 int cpuref_rachford_rice(int nc, const double* z, const double* log_k, double* beta, double* x,
                          double* y, int32_t* iterations);
 
+/* Two-phase (VLE) flash of each cell followed by build_pr76_pt_vle_closure and
+ * differentiate_pr76_pt_vle: status primal | linearization << 4 | reason << 8 (0x2000 when
+ * the reference would throw), primal [4][n] (Z_L, Z_V, rho_L, rho_V), jac [2nc+5][nc+1][n]
+ * (beta_V, x_L[nc], y_V[nc], Z_L, Z_V, rho_L, rho_V over q = p, T, z_0..z_{nc-2}),
+ * log_k_jac [nc][nc+1][n] and diag [3][n] (rcond, backward error, residual) may be NULL. */
+int cpuref_closure_batch(int nc, const double* tc, const double* pc, const double* omega,
+                         const double* kij, int64_t ncell, const double* p, const double* t,
+                         const double* z, double fugacity_tolerance, double min_fraction,
+                         double min_rcond, int threads, int32_t* status, double* primal, double* jac,
+                         double* log_k_jac, double* diag);
+
 #ifdef __cplusplus
 }
 #endif

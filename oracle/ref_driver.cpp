@@ -11,9 +11,14 @@
 //                    (modules/flash/include/mpmc/flash/pr76_pt_flash_backend.hpp:204-213)
 //   max_phases == 2: solve_pr76_pt_vle + project_pr76_pt_phase_set
 //                    (modules/flash/include/mpmc/flash/pr76_phase_set.hpp:42-52)
+//   closure:         solve_pr76_pt_vle + build_pr76_pt_vle_closure and
+//                    differentiate_pr76_pt_vle
+//                    (modules/physics/include/mpmc/physics/pr76_thermodynamic_closure.hpp:148-381,
+//                     modules/flash/include/mpmc/flash/pr76_sensitivity.hpp:51-390)
 #include <mpmc/flash/pr76_max3_phase_set.hpp>
 #include <mpmc/flash/pr76_phase_set.hpp>
 #include <mpmc/flash/pr76_three_phase.hpp>
+#include <mpmc/physics/pr76_thermodynamic_closure.hpp>
 #include <mpmc/thermodynamics/pr76_phase.hpp>
 
 #include <algorithm>
@@ -29,6 +34,7 @@
 
 namespace th = mpmc::thermodynamics;
 namespace fl = mpmc::flash;
+namespace ph = mpmc::physics;
 
 namespace {
 
@@ -156,6 +162,77 @@ void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evalua
     (void)model;
 }
 
+// Closure of one cell in the layout of mpmc_closure_batch: status word
+// primal | linearization << 4 | reason << 8 (0x2000: the reference threw).
+struct ClosureOutputs {
+    int32_t* status;
+    double* primal;    // [4][ncell] Z_L, Z_V, rho_L, rho_V
+    double* jac;       // [2nc+5][nc+1][ncell]
+    double* log_k_jac; // [nc][nc+1][ncell]
+    double* diag;      // [3][ncell] rcond, backward error, residual norm
+};
+
+void closure_cell(fl::Pr76VleEvaluator& evaluator, int nc, int64_t ncell, int64_t c, const double* p,
+                  const double* t, const double* z, double fugacity_tolerance, double min_fraction,
+                  double min_rcond, const ClosureOutputs& out) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const int q = nc + 1;
+    const int rows = 2 * nc + 5;
+    for (int k = 0; k < 4; ++k) { out.primal[k * ncell + c] = nan; }
+    for (int k = 0; k < rows * q; ++k) { out.jac[k * ncell + c] = nan; }
+    if (out.log_k_jac) { for (int k = 0; k < nc * q; ++k) { out.log_k_jac[k * ncell + c] = nan; } }
+    if (out.diag) { for (int k = 0; k < 3; ++k) { out.diag[k * ncell + c] = nan; } }
+    std::vector<double> feed(nc);
+    for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
+    try {
+        fl::PtSplitOptions options;
+        options.iteration.fugacity_tolerance = fugacity_tolerance;
+        const auto split = fl::solve_pr76_pt_vle(p[c], t[c], feed, evaluator, options);
+        ph::Pr76PtVleClosureOptions copt;
+        copt.sensitivity.minimum_derivative_phase_fraction = min_fraction;
+        copt.sensitivity.minimum_reciprocal_condition = min_rcond;
+        const auto closure = ph::build_pr76_pt_vle_closure(split, evaluator, copt);
+        out.status[c] = static_cast<int32_t>(closure.primal_status) |
+                        (static_cast<int32_t>(closure.linearization_status) << 4) |
+                        (static_cast<int32_t>(closure.linearization_reason) << 8);
+        if (closure.primal) {
+            out.primal[0 * ncell + c] = closure.primal->liquid.compressibility_factor;
+            out.primal[1 * ncell + c] = closure.primal->vapor.compressibility_factor;
+            out.primal[2 * ncell + c] = closure.primal->liquid.molar_density_mol_per_m3;
+            out.primal[3 * ncell + c] = closure.primal->vapor.molar_density_mol_per_m3;
+        }
+        if (closure.linearization) {
+            const auto& L = *closure.linearization;
+            auto put = [&](int row, int col, double v) { out.jac[(row * q + col) * ncell + c] = v; };
+            for (int col = 0; col < q; ++col) {
+                put(0, col, L.d_vapor_fraction(col));
+                for (int i = 0; i < nc; ++i) {
+                    put(1 + i, col, L.d_liquid_composition(i, col));
+                    put(1 + nc + i, col, L.d_vapor_composition(i, col));
+                }
+                put(2 * nc + 1, col, L.d_liquid_compressibility(col));
+                put(2 * nc + 2, col, L.d_vapor_compressibility(col));
+                put(2 * nc + 3, col, L.d_liquid_molar_density(col));
+                put(2 * nc + 4, col, L.d_vapor_molar_density(col));
+            }
+        }
+        if (closure.primal && (out.log_k_jac || out.diag)) {
+            const auto sens = fl::differentiate_pr76_pt_vle(split, evaluator, copt.sensitivity);
+            const auto& s = sens.sensitivity;
+            if (out.diag) {
+                out.diag[0 * ncell + c] = s.equilibrium_jacobian_rcond;
+                out.diag[1 * ncell + c] = s.linear_solve_backward_error;
+                out.diag[2 * ncell + c] = s.equilibrium_residual_norm;
+            }
+            if (out.log_k_jac && s.status == fl::PtSensitivityStatus::success) {
+                for (int k = 0; k < nc * q; ++k) { out.log_k_jac[k * ncell + c] = s.log_k_jacobian[k]; }
+            }
+        }
+    } catch (const std::exception&) {
+        out.status[c] = 2 | (1 << 4) | 0x2000;
+    }
+}
+
 template <typename Body>
 void parallel_cells(int64_t ncell, int threads, Body body) {
     if (threads <= 0) { threads = static_cast<int>(std::max(1U, std::thread::hardware_concurrency())); }
@@ -232,6 +309,27 @@ int ref_phase_batch(int nc, const double* tc, const double* pc, const double* om
                 for (int i = 0; i < nc; ++i) { ln_phi[(k * nc + i) * ncell + c] = values.ln_phi[i]; }
             }
         }
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+int ref_closure_batch(int nc, const double* tc, const double* pc, const double* omega,
+                      const double* kij, int64_t ncell, const double* p, const double* t,
+                      const double* z, double fugacity_tolerance, double min_fraction, double min_rcond,
+                      int threads, int32_t* status, double* primal, double* jac, double* log_k_jac,
+                      double* diag) {
+    try {
+        const auto parameters = make_parameters(nc, tc, pc, omega, kij);
+        const auto model = th::Pr76Phase<double>::from_parameters(parameters);
+        const ClosureOutputs out{status, primal, jac, log_k_jac, diag};
+        parallel_cells(ncell, threads, [&](auto&& run) {
+            fl::Pr76VleEvaluator evaluator(model);
+            run([&](int64_t c) {
+                closure_cell(evaluator, nc, ncell, c, p, t, z, fugacity_tolerance, min_fraction, min_rcond, out);
+            });
+        });
         return 0;
     } catch (const std::exception&) {
         return 1;

@@ -116,3 +116,41 @@ def rachford_rice(z, log_k):
     it = ctypes.c_int32()
     st = L.cpuref_rachford_rice(n, _ptr(z), _ptr(lk), ctypes.byref(beta), _ptr(x), _ptr(y), ctypes.byref(it))
     return st, beta.value, x, y, it.value
+
+
+_CLOSURE_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_double, ctypes.c_double,
+                 ctypes.c_double, ctypes.c_int] + [_P] * 5
+
+
+def closure(mix, p, t, z, fugacity_tolerance=1e-11, min_fraction=1.490116119384765625e-8, min_rcond=0.0,
+            threads=0, which="cpuref"):
+    """Two-phase (VLE) flash of each cell followed by its thermodynamic closure.
+
+    Returns status word (primal | linearization << 4 | reason << 8), primal [4][n]
+    (Z_L, Z_V, rho_L, rho_V), jac [2nc+5][nc+1][n] (d of beta_V, x_L, y_V, Z_L, Z_V,
+    rho_L, rho_V over q = (p, T, z_0..z_{nc-2})), log_k_jac [nc][nc+1][n], diag [3][n]."""
+    L = lib(which)
+    if L is None:
+        raise RuntimeError(f"checker library '{which}' is not built")
+    tc, pc, om, kij = _mix_arrays(mix)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    nc, n = z.shape
+    q = nc + 1
+    out = {
+        "status": np.zeros(n, np.int32),
+        "primal": np.zeros((4, n)),
+        "jac": np.zeros((2 * nc + 5, q, n)),
+        "log_k_jac": np.zeros((nc, q, n)),
+        "diag": np.zeros((3, n)),
+    }
+    prefix = "cpuref" if which == "cpuref" else "ref"
+    fn = getattr(L, prefix + "_closure_batch")
+    fn.argtypes = _CLOSURE_ARGS
+    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), float(fugacity_tolerance),
+            float(min_fraction), float(min_rcond), int(threads), _ptr(out["status"]), _ptr(out["primal"]),
+            _ptr(out["jac"]), _ptr(out["log_k_jac"]), _ptr(out["diag"]))
+    if rc != 0:
+        raise RuntimeError(f"{prefix}_closure_batch failed rc={rc}")
+    return out
