@@ -379,28 +379,33 @@ __device__ __noinline__ int32_t closure_cell(const Mix<NC>& m, int64_t n, int64_
         return ((dx[i] / xt[i] - dy[i] / yt[i]) + lp) - vp;
     };
 
-    // ---- equilibrium Jacobian A = dF/dlogK and B = dF/dq (pr76_sensitivity.hpp:228-247)
+    // ---- partial derivatives of the RR-manifold outputs (beta, x, y) for each seed
+    // s = logK_0..logK_{nc-1}, z_0..z_{nc-2}: one unit seed at a time, so every partial
+    // is the same number the reference's Dual lane carries. From them the equilibrium
+    // Jacobian A = dF/dlogK and B = dF/dq (pr76_sensitivity.hpp:228-247).
+    constexpr int S = 2 * NC - 1;
+    double Pb[S], Px[NC * S], Py[NC * S];
     double Am[NC * NC], Bm[NC * Q];
     {
         double dlk[NC], dz[NC], dx[NC], dy[NC], db;
         #pragma unroll 1
-        for (int j = 0; j < NC; ++j) {
-            for (int i = 0; i < NC; ++i) { dlk[i] = i == j ? 1.0 : 0.0; dz[i] = 0.0; }
+        for (int sd = 0; sd < S; ++sd) {
+            for (int i = 0; i < NC; ++i) {
+                dlk[i] = i == sd ? 1.0 : 0.0;
+                dz[i] = sd < NC ? 0.0 : (i == sd - NC ? 1.0 : (i == NC - 1 ? -1.0 : 0.0));
+            }
             rr_map(dlk, dz, db, dx, dy);
-            for (int i = 0; i < NC; ++i) { Am[i * NC + j] = dF(i, dx, dy, 0.0, 0.0); }
+            Pb[sd] = db;
+            for (int i = 0; i < NC; ++i) {
+                Px[i * S + sd] = dx[i];
+                Py[i * S + sd] = dy[i];
+                const double f = dF(i, dx, dy, 0.0, 0.0);
+                if (sd < NC) { Am[i * NC + sd] = f; } else { Bm[i * Q + 2 + (sd - NC)] = f; }
+            }
         }
         for (int i = 0; i < NC; ++i) {
             Bm[i * Q + 0] = JL[i * Q + 0] - JV[i * Q + 0];
             Bm[i * Q + 1] = JL[i * Q + 1] - JV[i * Q + 1];
-        }
-        #pragma unroll 1
-        for (int mcol = 0; mcol + 1 < NC; ++mcol) {
-            for (int i = 0; i < NC; ++i) {
-                dlk[i] = 0.0;
-                dz[i] = i == mcol ? 1.0 : (i == NC - 1 ? -1.0 : 0.0);
-            }
-            rr_map(dlk, dz, db, dx, dy);
-            for (int i = 0; i < NC; ++i) { Bm[i * Q + 2 + mcol] = dF(i, dx, dy, 0.0, 0.0); }
         }
     }
     bool fin_all = true;
@@ -469,12 +474,23 @@ __device__ __noinline__ int32_t closure_cell(const Mix<NC>& m, int64_t n, int64_
     auto put = [&](int row, int col, double v) { out.jac[(static_cast<int64_t>(row) * Q + col) * n + c] = v; };
     #pragma unroll 1
     for (int col = 0; col < Q && ok; ++col) {
-        double dlk[NC], dz[NC], dx[NC], dy[NC], db;
+        // total = partial wrt q + sum_j partial wrt logK_j * dlogK_j/dq, in the reference's
+        // order (pr76_sensitivity.hpp:312-323); RR outputs have no direct p, T partial
+        double dx[NC], dy[NC], dz[NC];
+        const int qs = col < 2 ? -1 : NC + (col - 2);
+        double db = qs < 0 ? 0.0 : Pb[qs];
+        for (int j = 0; j < NC; ++j) { db += Pb[j] * Dm[j * Q + col]; }
         for (int i = 0; i < NC; ++i) {
-            dlk[i] = Dm[i * Q + col];
+            double vx = qs < 0 ? 0.0 : Px[i * S + qs];
+            double vy = qs < 0 ? 0.0 : Py[i * S + qs];
+            for (int j = 0; j < NC; ++j) {
+                vx += Px[i * S + j] * Dm[j * Q + col];
+                vy += Py[i * S + j] * Dm[j * Q + col];
+            }
+            dx[i] = vx;
+            dy[i] = vy;
             dz[i] = col < 2 ? 0.0 : (i == col - 2 ? 1.0 : (i == NC - 1 ? -1.0 : 0.0));
         }
-        rr_map(dlk, dz, db, dx, dy);
         double ls = 0, vs = 0;
         ok = ok && finite(db);
         for (int i = 0; i < NC; ++i) {

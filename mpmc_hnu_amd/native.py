@@ -88,6 +88,14 @@ def library() -> ctypes.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise MpmcError(f"{LIB_PATH} is not built: run `python -m mpmc_hnu_amd.build`")
+    # torch carries its own HIP runtime under the same soname as the system one. Whichever
+    # is loaded first serves the whole process, and torch cannot initialise its devices on
+    # the other; importing torch first keeps one runtime shared by torch's tensors and
+    # this library's kernels.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(LIB_PATH))
     P = ctypes.c_void_p
     lib.mpmc_mixture_create.argtypes = [ctypes.c_int, P, P, P, P, ctypes.POINTER(P)]
