@@ -1,5 +1,5 @@
-"""Write tests/golden/ref_flash_vectors.npz: seeded inputs and the reference's own
-flash results for them, computed by the reference's header-only flash built from its
+"""Write tests/golden/ref_flash_vectors.npz and ref_closure_vectors.npz: seeded inputs
+and the reference's own flash and closure results for them, computed by the reference's header-only flash built from its
 checkout (oracle/_ref/libmpmc_ref.so, via oracle/ref_driver.cpp).
 
     make -C oracle ref && python tests/golden/make_golden.py
@@ -52,6 +52,18 @@ def main():
         data[f"{name}/out/e_prop"] = res["counters"][0]
         print(name, z.shape, np.unique(res["status"], return_counts=True))
     np.savez_compressed(HERE / "ref_flash_vectors.npz", **data)
+    closures = {}
+    for name, nc, n, seed in (("cl2", 2, 96, 72), ("cl3", 3, 96, 73), ("cl6", 6, 128, 76), ("cl12", 12, 128, 82)):
+        mix = W.mixture_linspaced(nc)
+        p, t, z = W.cells_uniform(nc, n, seed=seed)
+        res = ob.closure(mix, p, t, z, which="ref")
+        for k in ("tc", "pc", "omega", "kij"):
+            closures[f"{name}/{k}"] = np.asarray(mix[k], dtype=np.float64)
+        closures[f"{name}/p"], closures[f"{name}/t"], closures[f"{name}/z"] = p, t, z
+        for k in ("status", "primal", "jac", "log_k_jac"):
+            closures[f"{name}/out/{k}"] = res[k]
+        print(name, np.unique(res["status"], return_counts=True))
+    np.savez_compressed(HERE / "ref_closure_vectors.npz", **closures)
 
 
 if __name__ == "__main__":

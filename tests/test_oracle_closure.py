@@ -9,6 +9,8 @@ differentiate) pinned on the CPU:
   sum(dx) = 0 in each column, and the differentiated material balance;
 * against the reference's status policy tests (closure_test.cpp:264-317).
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -143,3 +145,21 @@ def test_oracle_closure_is_the_reference_closure(nc):
     assert ok.sum() > 5
     assert np.array_equal(a["log_k_jac"][:, :, ok], b["log_k_jac"][:, :, ok])
     assert np.array_equal(a["diag"][:, ok], b["diag"][:, ok])
+
+
+CLOSURE_VECTORS = Path(__file__).resolve().parent / "golden" / "ref_closure_vectors.npz"
+
+
+@pytest.mark.parametrize("name", ["cl2", "cl3", "cl6", "cl12"])
+def test_oracle_closure_matches_reference_vectors(name):
+    """Against closures the reference computed for seeded cells (tests/golden/make_golden.py):
+    bit for bit, without needing the reference build."""
+    d = np.load(CLOSURE_VECTORS)
+    mix = {k: d[f"{name}/{k}"] for k in ("tc", "pc", "omega", "kij")}
+    r = ob.closure(mix, d[f"{name}/p"], d[f"{name}/t"], d[f"{name}/z"])
+    assert np.array_equal(r["status"], d[f"{name}/out/status"])
+    assert np.array_equal(r["primal"], d[f"{name}/out/primal"], equal_nan=True)
+    assert np.array_equal(r["jac"], d[f"{name}/out/jac"], equal_nan=True)
+    ok = r["status"] == 0
+    assert ok.any()
+    assert np.array_equal(r["log_k_jac"][:, :, ok], d[f"{name}/out/log_k_jac"][:, :, ok])
