@@ -51,8 +51,9 @@ enum mpmc_split_status { MPMC_SPLIT_SINGLE = 0, MPMC_SPLIT_TWO = 1, MPMC_SPLIT_U
 #define MPMC_STATUS_SET(s) ((s) & 0xF)
 #define MPMC_STATUS_MAX3(s) (((s) >> 4) & 0xF)
 #define MPMC_STATUS_SPLIT(s) (((s) >> 8) & 0xF)
-/* flag: the cell hit a fixed-capacity limit of the device path (see DESIGN). */
-#define MPMC_FLAG_CAPACITY 0x1000
+/* flag: the reference rejects this cell's input with an exception (invalid_argument /
+ * domain_error); no phase set is published. */
+#define MPMC_FLAG_REJECTED 0x2000
 
 /* branch word per phase: provider root index | smooth << 8 ; -1 when the slot is empty */
 #define MPMC_BRANCH_INDEX(b) ((b) & 0xFF)
