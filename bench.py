@@ -67,6 +67,41 @@ def cpu_baseline(mix, p, t, z, sample: int):
     }
 
 
+def serve_leg(seconds: float) -> dict:
+    """Serving workload: Poisson arrivals at 200 jobs/s, cells per job log-uniform on
+    [1, 1e6], nc uniform in 2..10, through the continuous-batching server, FP64 and mixed."""
+    from mpmc_hnu_amd import serving as S, workloads as W
+
+    print(f"[bench] serving: generating {seconds} s of arrivals", file=sys.stderr, flush=True)
+    jobs = S.serving_jobs(seconds)
+    mixtures = {nc: W.mixture_linspaced(nc) for nc in range(2, 11)}
+    out = {"arrival_rate_per_s": 200.0, "arrival_seconds": seconds, "jobs": len(jobs),
+           "cells": int(sum(j["p"].size for j in jobs)), "small_job_cells": 65536,
+           "warm_starts": "not modelled (no continuation hints in this build)"}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        srv = S.Server(slots=2, precision=prec)
+        t0 = time.perf_counter()
+        res = S.run_load(srv, jobs, mixtures)
+        makespan = time.perf_counter() - t0
+        srv.close()
+        n = np.array([r[0] for r in res])
+        lat = np.array([r[1] for r in res]) * 1e3
+        small = n <= 65536
+        out[name] = {
+            "p50_ms_small": float(np.percentile(lat[small], 50)),
+            "p99_ms_small": float(np.percentile(lat[small], 99)),
+            "p50_ms_all": float(np.percentile(lat, 50)),
+            "p99_ms_all": float(np.percentile(lat, 99)),
+            "cells_per_s": float(n.sum() / makespan),
+            "makespan_s": makespan,
+            "slots": srv.slots,
+        }
+        print(f"[bench] serving {name}: p50/p99 small {out[name]['p50_ms_small']:.2f}/{out[name]['p99_ms_small']:.2f} ms, "
+              f"all {out[name]['p50_ms_all']:.1f}/{out[name]['p99_ms_all']:.1f} ms, {out[name]['cells_per_s']:.3e} cells/s",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +110,8 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, default=None, help="prefix of the 1,048,576-cell grid (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="seconds of Poisson arrivals for the serving leg (0: skip)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -158,6 +195,9 @@ def main(argv=None):
     counts = {int(k): int(v) for k, v in zip(*np.unique(host["phase_count"][(status & 0xF) == 0], return_counts=True))}
     print(f"[bench] mixed vs fp64: status diffs {status_diff}, phase-count diffs {count_diff}, "
           f"max rel diff {rel:.3e}; accepted {accepted}, phase counts {counts}", file=sys.stderr, flush=True)
+    serving = None
+    if a.serve_seconds > 0:
+        serving = serve_leg(a.serve_seconds)
     cpu = None
     if not a.no_cpu:
         print("[bench] timing CPU baseline", file=sys.stderr, flush=True)
@@ -208,6 +248,7 @@ def main(argv=None):
             "algorithmic_flop_per_launch": flops,
             "algorithmic_bytes_per_launch": algorithmic_bytes(nc, ncell),
         },
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

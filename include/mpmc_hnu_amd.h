@@ -206,6 +206,59 @@ int mpmc_closure_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, c
                        const mpmc_closure_options* options, const mpmc_closure_outputs* out,
                        void* stream);
 
+/* ---------------------------------------------------------------- serving
+ * Continuous-batching flash server: the PT service request loop (reference
+ * modules/runtime/include/mpmc/runtime/pt_service.hpp, PtFlashBackend::solve per request)
+ * on one device. Jobs are admitted as they arrive; each launch packs cells of all pending
+ * jobs on one mixture (small jobs first, then large ones in arrival order); `slots`
+ * launches are in flight at once on the one launch stream. Device memory is
+ * allocated once at creation (one flash workspace and per-slot chunk buffers, nc <= 12)
+ * and the slot count is reduced until it fits hbm_cap_bytes. Mixtures are kept resident
+ * per parameter set. Results are those of mpmc_flash_batch on the job's cells. */
+typedef struct mpmc_server* mpmc_server_t;
+
+typedef struct mpmc_server_options {
+    int32_t slots;             /* chunks in flight, default 2 */
+    int32_t precision;         /* as mpmc_flash_options.precision */
+    int32_t max_phases;        /* 2 or 3, default 3 */
+    int32_t reserved0;
+    double fugacity_tolerance; /* default 1e-11 */
+    int64_t chunk_cells;       /* cells per launch, default 262144 */
+    int64_t small_job_cells;   /* jobs up to this size are served first, and launches are capped
+                                * at this size while such jobs wait; default 65536 */
+    size_t hbm_cap_bytes;      /* device memory the server may hold, default 200 GiB */
+    void* stream;              /* launch stream (hipStream_t), NULL = the null stream */
+} mpmc_server_options;
+
+/* Host result buffers of one job, same SoA layout as mpmc_flash_outputs with n = ncell. */
+typedef struct mpmc_host_outputs {
+    int32_t* status;
+    int32_t* phase_count;
+    double* beta;
+    double* zfactor;
+    int32_t* branch;
+    double* x;
+    double* ln_phi;
+} mpmc_host_outputs;
+
+void mpmc_server_default_options(mpmc_server_options* options);
+int mpmc_server_create(const mpmc_server_options* options, mpmc_server_t* out);
+/* Drains every submitted job, then releases the device memory. */
+int mpmc_server_destroy(mpmc_server_t server);
+int mpmc_server_slots(mpmc_server_t server);
+size_t mpmc_server_device_bytes(mpmc_server_t server);
+/* Queue one job. p[ncell], t[ncell], z[nc][ncell] and the outputs are host memory that must
+ * stay valid until the job is done. Returns at once with the job id. */
+int mpmc_server_submit(mpmc_server_t server, int nc, const double* tc_k, const double* pc_pa,
+                       const double* omega, const double* kij, int64_t ncell, const double* p,
+                       const double* t, const double* z, const mpmc_host_outputs* out,
+                       int64_t* job_id);
+/* Block until the job is done; latency_s = completion - submission (seconds). */
+int mpmc_server_wait(mpmc_server_t server, int64_t job_id, double* latency_s);
+int mpmc_server_poll(mpmc_server_t server, int64_t job_id, int32_t* done, double* latency_s);
+/* Last error text of a server call on this thread (never NULL). */
+const char* mpmc_server_last_error(void);
+
 /* Last error text for this thread (never NULL). */
 const char* mpmc_last_error(void);
 

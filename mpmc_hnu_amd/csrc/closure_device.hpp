@@ -633,7 +633,9 @@ hipError_t launch_closure(const double* mixc, void* ws, size_t ws_bytes, int64_t
             per_cu < 1) {
             per_cu = 1;
         }
-        return static_cast<unsigned>((cus < 1 ? 1 : cus) * per_cu);
+        // 13 KB of private memory per lane at nc = 12: cap the grid (256 waves) so one
+        // dispatch reserves little scratch; the loop strides over the eligible cells
+        return std::min(static_cast<unsigned>((cus < 1 ? 1 : cus) * per_cu), 256u);
     }();
     hipLaunchKernelGGL(k_closure_zero, dim3(1), dim3(64), 0, s, count);
     hipLaunchKernelGGL(k_closure_classify<NC>, dim3(blocks), dim3(64), 0, s, n, in, out, list, count);

@@ -1089,8 +1089,12 @@ template <int NC, bool kN>
 hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, hipStream_t s) {
     static const unsigned b_tpd = persistent_blocks(k_tpd<NC, kN, false, kN>);
     static const unsigned b_split = persistent_blocks(k_split<NC, kN>);
-    static const unsigned b_tail = persistent_blocks(k_max3_tail<NC, kN>);
-    static const unsigned b_fb = persistent_blocks(k_fallback<NC, kN>);
+    // The three-phase tail and the serial fallback carry 8-22 KB of private memory per lane
+    // and serve few cells; their grids are capped so that one dispatch reserves at most
+    // ~100 MB of scratch (8192 and 1024 lanes), which keeps several launch streams of
+    // such dispatches within the runtime's scratch limits.
+    static const unsigned b_tail = std::min(persistent_blocks(k_max3_tail<NC, kN>), 128u);
+    static const unsigned b_fb = std::min(persistent_blocks(k_fallback<NC, kN>), 16u);
     const int64_t n = P.n;
     const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);

@@ -28,6 +28,15 @@ EXPORTED = (
     "mpmc_rr_batch",
     "mpmc_closure_default_options",
     "mpmc_closure_batch",
+    "mpmc_server_default_options",
+    "mpmc_server_create",
+    "mpmc_server_destroy",
+    "mpmc_server_slots",
+    "mpmc_server_device_bytes",
+    "mpmc_server_submit",
+    "mpmc_server_wait",
+    "mpmc_server_poll",
+    "mpmc_server_last_error",
     "mpmc_last_error",
 )
 
@@ -74,6 +83,24 @@ class ClosureOutputs(ctypes.Structure):
     ]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("max_phases", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("fugacity_tolerance", ctypes.c_double),
+        ("chunk_cells", ctypes.c_int64),
+        ("small_job_cells", ctypes.c_int64),
+        ("hbm_cap_bytes", ctypes.c_size_t),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+class HostOutputs(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("status", "phase_count", "beta", "zfactor", "branch", "x", "ln_phi")]
+
+
 class MpmcError(RuntimeError):
     pass
 
@@ -116,6 +143,18 @@ def library() -> ctypes.CDLL:
     lib.mpmc_closure_default_options.restype = None
     lib.mpmc_closure_batch.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOutputs),
                                        ctypes.POINTER(ClosureOptions), ctypes.POINTER(ClosureOutputs), P]
+    lib.mpmc_server_default_options.argtypes = [ctypes.POINTER(ServerOptions)]
+    lib.mpmc_server_default_options.restype = None
+    lib.mpmc_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]
+    lib.mpmc_server_destroy.argtypes = [P]
+    lib.mpmc_server_slots.argtypes = [P]
+    lib.mpmc_server_device_bytes.argtypes = [P]
+    lib.mpmc_server_device_bytes.restype = ctypes.c_size_t
+    lib.mpmc_server_submit.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int64, P, P, P,
+                                       ctypes.POINTER(HostOutputs), ctypes.POINTER(ctypes.c_int64)]
+    lib.mpmc_server_wait.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    lib.mpmc_server_poll.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]
+    lib.mpmc_server_last_error.restype = ctypes.c_char_p
     lib.mpmc_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
