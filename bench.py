@@ -67,6 +67,58 @@ def cpu_baseline(mix, p, t, z, sample: int):
     }
 
 
+def assembly_leg(dims, steps: int, warmup: int) -> dict:
+    """One Newton evaluation of the two-phase natural-variable system (residual + block
+    Jacobian) on a structured grid, nc=6, inputs resident in HBM."""
+    import torch
+
+    from mpmc_hnu_amd import assembly as A, flash, workloads as W
+
+    nc = 6
+    mix = W.mixture_linspaced(nc)
+    grid = W.tpfa_grid(*dims)
+    m = flash.Mixture.from_dict(mix)
+
+    def roots(p, t, x):
+        r = flash.to_host(flash.phase_batch(m, p, t, x))
+        return r["root_count"], r["root_flags"]
+
+    print(f"[bench] assembly: building state on {dims}", file=sys.stderr, flush=True)
+    q, br = W.two_phase_state(nc, grid["ncell"], roots, seed=8, dims=dims)
+    g = A.Grid(grid)
+    md = A.default_model(nc)
+    inv, st, flag = A.inventory(m, g, md, q, br)
+    prev = inv * 0.99
+    dt = 86400.0
+    out = A.assemble(m, g, md, q, br, prev, dt)
+    assert out["domain_error"] == 0, "assembly state raised a domain error"
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        A.assemble(m, g, md, q, br, prev, dt, out=out, check_domain=False)
+    torch.cuda.synchronize()
+    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    e1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    qd, bd, pd = out["_inputs"]
+    for k in range(steps):
+        e0[k].record(stream)
+        A.assemble(m, g, md, qd, bd, pd, dt, out=out, check_domain=False)
+        e1[k].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
+    ncell, nface, Q = grid["ncell"], grid["owner"].size, 2 * nc + 1
+    nnz = ncell * Q * Q + 2 * nface * (nc + 1) * Q
+    algo_bytes = 8.0 * (ncell * (Q + nc + 3) + nface * 2 + nnz + ncell * Q)
+    gbs = algo_bytes / (ms * 1e-3) / 1e9
+    print(f"[bench] assembly: {ms:.3f} ms per Newton evaluation, {gbs:.0f} GB/s algorithmic "
+          f"({ncell} cells, {nface} faces)", file=sys.stderr, flush=True)
+    g.close()
+    m.close()
+    return {"grid": list(dims), "cells": ncell, "faces": int(nface), "unknowns_per_cell": Q, "ms": ms,
+            "algorithmic_bytes": algo_bytes, "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS,
+            "frac": gbs / PEAK_HBM_GBS,
+            "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables"}
+
+
 def serve_leg(seconds: float) -> dict:
     """Serving workload: Poisson arrivals at 200 jobs/s, cells per job log-uniform on
     [1, 1e6], nc uniform in 2..10, through the continuous-batching server, FP64 and mixed."""
@@ -110,6 +162,8 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, default=None, help="prefix of the 1,048,576-cell grid (default: all)")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--assembly-dims", type=str, default="100,100,50",
+                    help="grid of the assembly leg (nx,ny,nz; empty: skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals for the serving leg (0: skip)")
     a = ap.parse_args(argv)
@@ -195,6 +249,9 @@ def main(argv=None):
     counts = {int(k): int(v) for k, v in zip(*np.unique(host["phase_count"][(status & 0xF) == 0], return_counts=True))}
     print(f"[bench] mixed vs fp64: status diffs {status_diff}, phase-count diffs {count_diff}, "
           f"max rel diff {rel:.3e}; accepted {accepted}, phase counts {counts}", file=sys.stderr, flush=True)
+    asm = None
+    if a.assembly_dims:
+        asm = assembly_leg(tuple(int(v) for v in a.assembly_dims.split(",")), a.steps, a.warmup)
     serving = None
     if a.serve_seconds > 0:
         serving = serve_leg(a.serve_seconds)
@@ -248,6 +305,7 @@ def main(argv=None):
             "algorithmic_flop_per_launch": flops,
             "algorithmic_bytes_per_launch": algorithmic_bytes(nc, ncell),
         },
+        "assembly": asm,
         "serving": serving,
         "cpu_baseline": cpu,
     }

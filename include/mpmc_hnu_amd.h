@@ -206,6 +206,71 @@ int mpmc_closure_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, c
                        const mpmc_closure_options* options, const mpmc_closure_outputs* out,
                        void* stream);
 
+/* ---------------------------------------------------------------- two-phase assembly
+ * Newton-iteration residual and Jacobian of the two-phase natural-variable system:
+ * TwoPhaseSnesAssemblyContext3D::evaluate_complete_assembly
+ * (modules/flow/discretization/petsc/include/mpmc/flow_discretization_petsc/
+ *  two_phase_snes_assembly.hpp:550-1090) on one rank; PETSc SNES/KSP stay on the host.
+ *
+ * Per cell q = (p, T, S_0, x_0 independent [nc-1], x_1 independent [nc-1]), Q = 2nc+1,
+ * the last component of each phase dependent (two_phase_cell_state.hpp:120-231). Rows per
+ * cell: nc component balances, energy, nc fugacity-equilibrium rows
+ * ln(x0_i/x1_i) + ln phi0_i - ln phi1_i. Phase properties come from the PR76 selected
+ * branch of each phase (root index per cell and phase, the flash's branch).
+ *
+ * Transport, caloric, rock and relative-permeability models are caller-owned in the
+ * reference (pr76_selected_phase_property_closure.hpp:32-45,
+ * selected_phase_production_cell_evaluator.hpp:24-64); here they are these closed forms:
+ *   mu = mu_a0 + mu_a1 rho_mass^2, h = cp (T - t0), U_rock = cv_rock (T - t0), kr_p = S_p^2.
+ *
+ * Layouts (device memory, cell or face index fastest):
+ *   q[Q][ncell], branch[2][ncell], prev[nc+1][ncell] (N_i and E of the previous time),
+ *   residual[Q][ncell], diag[Q*Q][ncell] (row-major Q x Q block of the cell's own columns),
+ *   offdiag[(nc+1)*Q][2 nface]: the block (rows of cell c, columns of its neighbour across
+ *   adjacency entry a) at column a, where a runs over the grid's cell -> face CSR
+ *   (mpmc_tpfa_grid_adjacency); equilibrium rows have no off-diagonal entries. Together:
+ *   a block-CSR matrix with Q x Q blocks. */
+typedef struct mpmc_tpfa_grid* mpmc_tpfa_grid_t;
+
+/* Internal faces of a mesh: owner/neighbour cells, T_f [m3] (harmonic combination of the
+ * half transmissibilities, tpfa_static_face_transmissibility_3d.hpp:125-182), gdot = g . (x_n
+ * - x_o) [m2/s2], thermal conductance [W/K]; per cell bulk volume [m3] and porosity. Device
+ * arrays, referenced (not copied) until destroy. */
+int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk_volume, const double* porosity,
+                          const int32_t* owner, const int32_t* neighbour, const double* transmissibility,
+                          const double* gdot, const double* thermal_conductance, mpmc_tpfa_grid_t* out);
+int mpmc_tpfa_grid_destroy(mpmc_tpfa_grid_t grid);
+/* The grid's cell -> face CSR: adj_off[ncell+1], adj_face[2 nface] (device memory). */
+int mpmc_tpfa_grid_adjacency(mpmc_tpfa_grid_t grid, const int32_t** adj_off, const int32_t** adj_face);
+
+typedef struct mpmc_assembly_model {
+    double mu_a0, mu_a1; /* Pa s, Pa s m6/kg2 */
+    double cp;           /* J/(kg K) */
+    double t0;           /* K */
+    double cv_rock;      /* J/(m3 K) */
+    double molar_mass[MPMC_MAX_COMPONENTS]; /* kg/mol */
+} mpmc_assembly_model;
+
+typedef struct mpmc_assembly_outputs {
+    int32_t* cell_status; /* [ncell] 0, or 1 where the cell's evaluation raised a domain error */
+    double* residual;     /* [Q][ncell] */
+    double* diag;         /* [Q*Q][ncell] */
+    double* offdiag;      /* [(nc+1)*Q][2 nface] */
+} mpmc_assembly_outputs;
+
+void mpmc_assembly_default_model(int nc, mpmc_assembly_model* model);
+/* One Newton evaluation. domain_error (optional host pointer): synchronises the stream and
+ * reports whether any cell raised a domain error (the reference's domain_error status; the
+ * outputs are then not a valid assembly). */
+int mpmc_assembly_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
+                     const double* q, const int32_t* branch, const double* prev, double dt,
+                     const mpmc_assembly_outputs* out, int32_t* domain_error, void* stream);
+/* Pore-volume inventories N_i and E at a state into inventory[nc+1][ncell] (the previous-time
+ * input of mpmc_assembly_2p). */
+int mpmc_inventory_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
+                      const double* q, const int32_t* branch, int32_t* cell_status, double* inventory,
+                      int32_t* domain_error, void* stream);
+
 /* ---------------------------------------------------------------- serving
  * Continuous-batching flash server: the PT service request loop (reference
  * modules/runtime/include/mpmc/runtime/pt_service.hpp, PtFlashBackend::solve per request)

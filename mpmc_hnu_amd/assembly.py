@@ -1,0 +1,126 @@
+"""Two-phase natural-variable residual/Jacobian assembly on the GPU (mpmc_tpfa_grid_*,
+mpmc_assembly_2p, mpmc_inventory_2p in include/mpmc_hnu_amd.h): the per-Newton-iteration
+work of the reference's TwoPhaseSnesAssemblyContext3D::evaluate_complete_assembly
+(modules/flow/discretization/petsc/include/mpmc/flow_discretization_petsc/
+two_phase_snes_assembly.hpp:550-1090); PETSc stays on the host."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+from .flash import Mixture, _dev, _stream_ptr, _torch
+
+V = ctypes.c_void_p
+
+
+def default_model(nc: int, molar_mass=None) -> native.AssemblyModel:
+    md = native.AssemblyModel()
+    native.library().mpmc_assembly_default_model(nc, ctypes.byref(md))
+    if molar_mass is not None:
+        for i, v in enumerate(molar_mass):
+            md.molar_mass[i] = float(v)
+    return md
+
+
+def model_array(md: native.AssemblyModel, nc: int) -> np.ndarray:
+    """The model as the oracle takes it: mu_a0, mu_a1, cp, t0, cv_rock, molar_mass[nc]."""
+    return np.array([md.mu_a0, md.mu_a1, md.cp, md.t0, md.cv_rock] + [md.molar_mass[i] for i in range(nc)])
+
+
+class Grid:
+    """Device-resident TPFA grid (faces, transmissibilities, bulk volume, porosity)."""
+
+    def __init__(self, g: dict):
+        torch = _torch()
+        lib = native.library()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.ncell, self.nface = int(g["ncell"]), int(g["owner"].size)
+        self.t = {k: _dev(g[k], torch.float64, dev) for k in ("bulk", "poro", "trans", "gdot", "gth")}
+        self.t["owner"] = _dev(g["owner"], torch.int32, dev)
+        self.t["neighbour"] = _dev(g["neighbour"], torch.int32, dev)
+        self.h = V()
+        t = self.t
+        native.check(lib.mpmc_tpfa_grid_create(self.ncell, self.nface, V(t["bulk"].data_ptr()), V(t["poro"].data_ptr()),
+                                               V(t["owner"].data_ptr()), V(t["neighbour"].data_ptr()),
+                                               V(t["trans"].data_ptr()), V(t["gdot"].data_ptr()),
+                                               V(t["gth"].data_ptr()), ctypes.byref(self.h)), "tpfa_grid_create")
+
+    def close(self):
+        if self.h:
+            native.library().mpmc_tpfa_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def adjacency_host(owner, neighbour, ncell: int):
+    """The cell -> face CSR the library builds (faces of a cell in face order, each face
+    listed once under its owner and once under its neighbour)."""
+    owner = np.asarray(owner)
+    neighbour = np.asarray(neighbour)
+    off = np.zeros(ncell + 1, np.int64)
+    np.add.at(off, owner + 1, 1)
+    np.add.at(off, neighbour + 1, 1)
+    off = np.cumsum(off)
+    fill = off[:-1].copy()
+    adj = np.zeros(2 * owner.size, np.int64)
+    for f in range(owner.size):
+        adj[fill[owner[f]]] = f
+        fill[owner[f]] += 1
+        adj[fill[neighbour[f]]] = f
+        fill[neighbour[f]] += 1
+    return off, adj
+
+
+def allocate(nc: int, grid: Grid, device) -> dict:
+    torch = _torch()
+    Q = 2 * nc + 1
+    return {
+        "cell_status": torch.empty(grid.ncell, dtype=torch.int32, device=device),
+        "residual": torch.empty((Q, grid.ncell), dtype=torch.float64, device=device),
+        "diag": torch.empty((Q * Q, grid.ncell), dtype=torch.float64, device=device),
+        "offdiag": torch.empty(((nc + 1) * Q, max(1, 2 * grid.nface)), dtype=torch.float64, device=device),
+    }
+
+
+def assemble(mixture: Mixture, grid: Grid, model, q, branch, prev, dt: float, out: dict | None = None,
+             stream=None, check_domain: bool = True):
+    torch = _torch()
+    lib = native.library()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    q = _dev(q, torch.float64, dev)
+    branch = _dev(branch, torch.int32, dev)
+    prev = _dev(prev, torch.float64, dev)
+    if out is None:
+        out = allocate(mixture.nc, grid, dev)
+    o = native.AssemblyOutputs(out["cell_status"].data_ptr(), out["residual"].data_ptr(), out["diag"].data_ptr(),
+                               out["offdiag"].data_ptr())
+    flag = ctypes.c_int32(0)
+    native.check(lib.mpmc_assembly_2p(mixture.handle, grid.h, ctypes.byref(model), V(q.data_ptr()),
+                                      V(branch.data_ptr()), V(prev.data_ptr()), float(dt), ctypes.byref(o),
+                                      ctypes.byref(flag) if check_domain else None, _stream_ptr(stream)),
+                 "assembly_2p")
+    out["_inputs"] = (q, branch, prev)
+    out["domain_error"] = int(flag.value) if check_domain else None
+    return out
+
+
+def inventory(mixture: Mixture, grid: Grid, model, q, branch, stream=None):
+    torch = _torch()
+    lib = native.library()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    q = _dev(q, torch.float64, dev)
+    branch = _dev(branch, torch.int32, dev)
+    status = torch.empty(grid.ncell, dtype=torch.int32, device=dev)
+    inv = torch.empty((mixture.nc + 1, grid.ncell), dtype=torch.float64, device=dev)
+    flag = ctypes.c_int32(0)
+    native.check(lib.mpmc_inventory_2p(mixture.handle, grid.h, ctypes.byref(model), V(q.data_ptr()),
+                                       V(branch.data_ptr()), V(status.data_ptr()), V(inv.data_ptr()),
+                                       ctypes.byref(flag), _stream_ptr(stream)), "inventory_2p")
+    return inv, status, int(flag.value)

@@ -1,6 +1,7 @@
 // C ABI of include/mpmc_hnu_amd.h: mixture residency, batched flash and batched
 // phase evaluation. Argument checks mirror the reference's input contracts; numerical
 // outcomes are per-cell statuses written by the kernels.
+#include "assembly_device.hpp"
 #include "closure_device.hpp"
 #include "flash_pipeline.hpp"
 
@@ -12,12 +13,14 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 namespace mpmc::dev {
 MPMC_FOR_EACH_NC(MPMC_DECLARE_LAUNCH)
 MPMC_FOR_EACH_NC(MPMC_DECLARE_PIPELINE)
 MPMC_FOR_EACH_NC2(MPMC_DECLARE_CLOSURE)
+MPMC_FOR_EACH_NC2(MPMC_DECLARE_ASSEMBLY)
 } // namespace mpmc::dev
 
 namespace {
@@ -367,6 +370,186 @@ int mpmc_closure_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const d
     default: return fail(MPMC_E_INVALID_ARGUMENT, "closure_batch: unsupported nc");
     }
     return e == hipSuccess ? MPMC_OK : hip_fail(e, "closure_batch launch");
+}
+
+// ---------------------------------------------------------------- two-phase assembly
+struct mpmc_tpfa_grid {
+    int64_t ncell, nface;
+    const double *bulk, *poro, *trans, *gdot, *gth;
+    const int32_t *owner, *neighbour;
+    int32_t* adj_off = nullptr;  // device CSR cell -> faces
+    int32_t* adj_face = nullptr;
+    void* rec = nullptr;         // per-cell face records
+    size_t rec_bytes = 0;
+    unsigned long long* flag = nullptr;
+};
+
+int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk, const double* poro, const int32_t* owner,
+                          const int32_t* neighbour, const double* trans, const double* gdot, const double* gth,
+                          mpmc_tpfa_grid_t* out) {
+    if (!out) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_create: null handle"); }
+    *out = nullptr;
+    if (ncell < 1 || nface < 0 || ncell > (int64_t{1} << 30) || nface > (int64_t{1} << 30)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_create: bad sizes");
+    }
+    const void* ptrs[] = {bulk, poro, trans, gdot, gth, owner, neighbour};
+    for (const void* ptr : ptrs) {
+        if (nface == 0 && (ptr == trans || ptr == gdot || ptr == gth || ptr == owner || ptr == neighbour)) { continue; }
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_create: arrays must be device memory"); }
+    }
+    // cell -> face adjacency, faces of each cell in face order (host counting sort, once per grid)
+    std::vector<int32_t> ho(static_cast<size_t>(nface)), hn(static_cast<size_t>(nface));
+    hipError_t e = hipSuccess;
+    if (nface > 0) {
+        e = hipMemcpy(ho.data(), owner, sizeof(int32_t) * nface, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) { e = hipMemcpy(hn.data(), neighbour, sizeof(int32_t) * nface, hipMemcpyDeviceToHost); }
+        if (e != hipSuccess) { return hip_fail(e, "tpfa_grid_create"); }
+    }
+    std::vector<int32_t> off(static_cast<size_t>(ncell) + 1, 0), adj(static_cast<size_t>(2 * nface));
+    for (int64_t f = 0; f < nface; ++f) {
+        if (ho[f] < 0 || ho[f] >= ncell || hn[f] < 0 || hn[f] >= ncell || ho[f] == hn[f]) {
+            return fail(MPMC_E_DOMAIN, "tpfa_grid_create: face cells out of range");
+        }
+        ++off[ho[f] + 1];
+        ++off[hn[f] + 1];
+    }
+    for (int64_t c = 0; c < ncell; ++c) { off[c + 1] += off[c]; }
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t f = 0; f < nface; ++f) {
+        adj[fill[ho[f]]++] = static_cast<int32_t>(f);
+        adj[fill[hn[f]]++] = static_cast<int32_t>(f);
+    }
+    auto* g = new (std::nothrow) mpmc_tpfa_grid{ncell, nface, bulk, poro, trans, gdot, gth, owner, neighbour};
+    if (!g) { return fail(MPMC_E_NO_MEMORY, "tpfa_grid_create: host allocation"); }
+    e = hipMalloc(&g->adj_off, sizeof(int32_t) * off.size());
+    if (e == hipSuccess) { e = hipMalloc(&g->adj_face, sizeof(int32_t) * std::max<size_t>(1, adj.size())); }
+    if (e == hipSuccess) { e = hipMalloc(&g->flag, 256); }
+    if (e == hipSuccess) { e = hipMemcpy(g->adj_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice); }
+    if (e == hipSuccess && !adj.empty()) {
+        e = hipMemcpy(g->adj_face, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        mpmc_tpfa_grid_destroy(g);
+        return hip_fail(e, "tpfa_grid_create");
+    }
+    *out = g;
+    return MPMC_OK;
+}
+
+int mpmc_tpfa_grid_destroy(mpmc_tpfa_grid_t g) {
+    if (!g) { return MPMC_OK; }
+    for (void* ptr : {static_cast<void*>(g->adj_off), static_cast<void*>(g->adj_face), g->rec,
+                      static_cast<void*>(g->flag)}) {
+        if (ptr) { (void)hipFree(ptr); }
+    }
+    delete g;
+    return MPMC_OK;
+}
+
+int mpmc_tpfa_grid_adjacency(mpmc_tpfa_grid_t g, const int32_t** adj_off, const int32_t** adj_face) {
+    if (!g || !adj_off || !adj_face) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_adjacency: null argument"); }
+    *adj_off = g->adj_off;
+    *adj_face = g->adj_face;
+    return MPMC_OK;
+}
+
+void mpmc_assembly_default_model(int nc, mpmc_assembly_model* md) {
+    if (!md) { return; }
+    std::memset(md, 0, sizeof(*md));
+    md->mu_a0 = 1.5e-5;
+    md->mu_a1 = 1.0e-9;
+    md->cp = 2000.0;
+    md->t0 = 273.15;
+    md->cv_rock = 2.0e6;
+    for (int i = 0; i < nc && i < MPMC_MAX_COMPONENTS; ++i) { md->molar_mass[i] = 0.016 + 0.014 * i; }
+}
+
+static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
+                         const int32_t* branch, const double* prev, double dt, const mpmc_assembly_outputs* out,
+                         int32_t* domain_error, bool inventory, void* stream) {
+    if (!m || !g || !md || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: null argument"); }
+    const int nc = m->nc;
+    if (nc < 2) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: two-phase assembly needs nc >= 2"); }
+    if (!inventory && (!std::isfinite(dt) || !(dt > 0))) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: dt must be positive"); }
+    for (int i = 0; i < 5; ++i) {
+        const double v = (&md->mu_a0)[i];
+        if (!std::isfinite(v)) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: model parameters must be finite"); }
+    }
+    for (int i = 0; i < nc; ++i) {
+        if (!std::isfinite(md->molar_mass[i]) || !(md->molar_mass[i] > 0)) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "assembly: molar masses must be positive");
+        }
+    }
+    const void* need_ptrs[] = {q, branch, out->cell_status, out->residual};
+    for (const void* ptr : need_ptrs) {
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: buffers must be device memory"); }
+    }
+    if (!inventory) {
+        const void* more[] = {prev, out->diag};
+        for (const void* ptr : more) {
+            if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: buffers must be device memory"); }
+        }
+        if (g->nface > 0 && !is_device_ptr(out->offdiag)) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "assembly: offdiag must be device memory");
+        }
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    size_t need = 0;
+    switch (nc) {
+#define MPMC_CASE(N) case N: need = mpmc::dev::asm_record_bytes<N>(g->ncell); break;
+        MPMC_FOR_EACH_NC2(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "assembly: unsupported nc");
+    }
+    hipError_t e = hipSuccess;
+    if (g->rec_bytes < need) {
+        if (g->rec) {
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) { return hip_fail(e, "assembly workspace sync"); }
+            (void)hipFree(g->rec);
+            g->rec = nullptr;
+            g->rec_bytes = 0;
+        }
+        e = hipMalloc(&g->rec, need);
+        if (e != hipSuccess) { g->rec = nullptr; return fail(MPMC_E_NO_MEMORY, "assembly: workspace allocation failed"); }
+        g->rec_bytes = need;
+    }
+    mpmc::dev::AsmModelDev dm{};
+    dm.mu_a0 = md->mu_a0; dm.mu_a1 = md->mu_a1; dm.cp = md->cp; dm.t0 = md->t0; dm.cv_rock = md->cv_rock;
+    for (int i = 0; i < MPMC_MAX_COMPONENTS; ++i) { dm.molar_mass[i] = md->molar_mass[i]; }
+    const mpmc::dev::AsmGrid G{g->ncell, g->nface, g->bulk, g->poro, g->owner, g->neighbour, g->trans, g->gdot,
+                               g->gth, g->adj_off, g->adj_face};
+    const mpmc::dev::AsmIo io{q, branch, prev, dt, out->cell_status, out->residual, out->diag, out->offdiag,
+                              static_cast<double*>(g->rec), g->flag};
+    e = hipErrorInvalidValue;
+    switch (nc) {
+#define MPMC_CASE(N) case N: e = mpmc::dev::launch_assembly<N>(m->dev_constants, dm, G, io, inventory, s); break;
+        MPMC_FOR_EACH_NC2(MPMC_CASE)
+#undef MPMC_CASE
+    default: break;
+    }
+    if (e != hipSuccess) { return hip_fail(e, "assembly launch"); }
+    if (domain_error) {
+        unsigned long long flag = 0;
+        e = hipMemcpyAsync(&flag, g->flag, sizeof(flag), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) { e = hipStreamSynchronize(s); }
+        if (e != hipSuccess) { return hip_fail(e, "assembly flag read"); }
+        *domain_error = flag ? 1 : 0;
+    }
+    return MPMC_OK;
+}
+
+int mpmc_assembly_2p(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
+                     const int32_t* branch, const double* prev, double dt, const mpmc_assembly_outputs* out,
+                     int32_t* domain_error, void* stream) {
+    return assembly_impl(m, g, md, q, branch, prev, dt, out, domain_error, false, stream);
+}
+
+int mpmc_inventory_2p(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
+                      const int32_t* branch, int32_t* cell_status, double* inventory, int32_t* domain_error,
+                      void* stream) {
+    const mpmc_assembly_outputs out{cell_status, inventory, nullptr, nullptr};
+    return assembly_impl(m, g, md, q, branch, nullptr, 1.0, &out, domain_error, true, stream);
 }
 
 } // extern "C"

@@ -28,6 +28,12 @@ EXPORTED = (
     "mpmc_rr_batch",
     "mpmc_closure_default_options",
     "mpmc_closure_batch",
+    "mpmc_tpfa_grid_create",
+    "mpmc_tpfa_grid_destroy",
+    "mpmc_tpfa_grid_adjacency",
+    "mpmc_assembly_default_model",
+    "mpmc_assembly_2p",
+    "mpmc_inventory_2p",
     "mpmc_server_default_options",
     "mpmc_server_create",
     "mpmc_server_destroy",
@@ -101,6 +107,15 @@ class HostOutputs(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("status", "phase_count", "beta", "zfactor", "branch", "x", "ln_phi")]
 
 
+class AssemblyModel(ctypes.Structure):
+    _fields_ = [("mu_a0", ctypes.c_double), ("mu_a1", ctypes.c_double), ("cp", ctypes.c_double),
+                ("t0", ctypes.c_double), ("cv_rock", ctypes.c_double), ("molar_mass", ctypes.c_double * 12)]
+
+
+class AssemblyOutputs(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("cell_status", "residual", "diag", "offdiag")]
+
+
 class MpmcError(RuntimeError):
     pass
 
@@ -143,6 +158,15 @@ def library() -> ctypes.CDLL:
     lib.mpmc_closure_default_options.restype = None
     lib.mpmc_closure_batch.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOutputs),
                                        ctypes.POINTER(ClosureOptions), ctypes.POINTER(ClosureOutputs), P]
+    lib.mpmc_tpfa_grid_create.argtypes = [ctypes.c_int64, ctypes.c_int64] + [P] * 7 + [ctypes.POINTER(P)]
+    lib.mpmc_tpfa_grid_destroy.argtypes = [P]
+    lib.mpmc_tpfa_grid_adjacency.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
+    lib.mpmc_assembly_default_model.argtypes = [ctypes.c_int, ctypes.POINTER(AssemblyModel)]
+    lib.mpmc_assembly_default_model.restype = None
+    lib.mpmc_assembly_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, ctypes.c_double,
+                                     ctypes.POINTER(AssemblyOutputs), ctypes.POINTER(ctypes.c_int32), P]
+    lib.mpmc_inventory_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, P,
+                                      ctypes.POINTER(ctypes.c_int32), P]
     lib.mpmc_server_default_options.argtypes = [ctypes.POINTER(ServerOptions)]
     lib.mpmc_server_default_options.restype = None
     lib.mpmc_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]

@@ -147,3 +147,99 @@ def sour_gas_sweep(ncell: int = 2048, seed: int = 11):
     j = np.argmax(z, axis=0)
     z[j, np.arange(ncell)] += resid
     return mixture_sour_gas(), p, t, np.ascontiguousarray(z)
+
+
+# ---------------------------------------------------------------- two-phase assembly grid
+MILLIDARCY = 9.869233e-16  # m2
+GRAVITY = 9.80665          # m/s2, acting along -z
+
+
+def tpfa_grid(nx: int = 100, ny: int = 100, nz: int = 50, dx: float = 10.0, dy: float = 10.0, dz: float = 2.0,
+              seed: int = 4, log_k_sigma: float = 1.5, thermal_conductivity: float = 2.0) -> dict:
+    """Structured TPFA grid: cells c = i + nx (j + ny k); internal faces along x, y, z with
+    owner the lower-index cell. ln k ~ N(ln 100 mD, sigma^2) i.i.d. per cell, porosity
+    ~ U[0.1, 0.3]. Half transmissibility T_h = A (n.K.d)/(d.d) = 2 k A / h for an
+    axis-aligned face (tpfa_half_transmissibility_3d.hpp:11-17), T_f = T_o T_n / (T_o + T_n);
+    gdot = g . (x_n - x_o) with g = -9.80665 e_z; thermal conductance lambda A / h."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ncell = nx * ny * nz
+    k = np.exp(np.log(100.0 * MILLIDARCY) + log_k_sigma * rng.standard_normal(ncell))
+    poro = rng.uniform(0.1, 0.3, size=ncell)
+    idx = np.arange(ncell).reshape(nz, ny, nx)
+    owners, neighbours, trans, gdot, gth = [], [], [], [], []
+    for axis, (h, area, step) in enumerate(((dx, dy * dz, (0, 0, 1)), (dy, dx * dz, (0, 1, 0)),
+                                            (dz, dx * dy, (1, 0, 0)))):
+        sl_o = tuple(slice(0, idx.shape[a] - step[a]) for a in range(3))
+        sl_n = tuple(slice(step[a], idx.shape[a]) for a in range(3))
+        o = idx[sl_o].ravel()
+        nb = idx[sl_n].ravel()
+        to = 2.0 * k[o] * area / h
+        tn = 2.0 * k[nb] * area / h
+        owners.append(o)
+        neighbours.append(nb)
+        trans.append((to * tn) / (to + tn))
+        gdot.append(np.full(o.size, -GRAVITY * h if axis == 2 else 0.0))
+        gth.append(np.full(o.size, thermal_conductivity * area / h))
+    return {
+        "dims": (nx, ny, nz), "ncell": ncell,
+        "bulk": np.full(ncell, dx * dy * dz), "poro": poro,
+        "owner": np.concatenate(owners).astype(np.int32), "neighbour": np.concatenate(neighbours).astype(np.int32),
+        "trans": np.concatenate(trans), "gdot": np.concatenate(gdot), "gth": np.concatenate(gth),
+    }
+
+
+def two_phase_state(nc: int, ncell: int, roots, seed: int = 6, dims=None):
+    """A two-phase natural-variable state per cell: p, T smooth fields around 20 MPa / 360 K
+    (on `dims` when given), S_0 ~ U[0.2, 0.8], liquid-like x_0 ~ Dirichlet(3 .. 1) and
+    vapour-like x_1 ~ Dirichlet(1 .. 3) over the components. ``roots(p, t, x)`` returns
+    (root_count[n], root_flags[3][n]) for compositions x[nc][n]; phase 0 takes the smallest
+    and phase 1 the largest root with a valid derivative. Cells without one are redrawn.
+    Returns q[2nc+1][ncell] and branch[2][ncell]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if dims is not None:
+        shape = (dims[2], dims[1], dims[0])
+        sigma = max(0.5, min(6.0, min(shape) / 3.0))
+        p = _minmax(gaussian_field(shape, sigma, rng), 15.0 * MPA, 25.0 * MPA).ravel()
+        t = _minmax(gaussian_field(shape, sigma, rng), 330.0, 390.0).ravel()
+    else:
+        p = rng.uniform(15.0, 25.0, size=ncell) * MPA
+        t = rng.uniform(330.0, 390.0, size=ncell)
+    q = np.zeros((2 * nc + 1, ncell))
+    branch = np.zeros((2, ncell), np.int32)
+    todo = np.arange(ncell)
+    a_liq = np.linspace(3.0, 1.0, nc)[:, None]
+    a_vap = np.linspace(1.0, 3.0, nc)[:, None]
+    for _ in range(50):
+        m = todo.size
+        if m == 0:
+            break
+        x0 = rng.gamma(np.broadcast_to(a_liq, (nc, m)))
+        x0 /= x0.sum(axis=0, keepdims=True)
+        x1 = rng.gamma(np.broadcast_to(a_vap, (nc, m)))
+        x1 /= x1.sum(axis=0, keepdims=True)
+        x0 = np.maximum(x0, 1e-6)
+        x1 = np.maximum(x1, 1e-6)
+        # dependent last component as the chart defines it
+        x0[nc - 1] = 1.0 - x0[:nc - 1].sum(axis=0)
+        x1[nc - 1] = 1.0 - x1[:nc - 1].sum(axis=0)
+        good = (x0[nc - 1] > 1e-6) & (x1[nc - 1] > 1e-6)
+        br = np.full((2, m), -1, np.int32)
+        for ph, xx in enumerate((x0, x1)):
+            cnt, flags = roots(p[todo], t[todo], np.ascontiguousarray(xx))
+            valid = ((flags & 0x10) != 0) & (np.arange(3)[:, None] < np.asarray(cnt)[None, :])
+            anyv = valid.any(axis=0)
+            first = np.argmax(valid, axis=0)
+            last = 2 - np.argmax(valid[::-1], axis=0)
+            br[ph] = np.where(anyv, first if ph == 0 else last, -1)
+        good &= (br >= 0).all(axis=0)
+        sel = todo[good]
+        q[0, sel] = p[sel]
+        q[1, sel] = t[sel]
+        q[2, sel] = rng.uniform(0.2, 0.8, size=sel.size)
+        q[3:3 + nc - 1, sel] = x0[:nc - 1, good]
+        q[3 + nc - 1:, sel] = x1[:nc - 1, good]
+        branch[:, sel] = br[:, good]
+        todo = todo[~good]
+    if todo.size:
+        raise RuntimeError(f"two_phase_state: {todo.size} cells without a valid root")
+    return np.ascontiguousarray(q), branch

@@ -1456,19 +1456,34 @@ Num log1p_over_x_n(const Num& u) {
 // <Reduced=true, WithSums=true> (pr76_mixture.hpp:124-219) and Pr76Pure::evaluate
 // (pr76_pure.hpp:144-165): n-1 independent fractions, x_last = 1 - sum in Number
 // arithmetic; roots on the primal and the first-order IFT for the selected root.
-template <typename Num>
-void phase_reduced(const Mixture& m, const Num& p, const Num& t, const Num* xr, int root_index, Num& zout,
-                   Num* lnphi) {
+// Reduced = false is Pr76Phase::evaluate_full (pr76_phase.hpp:135-140): xr holds all n
+// fractions, each checked to lie in [0, 1] and their compensated primal sum within 64 eps
+// of one (pr76_mixture.hpp:146-158).
+template <typename Num, bool Reduced = true>
+void phase_eval(const Mixture& m, const Num& p, const Num& t, const Num* xr, int root_index, Num& zout,
+                Num* lnphi) {
     const int n = m.n;
     if (!fin(p) || !(val(p) > 0)) { throw AdDomain{}; }
     Num last(1.0), isum(0.0);
-    for (int i = 0; i + 1 < n; ++i) {
-        if (!fin(xr[i]) || val(xr[i]) < 0 || val(xr[i]) > 1) { throw AdDomain{}; }
-        isum += xr[i];
+    if constexpr (Reduced) {
+        for (int i = 0; i + 1 < n; ++i) {
+            if (!fin(xr[i]) || val(xr[i]) < 0 || val(xr[i]) > 1) { throw AdDomain{}; }
+            isum += xr[i];
+        }
+        last -= isum;
+        if (!fin(last) || val(last) < 0 || val(last) > 1) { throw AdDomain{}; }
+    } else {
+        long double sum = 0, comp = 0;
+        for (int i = 0; i < n; ++i) {
+            if (!fin(xr[i]) || val(xr[i]) < 0 || val(xr[i]) > 1) { throw AdDomain{}; }
+            const long double inc = val(xr[i]) - comp;
+            const long double next = sum + inc;
+            comp = (next - sum) - inc;
+            sum = next;
+        }
+        if (std::fabs(sum - 1.0L) > static_cast<long double>(64.0 * EPS)) { throw AdDomain{}; }
     }
-    last -= isum;
-    if (!fin(last) || val(last) < 0 || val(last) > 1) { throw AdDomain{}; }
-    auto x = [&](int i) -> const Num& { return i == n - 1 ? last : xr[i]; };
+    auto x = [&](int i) -> const Num& { return (Reduced && i == n - 1) ? last : xr[i]; };
     if (!fin(t) || !(val(t) > 0)) { throw AdDomain{}; }
     Num roots_a[NCMAX], sums[NCMAX];
     Num att(0.0), cov(0.0);
@@ -1547,6 +1562,12 @@ void phase_reduced(const Mixture& m, const Num& p, const Num& t, const Num* xr, 
     }
     if (!fin(z)) { throw RangeFail{}; }
     zout = z;
+}
+
+template <typename Num>
+void phase_reduced(const Mixture& m, const Num& p, const Num& t, const Num* xr, int root_index, Num& zout,
+                   Num* lnphi) {
+    phase_eval<Num, true>(m, p, t, xr, root_index, zout, lnphi);
 }
 
 // ---------------------------------------------------------------- closure
@@ -1922,6 +1943,304 @@ void build_closure(const Mixture& m, double p, double t, const VleResult& V, con
     }
 }
 
+// ---------------------------------------------------------------- two-phase assembly
+// Build-defined transport/caloric/rock models. The reference leaves viscosity, enthalpy,
+// rock storage and relative permeability to caller-owned providers
+// (pr76_selected_phase_property_closure.hpp:32-45, selected_phase_production_cell_evaluator.hpp:24-64);
+// these are this build's, evaluated in Number arithmetic so that AD flows through them:
+//   mu  = a0 + a1 * rho_mass^2            [Pa s]
+//   h   = cp * (T - T0)                   [J/kg]
+//   U_r = cv_rock * (T - T0)              [J per rock m3]
+//   kr_p = S_p^2 (Corey, exponent 2, no residual saturations)
+struct AsmModel {
+    double mu_a0, mu_a1, cp, t0, cv_rock;
+    double molar_mass[NCMAX];
+};
+
+// Natural-variable chart of one two-phase cell (two_phase_cell_state.hpp:120-231):
+// q = (p, T, S_0, x_0 independent [n-1], x_1 independent [n-1]), the last component of
+// each phase dependent.
+inline int q_count(int n) { return 2 * n + 1; }
+inline int x_col(int n, int phase, int k) { return 3 + phase * (n - 1) + k; }
+
+struct CellLin {
+    // per phase: primal and full-q gradients (pr76_selected_phase_property_closure.hpp:497-530)
+    double c[2], rho[2], mu[2], h[2], u[2], kr[2], lam[2];
+    std::vector<double> dc[2], drho[2], dmu[2], dh[2], du[2], dkr[2], dlam[2];
+    double x[2][NCMAX];
+    double sat[2];
+    double p, t;
+    // equilibrium rows (n) and their Jacobian [n][q]
+    double eq[NCMAX];
+    std::vector<double> deq;
+    // accumulation: N_i, total, E and gradients
+    double acc[NCMAX], acc_total, energy;
+    std::vector<double> dacc, dacc_total, denergy;
+};
+
+// reconstruct_composition (pr76_selected_phase_property_closure.hpp:115-151)
+template <typename Num>
+void reconstruct(int n, int phase, const Num* q, Num* x) {
+    Num sum(0.0);
+    for (int i = 0; i + 1 < n; ++i) {
+        x[i] = q[x_col(n, phase, i)];
+        if (!fin(x[i]) || !(val(x[i]) > 0)) { throw RangeFail{}; }
+        sum += x[i];
+    }
+    x[n - 1] = 1.0 - sum;
+    if (!fin(x[n - 1]) || !(val(x[n - 1]) > 0)) { throw RangeFail{}; }
+}
+
+// Pr76SelectedPhasePropertyClosure::evaluate (:336-465) for one phase: molar mass, molar
+// density (evaluate_selected_phase_molar_density), ln(phi) (evaluate_selected_phase_fugacity),
+// mass density, provider mu and h, u = h - p / rho_mass. Outputs c, rho, mu, h, u, lnphi[n].
+template <typename Num>
+void selected_phase(const Mixture& m, const AsmModel& md, int branch, const Num& p, const Num& t, const Num* x,
+                    Num* out) {
+    const int n = m.n;
+    if (!fin(p) || !(val(p) > 0) || !fin(t) || !(val(t) > 0)) { throw RangeFail{}; }
+    Num mm(0.0);
+    for (int i = 0; i < n; ++i) {
+        if (!fin(x[i]) || !(val(x[i]) > 0)) { throw RangeFail{}; }
+        mm += x[i] * md.molar_mass[i];
+    }
+    if (!fin(mm) || !(val(mm) > 0)) { throw RangeFail{}; }
+    Num z, lp[NCMAX];
+    phase_eval<Num, false>(m, p, t, x, branch, z, lp); // density evaluation
+    const Num den = (z * R_GAS) * t;
+    const Num c = p / den;
+    Num z2, lp2[NCMAX];
+    phase_eval<Num, false>(m, p, t, x, branch, z2, lp2); // fugacity evaluation
+    const Num rho = c * mm;
+    if (!fin(c) || !(val(c) > 0) || !fin(rho) || !(val(rho) > 0)) { throw RangeFail{}; }
+    const Num mu = md.mu_a0 + md.mu_a1 * (rho * rho);
+    const Num h = md.cp * (t - md.t0);
+    if (!fin(mu) || !(val(mu) > 0) || !fin(h)) { throw RangeFail{}; }
+    const Num u = h - p / rho;
+    if (!fin(u)) { throw RangeFail{}; }
+    out[0] = c;
+    out[1] = rho;
+    out[2] = mu;
+    out[3] = h;
+    out[4] = u;
+    for (int i = 0; i < n; ++i) {
+        if (!fin(lp2[i])) { throw RangeFail{}; }
+        out[5 + i] = lp2[i];
+    }
+}
+
+// evaluate_pr76_selected_phase_property_chart<4> (:533-820) + the 2P bridge (:952-1098),
+// relative permeability, mobility (two_phase_transport.hpp:38-137), component and energy
+// accumulation (two_phase_accumulation.hpp:18-338) for one cell.
+void cell_linearization(const Mixture& m, const AsmModel& md, const double* qv, const int* branch, double porosity,
+                        CellLin& L) {
+    const int n = m.n;
+    const int Q = q_count(n);
+    const int stride = 5 + n;
+    const int nout = 2 * stride + n;
+    std::vector<double> vals(nout), J(static_cast<size_t>(nout) * Q);
+    auto fn = [&](const Dual<4>* v, Dual<4>* out) {
+        Dual<4> x[2][NCMAX], props[2][5 + NCMAX];
+        for (int ph = 0; ph < 2; ++ph) {
+            reconstruct(n, ph, v, x[ph]);
+            selected_phase(m, md, branch[ph], v[0], v[1], x[ph], props[ph]);
+            for (int k = 0; k < stride; ++k) { out[ph * stride + k] = props[ph][k]; }
+        }
+        for (int i = 0; i < n; ++i) {
+            out[2 * stride + i] = dlog(x[0][i] / x[1][i]) + props[0][5 + i] - props[1][5 + i] + dlog(v[0] / v[0]);
+        }
+    };
+    // blocked forward mode; the chart has up to 25 inputs and 2(5+n)+n outputs
+    constexpr int K = 4;
+    std::vector<Dual<K>> in(Q), out(nout);
+    for (int b0 = 0; b0 < Q; b0 += K) {
+        for (int i = 0; i < Q; ++i) {
+            in[i] = Dual<K>(qv[i]);
+            if (i >= b0 && i < b0 + K) { in[i].d[i - b0] = 1.0; }
+        }
+        fn(in.data(), out.data());
+        for (int o = 0; o < nout; ++o) {
+            if (b0 == 0) { vals[o] = out[o].v; }
+            for (int k = 0; k < K && b0 + k < Q; ++k) { J[static_cast<size_t>(o) * Q + b0 + k] = out[o].d[k]; }
+        }
+    }
+    auto grad = [&](int o) { return std::vector<double>(J.begin() + static_cast<long>(o) * Q, J.begin() + static_cast<long>(o + 1) * Q); };
+    L.p = qv[0];
+    L.t = qv[1];
+    L.sat[0] = qv[2];
+    L.sat[1] = 1.0 - qv[2];
+    for (int ph = 0; ph < 2; ++ph) {
+        double sum = 0;
+        for (int i = 0; i + 1 < n; ++i) { L.x[ph][i] = qv[x_col(n, ph, i)]; sum += L.x[ph][i]; }
+        L.x[ph][n - 1] = 1.0 - sum;
+        const int b = ph * stride;
+        L.c[ph] = vals[b]; L.rho[ph] = vals[b + 1]; L.mu[ph] = vals[b + 2]; L.h[ph] = vals[b + 3]; L.u[ph] = vals[b + 4];
+        L.dc[ph] = grad(b); L.drho[ph] = grad(b + 1); L.dmu[ph] = grad(b + 2); L.dh[ph] = grad(b + 3); L.du[ph] = grad(b + 4);
+        // Corey kr = S^2 and its gradient (S_0 = q[2], S_1 = 1 - q[2])
+        L.kr[ph] = L.sat[ph] * L.sat[ph];
+        L.dkr[ph].assign(Q, 0.0);
+        L.dkr[ph][2] = ph == 0 ? 2.0 * L.sat[0] : -2.0 * L.sat[1];
+        // mobility, two_phase_transport.hpp:92-125
+        const double mu = L.mu[ph], kr = L.kr[ph];
+        if (!std::isfinite(mu) || !(mu > 0) || !std::isfinite(kr) || kr < 0) { throw RangeFail{}; }
+        L.lam[ph] = kr / mu;
+        L.dlam[ph].assign(Q, 0.0);
+        const double mu2 = mu * mu;
+        for (int col = 0; col < Q; ++col) {
+            L.dlam[ph][col] = (L.dkr[ph][col] * mu - kr * L.dmu[ph][col]) / mu2;
+            if (!std::isfinite(L.dlam[ph][col])) { throw RangeFail{}; }
+        }
+    }
+    L.deq.assign(static_cast<size_t>(n) * Q, 0.0);
+    for (int i = 0; i < n; ++i) {
+        L.eq[i] = vals[2 * stride + i];
+        for (int col = 0; col < Q; ++col) { L.deq[static_cast<size_t>(i) * Q + col] = J[static_cast<size_t>(2 * stride + i) * Q + col]; }
+    }
+    // component accumulation (two_phase_accumulation.hpp:18-160)
+    auto dsat = [](int ph, int col) { return col != 2 ? 0.0 : (ph == 0 ? 1.0 : -1.0); };
+    auto dcomp = [&](int ph, int i, int col) {
+        const int first = x_col(n, ph, 0);
+        if (col < first || col >= first + n - 1) { return 0.0; }
+        if (i == n - 1) { return -1.0; }
+        return col - first == i ? 1.0 : 0.0;
+    };
+    L.acc_total = 0;
+    for (int i = 0; i < n; ++i) { L.acc[i] = 0; }
+    for (int ph = 0; ph < 2; ++ph) {
+        L.acc_total += porosity * L.sat[ph] * L.c[ph];
+        for (int i = 0; i < n; ++i) { L.acc[i] += porosity * L.sat[ph] * L.c[ph] * L.x[ph][i]; }
+    }
+    L.dacc.assign(static_cast<size_t>(n) * Q, 0.0);
+    L.dacc_total.assign(Q, 0.0);
+    for (int col = 0; col < Q; ++col) {
+        for (int ph = 0; ph < 2; ++ph) {
+            const double s = L.sat[ph], ds = dsat(ph, col), c = L.c[ph], dc = L.dc[ph][col];
+            L.dacc_total[col] += porosity * (ds * c + s * dc);
+            for (int i = 0; i < n; ++i) {
+                const double dx = dcomp(ph, i, col);
+                L.dacc[static_cast<size_t>(i) * Q + col] += porosity * (ds * c * L.x[ph][i] + s * dc * L.x[ph][i] + s * c * dx);
+            }
+        }
+    }
+    // energy accumulation (two_phase_accumulation.hpp:184-338) with the rock model
+    const double urock = md.cv_rock * (L.t - md.t0);
+    double fluid = 0;
+    for (int ph = 0; ph < 2; ++ph) { fluid += porosity * L.sat[ph] * L.rho[ph] * L.u[ph]; }
+    const double rock_bulk = (1.0 - porosity) * urock;
+    L.energy = fluid + rock_bulk;
+    L.denergy.assign(Q, 0.0);
+    for (int col = 0; col < Q; ++col) {
+        const double durock = col == 1 ? md.cv_rock : 0.0;
+        double g = (1.0 - porosity) * durock;
+        for (int ph = 0; ph < 2; ++ph) {
+            const double s = L.sat[ph], ds = dsat(ph, col), rho = L.rho[ph], drho = L.drho[ph][col];
+            const double u = L.u[ph], du = L.du[ph][col];
+            g += porosity * (ds * rho * u + s * drho * u + s * rho * du);
+        }
+        L.denergy[col] = g;
+    }
+}
+
+struct AsmOut {
+    double* residual; // [Q][ncell]
+    double* diag;     // [Q*Q][ncell]
+    double* offdiag;  // [2][(n+1)*Q][nface]
+};
+
+// Face: build_two_phase_potential_upwind_linearization (two_phase_transport.hpp:173-321)
+// and build_two_phase_tpfa_face_linearization (two_phase_tpfa.hpp:129-667). Adds the
+// owner and neighbour rows' contributions (rates / bulk volume) to res/diag and writes the
+// two off-diagonal blocks.
+void face_contribution(int n, const CellLin& o, const CellLin& nb, double tf, double gdot, double gth, double vo,
+                       double vn, double* res_o, double* res_n, double* diag_o, double* diag_n, double* off_on,
+                       double* off_no) {
+    const int Q = q_count(n);
+    auto dcomp = [&](int ph, int i, int col) {
+        const int first = x_col(n, ph, 0);
+        if (col < first || col >= first + n - 1) { return 0.0; }
+        if (i == n - 1) { return -1.0; }
+        return col - first == i ? 1.0 : 0.0;
+    };
+    std::vector<double> rate(n, 0.0), jo(static_cast<size_t>(n) * Q, 0.0), jn(static_cast<size_t>(n) * Q, 0.0);
+    double erate = 0;
+    std::vector<double> eo(Q, 0.0), en(Q, 0.0);
+    for (int ph = 0; ph < 2; ++ph) {
+        const double rf = 0.5 * (o.rho[ph] + nb.rho[ph]);
+        const double delta = (nb.p - o.p) - rf * gdot;
+        if (!std::isfinite(rf) || !(rf > 0) || !std::isfinite(delta)) { throw RangeFail{}; }
+        std::vector<double> dphi_o(Q), dphi_n(Q);
+        for (int col = 0; col < Q; ++col) {
+            dphi_o[col] = -(col == 0 ? 1.0 : 0.0) - 0.5 * o.drho[ph][col] * gdot;
+            dphi_n[col] = (col == 0 ? 1.0 : 0.0) - 0.5 * nb.drho[ph][col] * gdot;
+        }
+        const bool own = !(delta > 0.0); // owner on a negative potential and on the exact tie
+        const double lam = own ? o.lam[ph] : nb.lam[ph];
+        const double flux = -tf * lam * delta;
+        if (!std::isfinite(flux)) { throw RangeFail{}; }
+        std::vector<double> fo(Q), fn(Q);
+        for (int col = 0; col < Q; ++col) {
+            fo[col] = -tf * (lam * dphi_o[col] + delta * (own ? o.dlam[ph][col] : 0.0));
+            fn[col] = -tf * (lam * dphi_n[col] + delta * (own ? 0.0 : nb.dlam[ph][col]));
+        }
+        const CellLin& up = own ? o : nb;
+        const double c = up.c[ph];
+        for (int i = 0; i < n; ++i) {
+            const double content = c * up.x[ph][i];
+            rate[i] += content * flux;
+            for (int col = 0; col < Q; ++col) {
+                double dco = 0.0, dcn = 0.0;
+                if (own) { dco = o.dc[ph][col] * up.x[ph][i] + c * dcomp(ph, i, col); }
+                else { dcn = nb.dc[ph][col] * up.x[ph][i] + c * dcomp(ph, i, col); }
+                jo[static_cast<size_t>(i) * Q + col] += dco * flux + content * fo[col];
+                jn[static_cast<size_t>(i) * Q + col] += dcn * flux + content * fn[col];
+            }
+        }
+        const double rho = up.rho[ph], h = up.h[ph];
+        const double ed = rho * h;
+        erate += ed * flux;
+        for (int col = 0; col < Q; ++col) {
+            const double dedo = own ? o.drho[ph][col] * h + rho * o.dh[ph][col] : 0.0;
+            const double dedn = own ? 0.0 : nb.drho[ph][col] * h + rho * nb.dh[ph][col];
+            eo[col] += dedo * flux + ed * fo[col];
+            en[col] += dedn * flux + ed * fn[col];
+        }
+    }
+    const double cond = gth * (o.t - nb.t);
+    const double etot = erate + cond;
+    eo[1] += gth;
+    en[1] -= gth;
+    // owner row: +rate / V_o ; neighbour row: -rate / V_n (two_phase_tpfa.hpp:526-640)
+    for (int i = 0; i < n; ++i) {
+        res_o[i] += rate[i] / vo;
+        res_n[i] += -rate[i] / vn;
+        for (int col = 0; col < Q; ++col) {
+            diag_o[i * Q + col] += jo[static_cast<size_t>(i) * Q + col] / vo;
+            off_on[i * Q + col] = jn[static_cast<size_t>(i) * Q + col] / vo;
+            off_no[i * Q + col] = -jo[static_cast<size_t>(i) * Q + col] / vn;
+            diag_n[i * Q + col] += -jn[static_cast<size_t>(i) * Q + col] / vn;
+        }
+    }
+    res_o[n] += etot / vo;
+    res_n[n] += -etot / vn;
+    for (int col = 0; col < Q; ++col) {
+        diag_o[n * Q + col] += eo[col] / vo;
+        off_on[n * Q + col] = en[col] / vo;
+        off_no[n * Q + col] = eo[col] * (-1.0 / vn);
+        diag_n[n * Q + col] += en[col] * (-1.0 / vn);
+    }
+}
+
+AsmModel make_model(int n, const double* model) {
+    AsmModel md{};
+    md.mu_a0 = model[0];
+    md.mu_a1 = model[1];
+    md.cp = model[2];
+    md.t0 = model[3];
+    md.cv_rock = model[4];
+    for (int i = 0; i < n; ++i) { md.molar_mass[i] = model[5 + i]; }
+    return md;
+}
+
 template <typename Body>
 void parallel_cells(int64_t ncell, int threads, Body body) {
     if (threads <= 0) { threads = static_cast<int>(std::max(1U, std::thread::hardware_concurrency())); }
@@ -2073,4 +2392,106 @@ extern "C" int cpuref_closure_batch(int nc, const double* tc, const double* pc, 
         }
     });
     return 0;
+}
+
+// TwoPhaseSnesAssemblyContext3D::evaluate_complete_assembly
+// (two_phase_snes_assembly.hpp:550-1090) on one rank: every cell's linearization, the
+// backward-Euler accumulation residuals, every face's TPFA contributions, then the
+// fugacity-equilibrium rows. Row order per cell: components, energy, equilibrium
+// (two_phase_cell_state.hpp:198-222). Returns 0, or 2 when any cell raised a domain error
+// (the reference's global domain_error status; nothing is assembled then).
+extern "C" int cpuref_assembly(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
+                               const double* model, int64_t ncell, const double* q, const int32_t* branch,
+                               const double* bulk, const double* poro, const double* prev, double dt, int64_t nface,
+                               const int32_t* owner, const int32_t* neighbour, const double* trans, const double* gdot,
+                               const double* gth, int threads, int32_t* cell_status, double* residual, double* diag,
+                               double* offdiag) {
+    if (nc < 2 || nc > NCMAX || ncell < 0 || nface < 0 || !(dt > 0)) { return 1; }
+    const Mixture m = make_mixture(nc, tc, pc, omega, kij);
+    const AsmModel md = make_model(nc, model);
+    const int Q = q_count(nc);
+    std::vector<CellLin> cells(static_cast<size_t>(ncell));
+    std::atomic<int> domain{0};
+    parallel_cells(ncell, threads, [&](int64_t c) {
+        double qv[2 * NCMAX + 1];
+        for (int k = 0; k < Q; ++k) { qv[k] = q[k * ncell + c]; }
+        const int br[2] = {branch[c], branch[ncell + c]};
+        cell_status[c] = 0;
+        try {
+            cell_linearization(m, md, qv, br, poro[c], cells[c]);
+        } catch (...) {
+            cell_status[c] = 1;
+            domain = 1;
+        }
+    });
+    if (domain) { return 2; }
+    const double inv_dt = 1.0 / dt;
+    for (int64_t c = 0; c < ncell; ++c) {
+        const CellLin& L = cells[c];
+        for (int i = 0; i < nc; ++i) {
+            residual[i * ncell + c] = (L.acc[i] - prev[i * ncell + c]) * inv_dt;
+            for (int col = 0; col < Q; ++col) { diag[(i * Q + col) * ncell + c] = L.dacc[static_cast<size_t>(i) * Q + col] * inv_dt; }
+        }
+        residual[nc * ncell + c] = (L.energy - prev[nc * ncell + c]) / dt;
+        for (int col = 0; col < Q; ++col) { diag[(nc * Q + col) * ncell + c] = L.denergy[col] / dt; }
+        for (int i = 0; i < nc; ++i) {
+            residual[(nc + 1 + i) * ncell + c] = L.eq[i];
+            for (int col = 0; col < Q; ++col) {
+                diag[((nc + 1 + i) * Q + col) * ncell + c] = L.deq[static_cast<size_t>(i) * Q + col];
+            }
+        }
+    }
+    const int rows = nc + 1;
+    std::vector<double> ro(Q), rn(Q), dgo(static_cast<size_t>(Q) * Q), dgn(static_cast<size_t>(Q) * Q);
+    std::vector<double> on(static_cast<size_t>(rows) * Q), no(static_cast<size_t>(rows) * Q);
+    for (int64_t f = 0; f < nface; ++f) {
+        const int64_t a = owner[f], b = neighbour[f];
+        std::fill(ro.begin(), ro.end(), 0.0);
+        std::fill(rn.begin(), rn.end(), 0.0);
+        std::fill(dgo.begin(), dgo.end(), 0.0);
+        std::fill(dgn.begin(), dgn.end(), 0.0);
+        try {
+            face_contribution(nc, cells[a], cells[b], trans[f], gdot[f], gth[f], bulk[a], bulk[b], ro.data(), rn.data(),
+                              dgo.data(), dgn.data(), on.data(), no.data());
+        } catch (...) {
+            return 3;
+        }
+        for (int r = 0; r < rows; ++r) {
+            residual[r * ncell + a] += ro[r];
+            residual[r * ncell + b] += rn[r];
+            for (int col = 0; col < Q; ++col) {
+                diag[(r * Q + col) * ncell + a] += dgo[r * Q + col];
+                diag[(r * Q + col) * ncell + b] += dgn[r * Q + col];
+                offdiag[(static_cast<int64_t>(0 * rows * Q) + r * Q + col) * nface + f] = on[r * Q + col];
+                offdiag[(static_cast<int64_t>(1 * rows * Q) + r * Q + col) * nface + f] = no[r * Q + col];
+            }
+        }
+    }
+    return 0;
+}
+
+// Pore-volume component and energy inventories N_i, E at a state (the previous-time
+// snapshots of build_backward_euler_*_accumulation_residual): acc[(nc+1)][ncell].
+extern "C" int cpuref_accumulation(int nc, const double* tc, const double* pc, const double* omega,
+                                   const double* kij, const double* model, int64_t ncell, const double* q,
+                                   const int32_t* branch, const double* poro, int threads, double* acc) {
+    if (nc < 2 || nc > NCMAX || ncell < 0) { return 1; }
+    const Mixture m = make_mixture(nc, tc, pc, omega, kij);
+    const AsmModel md = make_model(nc, model);
+    const int Q = q_count(nc);
+    std::atomic<int> domain{0};
+    parallel_cells(ncell, threads, [&](int64_t c) {
+        double qv[2 * NCMAX + 1];
+        for (int k = 0; k < Q; ++k) { qv[k] = q[k * ncell + c]; }
+        const int br[2] = {branch[c], branch[ncell + c]};
+        CellLin L;
+        try {
+            cell_linearization(m, md, qv, br, poro[c], L);
+            for (int i = 0; i < nc; ++i) { acc[i * ncell + c] = L.acc[i]; }
+            acc[nc * ncell + c] = L.energy;
+        } catch (...) {
+            domain = 1;
+        }
+    });
+    return domain ? 2 : 0;
 }

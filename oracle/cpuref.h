@@ -42,6 +42,22 @@ int cpuref_closure_batch(int nc, const double* tc, const double* pc, const doubl
                          double min_rcond, int threads, int32_t* status, double* primal, double* jac,
                          double* log_k_jac, double* diag);
 
+/* Two-phase natural-variable residual and Jacobian (TwoPhaseSnesAssemblyContext3D::
+ * evaluate_complete_assembly) on a list of cells and internal faces. model = {mu_a0, mu_a1,
+ * cp, T0, cv_rock, molar_mass[nc]}; q[2nc+1][ncell]; branch[2][ncell] selected root index
+ * per phase; prev[nc+1][ncell] previous N_i and E; outputs residual[Q][ncell],
+ * diag[Q*Q][ncell] (row-major block), offdiag[2][(nc+1)*Q][nface] (owner row x neighbour
+ * column, neighbour row x owner column). Returns 0, 2 on a cell domain error. */
+int cpuref_assembly(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
+                    const double* model, int64_t ncell, const double* q, const int32_t* branch,
+                    const double* bulk, const double* poro, const double* prev, double dt, int64_t nface,
+                    const int32_t* owner, const int32_t* neighbour, const double* trans, const double* gdot,
+                    const double* gth, int threads, int32_t* cell_status, double* residual, double* diag,
+                    double* offdiag);
+int cpuref_accumulation(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
+                        const double* model, int64_t ncell, const double* q, const int32_t* branch,
+                        const double* poro, int threads, double* acc);
+
 #ifdef __cplusplus
 }
 #endif

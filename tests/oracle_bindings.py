@@ -154,3 +154,55 @@ def closure(mix, p, t, z, fugacity_tolerance=1e-11, min_fraction=1.4901161193847
     if rc != 0:
         raise RuntimeError(f"{prefix}_closure_batch failed rc={rc}")
     return out
+
+
+def assembly(mix, model, q, branch, grid, prev, dt, threads=0):
+    """The oracle's two-phase assembly (cpuref_assembly): returns rc, cell_status,
+    residual[Q][n], diag[Q*Q][n], offdiag[2][(nc+1)*Q][nface]."""
+    L = lib("cpuref")
+    tc, pc, om, kij = _mix_arrays(mix)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    nc = (q.shape[0] - 1) // 2
+    Q = q.shape[0]
+    n = q.shape[1]
+    nf = grid["owner"].size
+    br = np.ascontiguousarray(branch, dtype=np.int32)
+    model = np.ascontiguousarray(model, dtype=np.float64)
+    prev = np.ascontiguousarray(prev, dtype=np.float64)
+    g = {k: np.ascontiguousarray(grid[k]) for k in ("bulk", "poro", "owner", "neighbour", "trans", "gdot", "gth")}
+    out = {"cell_status": np.zeros(n, np.int32), "residual": np.zeros((Q, n)), "diag": np.zeros((Q * Q, n)),
+           "offdiag": np.zeros((2, (nc + 1) * Q, max(1, nf)))}
+    fn = L.cpuref_assembly
+    fn.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_double,
+                   ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_int, _P, _P, _P, _P]
+    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), _ptr(model), n, _ptr(q), _ptr(br), _ptr(g["bulk"]),
+            _ptr(g["poro"]), _ptr(prev), float(dt), nf, _ptr(g["owner"]), _ptr(g["neighbour"]), _ptr(g["trans"]),
+            _ptr(g["gdot"]), _ptr(g["gth"]), int(threads), _ptr(out["cell_status"]), _ptr(out["residual"]),
+            _ptr(out["diag"]), _ptr(out["offdiag"]))
+    out["rc"] = rc
+    return out
+
+
+def inventory(mix, model, q, branch, poro, threads=0):
+    L = lib("cpuref")
+    tc, pc, om, kij = _mix_arrays(mix)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    nc = (q.shape[0] - 1) // 2
+    n = q.shape[1]
+    acc = np.zeros((nc + 1, n))
+    fn = L.cpuref_accumulation
+    fn.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P]
+    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), _ptr(np.ascontiguousarray(model, dtype=np.float64)), n,
+            _ptr(q), _ptr(np.ascontiguousarray(branch, dtype=np.int32)), _ptr(np.ascontiguousarray(poro)),
+            int(threads), _ptr(acc))
+    if rc != 0:
+        raise RuntimeError(f"cpuref_accumulation rc={rc}")
+    return acc
+
+
+def phase_roots(mix):
+    """roots(p, t, x) -> (root_count, root_flags) through the oracle, for state generators."""
+    def roots(p, t, x):
+        r = phase(mix, p, t, x)
+        return r["root_count"], r["root_flags"]
+    return roots
