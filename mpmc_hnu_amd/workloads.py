@@ -188,8 +188,9 @@ def tpfa_grid(nx: int = 100, ny: int = 100, nz: int = 50, dx: float = 10.0, dy: 
     }
 
 
-def two_phase_state(nc: int, ncell: int, roots, seed: int = 6, dims=None):
-    """A two-phase natural-variable state per cell: p, T smooth fields around 20 MPa / 360 K
+def two_phase_state(nc: int, ncell: int, roots, seed: int = 6, dims=None, p_range=(15.0 * MPA, 25.0 * MPA),
+                    t_range=(330.0, 390.0)):
+    """A two-phase natural-variable state per cell: p, T smooth fields over p_range / t_range
     (on `dims` when given), S_0 ~ U[0.2, 0.8], liquid-like x_0 ~ Dirichlet(3 .. 1) and
     vapour-like x_1 ~ Dirichlet(1 .. 3) over the components. ``roots(p, t, x)`` returns
     (root_count[n], root_flags[3][n]) for compositions x[nc][n]; phase 0 takes the smallest
@@ -199,11 +200,11 @@ def two_phase_state(nc: int, ncell: int, roots, seed: int = 6, dims=None):
     if dims is not None:
         shape = (dims[2], dims[1], dims[0])
         sigma = max(0.5, min(6.0, min(shape) / 3.0))
-        p = _minmax(gaussian_field(shape, sigma, rng), 15.0 * MPA, 25.0 * MPA).ravel()
-        t = _minmax(gaussian_field(shape, sigma, rng), 330.0, 390.0).ravel()
+        p = _minmax(gaussian_field(shape, sigma, rng), *p_range).ravel()
+        t = _minmax(gaussian_field(shape, sigma, rng), *t_range).ravel()
     else:
-        p = rng.uniform(15.0, 25.0, size=ncell) * MPA
-        t = rng.uniform(330.0, 390.0, size=ncell)
+        p = rng.uniform(*p_range, size=ncell)
+        t = rng.uniform(*t_range, size=ncell)
     q = np.zeros((2 * nc + 1, ncell))
     branch = np.zeros((2, ncell), np.int32)
     todo = np.arange(ncell)

@@ -1,8 +1,11 @@
 // cpuref — the oracle. Test infrastructure only: an FP64 CPU restatement of the
 // reference's PR76 PT flash (TPD multistart stability, bisection cubic roots,
 // Rachford-Rice, logK-SSI split, three-phase SSI with generalized RR, max3
-// orchestration and phase-set publication). It is loaded by tests/ and by
-// bench.py's CPU-baseline leg and never by the product library.
+// orchestration and phase-set publication), of its forward-AD flash closure (IFT
+// sensitivities of a two-phase split) and of its two-phase natural-variable TPFA
+// assembly (property chart, mobility, accumulation, upwind faces, block Jacobian).
+// It is loaded by tests/ and by bench.py's CPU-baseline leg and never by the product
+// library.
 //
 // Each function cites the reference file:line it restates (paths relative to the
 // reference checkout). The restatement keeps the reference's arithmetic order
@@ -14,8 +17,10 @@
 // reference's own tests hold (tests/thermodynamics/pr76_pt/pt_test.cpp:90-150,
 // tests/flash/pt_split/split_test.cpp:316-341, ternary_references.hpp,
 // tests/flash/pr76_three_phase/sour_gas_references.hpp), against vectors written by
-// the reference itself (tests/golden/*.json, oracle/ref_driver.cpp), and against
-// closed forms (Cardano roots in long double, RR material balance).
+// the reference itself (tests/golden/*.npz, oracle/ref_driver.cpp; the closure and the
+// assembly bit for bit), and against what the mathematics fixes (Cardano roots in long
+// double, RR material balance, closure derivatives and assembly Jacobians against
+// central differences, conservation of every face's fluxes between its two cells).
 #include "cpuref.h"
 
 #include <algorithm>

@@ -1,8 +1,9 @@
-"""Write tests/golden/ref_flash_vectors.npz and ref_closure_vectors.npz: seeded inputs
-and the reference's own flash and closure results for them, computed by the reference's header-only flash built from its
-checkout (oracle/_ref/libmpmc_ref.so, via oracle/ref_driver.cpp).
+"""Write tests/golden/ref_flash_vectors.npz, ref_closure_vectors.npz and
+ref_assembly_vectors.npz: seeded inputs and the reference's own flash, closure and
+two-phase assembly results for them, computed by the reference's header-only code built
+from its checkout (oracle/_ref/libmpmc_ref.so, via oracle/ref_driver.cpp).
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [flash closure assembly]
 
 Only data is written (inputs and outputs); no reference source is copied.
 """
@@ -31,9 +32,17 @@ CASES = {
 }
 
 
-def main():
-    if not ob.have_ref():
-        raise SystemExit("reference build missing: make -C oracle ref")
+ASSEMBLY_MODEL_BASE = [1.5e-5, 1.0e-9, 2000.0, 273.15, 2.0e6]
+LOW_P = dict(p_range=(0.5 * W.MPA, 3.0 * W.MPA), t_range=(290.0, 390.0))
+ASSEMBLY_CASES = {
+    # name: (nc, grid dims, low-pressure state with three-root cells)
+    "as2": (2, (3, 2, 2), False),
+    "as3lp": (3, (4, 4, 3), True),
+    "as6": (6, (3, 2, 2), False),
+}
+
+
+def flash_vectors():
     data = {}
     for name, (mixf, cellf, tol, maxph) in CASES.items():
         if name == "sour":
@@ -52,6 +61,9 @@ def main():
         data[f"{name}/out/e_prop"] = res["counters"][0]
         print(name, z.shape, np.unique(res["status"], return_counts=True))
     np.savez_compressed(HERE / "ref_flash_vectors.npz", **data)
+
+
+def closure_vectors():
     closures = {}
     for name, nc, n, seed in (("cl2", 2, 96, 72), ("cl3", 3, 96, 73), ("cl6", 6, 128, 76), ("cl12", 12, 128, 82)):
         mix = W.mixture_linspaced(nc)
@@ -66,5 +78,36 @@ def main():
     np.savez_compressed(HERE / "ref_closure_vectors.npz", **closures)
 
 
+def assembly_vectors():
+    out = {}
+    for name, (nc, dims, low_p) in ASSEMBLY_CASES.items():
+        mix = W.mixture_linspaced(nc)
+        grid = W.tpfa_grid(*dims, seed=nc + 3)
+        q, br = W.two_phase_state(nc, grid["ncell"], ob.phase_roots(mix), seed=40 + nc, dims=dims,
+                                  **(LOW_P if low_p else {}))
+        model = np.array(ASSEMBLY_MODEL_BASE + [0.016 + 0.014 * i for i in range(nc)])
+        prev = ob.inventory(mix, model, q, br, grid["poro"]) * 0.98
+        dt = 86400.0
+        res = ob.assembly(mix, model, q, br, grid, prev, dt, which="ref")
+        assert res["rc"] == 0
+        for k in ("tc", "pc", "omega", "kij"):
+            out[f"{name}/{k}"] = np.asarray(mix[k], dtype=np.float64)
+        for k in ("bulk", "poro", "owner", "neighbour", "trans", "gdot", "gth"):
+            out[f"{name}/grid/{k}"] = grid[k]
+        out[f"{name}/model"], out[f"{name}/q"], out[f"{name}/branch"] = model, q, br
+        out[f"{name}/prev"], out[f"{name}/dt"] = prev, np.float64(dt)
+        for k in ("cell_status", "residual", "diag", "offdiag"):
+            out[f"{name}/out/{k}"] = res[k]
+        print(name, q.shape, "largest-root vapour cells:", int((br[1] == 2).sum()))
+    np.savez_compressed(HERE / "ref_assembly_vectors.npz", **out)
+
+
+def main(which):
+    if not ob.have_ref():
+        raise SystemExit("reference build missing: make -C oracle ref")
+    for w in which:
+        {"flash": flash_vectors, "closure": closure_vectors, "assembly": assembly_vectors}[w]()
+
+
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or ["flash", "closure", "assembly"])

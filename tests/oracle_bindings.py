@@ -156,10 +156,13 @@ def closure(mix, p, t, z, fugacity_tolerance=1e-11, min_fraction=1.4901161193847
     return out
 
 
-def assembly(mix, model, q, branch, grid, prev, dt, threads=0):
-    """The oracle's two-phase assembly (cpuref_assembly): returns rc, cell_status,
-    residual[Q][n], diag[Q*Q][n], offdiag[2][(nc+1)*Q][nface]."""
-    L = lib("cpuref")
+def assembly(mix, model, q, branch, grid, prev, dt, threads=0, which="cpuref"):
+    """The two-phase assembly of the oracle (cpuref_assembly) or of the reference's own
+    building blocks (ref_assembly): returns rc, cell_status, residual[Q][n], diag[Q*Q][n],
+    offdiag[2][(nc+1)*Q][nface] (side 0: owner row x neighbour column)."""
+    L = lib(which)
+    if L is None:
+        raise RuntimeError(f"checker library '{which}' is not built")
     tc, pc, om, kij = _mix_arrays(mix)
     q = np.ascontiguousarray(q, dtype=np.float64)
     nc = (q.shape[0] - 1) // 2
@@ -172,7 +175,7 @@ def assembly(mix, model, q, branch, grid, prev, dt, threads=0):
     g = {k: np.ascontiguousarray(grid[k]) for k in ("bulk", "poro", "owner", "neighbour", "trans", "gdot", "gth")}
     out = {"cell_status": np.zeros(n, np.int32), "residual": np.zeros((Q, n)), "diag": np.zeros((Q * Q, n)),
            "offdiag": np.zeros((2, (nc + 1) * Q, max(1, nf)))}
-    fn = L.cpuref_assembly
+    fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_assembly")
     fn.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_double,
                    ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_int, _P, _P, _P, _P]
     rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), _ptr(model), n, _ptr(q), _ptr(br), _ptr(g["bulk"]),

@@ -44,13 +44,19 @@ def _close(a, b, rel=REL):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nc,dims", [(2, (4, 3, 2)), (3, (3, 3, 3)), (6, (5, 4, 3)), (12, (3, 2, 2))])
-def test_gpu_assembly_matches_oracle(gpu, nc, dims):
+@pytest.mark.parametrize("nc,dims,low_p", [(2, (4, 3, 2), False), (3, (3, 3, 3), False), (6, (5, 4, 3), False),
+                                           (12, (3, 2, 2), False), (3, (4, 4, 3), True)])
+def test_gpu_assembly_matches_oracle(gpu, nc, dims, low_p):
+    """low_p: 0.5-3 MPa, where some vapour-like cells take the largest of three roots."""
     from mpmc_hnu_amd import assembly as A
 
     mix = W.mixture_linspaced(nc)
-    grid = W.tpfa_grid(*dims, seed=nc)
-    q, br = W.two_phase_state(nc, grid["ncell"], ob.phase_roots(mix), seed=20 + nc, dims=dims)
+    grid = W.tpfa_grid(*dims, seed=nc + 3 if low_p else nc)
+    kw = dict(p_range=(0.5 * W.MPA, 3.0 * W.MPA), t_range=(290.0, 390.0)) if low_p else {}
+    q, br = W.two_phase_state(nc, grid["ncell"], ob.phase_roots(mix), seed=(40 if low_p else 20) + nc, dims=dims,
+                              **kw)
+    if low_p:
+        assert np.any(br[1] == 2)
     model = A.model_array(A.default_model(nc), nc)
     prev = ob.inventory(mix, model, q, br, grid["poro"]) * 0.98
     dt = 86400.0

@@ -74,3 +74,46 @@ def test_assembly_conserves_between_cells(nc):
     inv0 = ob.inventory(mix, model, q0, br, grid["poro"])
     res = ob.assembly(mix, model, q0, br, g0, inv0, dt)
     assert np.all(res["residual"][:nc + 1] == 0.0)
+
+
+LOW_P = dict(p_range=(0.5 * W.MPA, 3.0 * W.MPA), t_range=(290.0, 390.0))
+
+
+@pytest.mark.skipif(not ob.have_ref(), reason="reference build absent (oracle/_ref)")
+@pytest.mark.parametrize("nc,dims,low_p", [(2, (3, 2, 2), False), (3, (3, 3, 2), False), (6, (3, 2, 2), False),
+                                           (2, (4, 4, 3), True), (3, (4, 4, 3), True), (12, (2, 2, 2), False)])
+def test_assembly_is_the_reference_bit_for_bit(nc, dims, low_p):
+    """cpuref_assembly against ref_assembly: the reference's own property chart, 2P flow
+    bridge, mobility, accumulation/backward-Euler, potential upwind and TPFA face
+    (flow/*.hpp, flow_discretization/two_phase_tpfa.hpp) compiled from its headers and
+    summed into the same rows; identical to the last bit."""
+    mix = W.mixture_linspaced(nc)
+    grid = W.tpfa_grid(*dims, seed=nc + 3)
+    kw = LOW_P if low_p else {}
+    q, br = W.two_phase_state(nc, grid["ncell"], ob.phase_roots(mix), seed=40 + nc, dims=dims, **kw)
+    if low_p:
+        assert np.any(br[1] == 2), "the low-pressure case must exercise the largest of three roots"
+    model = np.array(MODEL_BASE + [0.016 + 0.014 * i for i in range(nc)])
+    prev = ob.inventory(mix, model, q, br, grid["poro"]) * 0.98
+    o = ob.assembly(mix, model, q, br, grid, prev, 86400.0)
+    r = ob.assembly(mix, model, q, br, grid, prev, 86400.0, which="ref")
+    assert o["rc"] == 0 and r["rc"] == 0
+    for k in ("cell_status", "residual", "diag", "offdiag"):
+        assert np.array_equal(o[k], r[k]), k
+
+
+ASSEMBLY_VECTORS = __import__("pathlib").Path(__file__).resolve().parent / "golden" / "ref_assembly_vectors.npz"
+
+
+@pytest.mark.parametrize("name", ["as2", "as3lp", "as6"])
+def test_assembly_reproduces_reference_vectors(name):
+    """The oracle on the committed reference outputs (tests/golden/make_golden.py, written
+    by the reference's assembly building blocks), bit for bit; runs without oracle/_ref."""
+    v = np.load(ASSEMBLY_VECTORS)
+    mix = {k: v[f"{name}/{k}"] for k in ("tc", "pc", "omega", "kij")}
+    grid = {k: v[f"{name}/grid/{k}"] for k in ("bulk", "poro", "owner", "neighbour", "trans", "gdot", "gth")}
+    o = ob.assembly(mix, v[f"{name}/model"], v[f"{name}/q"], v[f"{name}/branch"], grid, v[f"{name}/prev"],
+                    float(v[f"{name}/dt"]))
+    assert o["rc"] == 0
+    for k in ("cell_status", "residual", "diag", "offdiag"):
+        assert np.array_equal(o[k], v[f"{name}/out/{k}"]), k
