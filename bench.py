@@ -166,7 +166,10 @@ def main(argv=None):
                     help="grid of the assembly leg (nx,ny,nz; empty: skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals for the serving leg (0: skip)")
+    ap.add_argument("--legs", type=str, default="flash,assembly,serving",
+                    help="comma list of legs to run; the JSON line is complete only with all three")
     a = ap.parse_args(argv)
+    legs = set(a.legs.split(","))
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               "(cells are independent, so N GPUs would shard cell ranges with no exchange).", file=sys.stderr)
@@ -177,6 +180,15 @@ def main(argv=None):
     from mpmc_hnu_amd import flash, workloads as W
 
     torch.cuda.set_device(0)
+    if "flash" not in legs:
+        # a partial run (profiling one leg): print that leg's object only
+        part = {}
+        if "assembly" in legs and a.assembly_dims:
+            part["assembly"] = assembly_leg(tuple(int(v) for v in a.assembly_dims.split(",")), a.steps, a.warmup)
+        if "serving" in legs and a.serve_seconds > 0:
+            part["serving"] = serve_leg(a.serve_seconds)
+        print(json.dumps(part), flush=True)
+        return
     mix = W.mixture_linspaced(6)
     p, t, z = W.cells_c2(ncell=a.cells)
     nc, ncell = z.shape
@@ -250,10 +262,10 @@ def main(argv=None):
     print(f"[bench] mixed vs fp64: status diffs {status_diff}, phase-count diffs {count_diff}, "
           f"max rel diff {rel:.3e}; accepted {accepted}, phase counts {counts}", file=sys.stderr, flush=True)
     asm = None
-    if a.assembly_dims:
+    if a.assembly_dims and "assembly" in legs:
         asm = assembly_leg(tuple(int(v) for v in a.assembly_dims.split(",")), a.steps, a.warmup)
     serving = None
-    if a.serve_seconds > 0:
+    if a.serve_seconds > 0 and "serving" in legs:
         serving = serve_leg(a.serve_seconds)
     cpu = None
     if not a.no_cpu:

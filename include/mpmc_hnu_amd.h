@@ -226,10 +226,12 @@ int mpmc_closure_batch(mpmc_mixture_t mixture, int64_t ncell, const double* p, c
  * Layouts (device memory, cell or face index fastest):
  *   q[Q][ncell], branch[2][ncell], prev[nc+1][ncell] (N_i and E of the previous time),
  *   residual[Q][ncell], diag[Q*Q][ncell] (row-major Q x Q block of the cell's own columns),
- *   offdiag[(nc+1)*Q][2 nface]: the block (rows of cell c, columns of its neighbour across
- *   adjacency entry a) at column a, where a runs over the grid's cell -> face CSR
- *   (mpmc_tpfa_grid_adjacency); equilibrium rows have no off-diagonal entries. Together:
- *   a block-CSR matrix with Q x Q blocks. */
+ *   offdiag[(nc+1)*Q][max_degree][ncell]: for slot j < degree(c) of cell c, the block (rows
+ *   of c, columns of the cell across face adj_face[adj_off[c] + j]) at [.][j][c]; slots past
+ *   a cell's degree hold zeros. The slots follow the grid's cell -> face CSR
+ *   (mpmc_tpfa_grid_adjacency), so (j, c) is CSR position adj_off[c] + j, and the matrix is
+ *   block-CSR with Q x Q blocks stored ELL-style, cell fastest, so that every store is
+ *   coalesced. Equilibrium rows have no off-diagonal entries. */
 typedef struct mpmc_tpfa_grid* mpmc_tpfa_grid_t;
 
 /* Internal faces of a mesh: owner/neighbour cells, T_f [m3] (harmonic combination of the
@@ -240,8 +242,13 @@ int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk_volum
                           const int32_t* owner, const int32_t* neighbour, const double* transmissibility,
                           const double* gdot, const double* thermal_conductance, mpmc_tpfa_grid_t* out);
 int mpmc_tpfa_grid_destroy(mpmc_tpfa_grid_t grid);
-/* The grid's cell -> face CSR: adj_off[ncell+1], adj_face[2 nface] (device memory). */
+/* The grid's cell -> face CSR: adj_off[ncell+1], adj_face[2 nface] (device memory); a
+ * cell's faces are in face order. */
 int mpmc_tpfa_grid_adjacency(mpmc_tpfa_grid_t grid, const int32_t** adj_off, const int32_t** adj_face);
+/* The largest number of faces of one cell: the slot count of the off-diagonal blocks. The
+ * reference's DMPlex adjacency (two_phase_snes_assembly.hpp:1719-1784 preallocation) has no
+ * such call; it sizes the ELL layout here. */
+int mpmc_tpfa_grid_max_degree(mpmc_tpfa_grid_t grid, int32_t* max_degree);
 
 typedef struct mpmc_assembly_model {
     double mu_a0, mu_a1; /* Pa s, Pa s m6/kg2 */
@@ -255,7 +262,7 @@ typedef struct mpmc_assembly_outputs {
     int32_t* cell_status; /* [ncell] 0, or 1 where the cell's evaluation raised a domain error */
     double* residual;     /* [Q][ncell] */
     double* diag;         /* [Q*Q][ncell] */
-    double* offdiag;      /* [(nc+1)*Q][2 nface] */
+    double* offdiag;      /* [(nc+1)*Q][max_degree][ncell] */
 } mpmc_assembly_outputs;
 
 void mpmc_assembly_default_model(int nc, mpmc_assembly_model* model);

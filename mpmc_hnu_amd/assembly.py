@@ -46,6 +46,9 @@ class Grid:
                                                V(t["owner"].data_ptr()), V(t["neighbour"].data_ptr()),
                                                V(t["trans"].data_ptr()), V(t["gdot"].data_ptr()),
                                                V(t["gth"].data_ptr()), ctypes.byref(self.h)), "tpfa_grid_create")
+        deg = ctypes.c_int32(0)
+        native.check(lib.mpmc_tpfa_grid_max_degree(self.h, ctypes.byref(deg)), "tpfa_grid_max_degree")
+        self.max_degree = int(deg.value)
 
     def close(self):
         if self.h:
@@ -78,6 +81,19 @@ def adjacency_host(owner, neighbour, ncell: int):
     return off, adj
 
 
+def ell_to_csr(offdiag, off, nface: int):
+    """Off-diagonal blocks from the library's (slot, cell) layout [rows*Q][max_degree][ncell]
+    to CSR positions [rows*Q][2 nface]: slot j of cell c is CSR position off[c] + j."""
+    offdiag = np.asarray(offdiag)
+    ncell = off.size - 1
+    deg = np.diff(off)
+    out = np.zeros((offdiag.shape[0], 2 * nface))
+    for j in range(offdiag.shape[1]):
+        cells = np.nonzero(deg > j)[0]
+        out[:, off[cells] + j] = offdiag[:, j, cells]
+    return out
+
+
 def allocate(nc: int, grid: Grid, device) -> dict:
     torch = _torch()
     Q = 2 * nc + 1
@@ -85,7 +101,8 @@ def allocate(nc: int, grid: Grid, device) -> dict:
         "cell_status": torch.empty(grid.ncell, dtype=torch.int32, device=device),
         "residual": torch.empty((Q, grid.ncell), dtype=torch.float64, device=device),
         "diag": torch.empty((Q * Q, grid.ncell), dtype=torch.float64, device=device),
-        "offdiag": torch.empty(((nc + 1) * Q, max(1, 2 * grid.nface)), dtype=torch.float64, device=device),
+        "offdiag": torch.empty(((nc + 1) * Q, max(1, grid.max_degree), grid.ncell), dtype=torch.float64,
+                               device=device),
     }
 
 

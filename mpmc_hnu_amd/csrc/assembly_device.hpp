@@ -17,12 +17,16 @@
 //   * component and energy accumulation with backward Euler (two_phase_accumulation.hpp:
 //     18-338, component_accumulation_time.hpp:189-300, energy_accumulation.hpp:511-580);
 //   * a compact record per phase for the face pass.
-// k_asm_rows, one lane per cell: walks the cell's faces (CSR adjacency), evaluates each
-// face's potential upwind and TPFA linearization (two_phase_transport.hpp:173-321,
-// two_phase_tpfa.hpp:129-667) from both cells' records, and adds this cell's side:
-// residual rows and diagonal block, and writes its off-diagonal block for that face.
-// Each face is evaluated by both its cells, so there are no atomics and the sums are
-// deterministic; the extra flops are cheap next to the Jacobian bytes that bound the pass.
+// k_asm_rows, one wave per (64 cells, column chunk): the cell's columns are split into
+// chunks {p, T, S_0}, then runs of x_0 columns and of x_1 columns, so that one lane keeps
+// only (nc+1) x (chunk width) Jacobian sums in registers. Per lane: the accumulation rows
+// and backward Euler of its columns (and, for the first chunk, the residual), then every
+// face of the cell in slot order: potential upwind and TPFA linearization
+// (two_phase_transport.hpp:173-321, two_phase_tpfa.hpp:129-667) from both cells'
+// records, added to this cell's rows, and the off-diagonal block of that face written
+// to the cell's slot. Each face is evaluated by both its cells, so there are no atomics
+// and the sums are deterministic. Off-diagonal blocks are stored by (slot, cell), cell
+// fastest, so every store of a wave is one contiguous 512-byte run.
 #pragma once
 
 #include "closure_device.hpp"
@@ -48,6 +52,8 @@ struct AsmGrid {
     const double* gth;         // [nface] thermal conductance W/K
     const int32_t* adj_off;    // [ncell + 1] CSR of cell -> adjacent faces
     const int32_t* adj_face;   // [2 nface]
+    const int32_t* slot_face;  // [max_degree][ncell]: face of slot j of cell c (CSR order), -1 past the degree
+    int32_t max_degree;
 };
 
 struct AsmIo {
@@ -58,23 +64,51 @@ struct AsmIo {
     int32_t* status;        // [ncell] 0 ok, 1 domain error
     double* residual;       // [Q][ncell]
     double* diag;           // [Q*Q][ncell]
-    double* offdiag;        // [(nc+1)*Q][2 nface], by CSR position
+    double* offdiag;        // [(nc+1)*Q][max_degree][ncell], by (slot, cell)
     double* rec;            // workspace: per-cell face records
     unsigned long long* flag; // domain-error flag (set to 1)
 };
 
 // Record per cell for the face pass, SoA over cells: per phase p, with the phase's
 // sub-chart columns s = 0 (p), 1 (T), 2 + k (x_p,k):
-//   lam, dlam[nc+1], dlam_dS, rho, drho[nc+1], c, dc[nc+1], h, dh[nc+1], x[nc]
+//   lam, dlam[nc+1], dlam_dS, rho, drho[nc+1], c, dc[nc+1], h, dh[nc+1], x[nc], u, du[nc+1]
 // then p, T.
 template <int NC>
 struct Rec {
     static constexpr int S = NC + 1;
-    static constexpr int kPhase = 1 + S + 1 + 1 + S + 1 + S + 1 + S + NC;
     static constexpr int kLam = 0, kDlam = 1, kDlamS = 1 + S, kRho = 2 + S, kDrho = 3 + S, kC = 3 + 2 * S,
-                         kDc = 4 + 2 * S, kH = 4 + 3 * S, kDh = 5 + 3 * S, kX = 5 + 4 * S;
+                         kDc = 4 + 2 * S, kH = 4 + 3 * S, kDh = 5 + 3 * S, kX = 5 + 4 * S, kU = 5 + 4 * S + NC,
+                         kDu = 6 + 4 * S + NC;
+    static constexpr int kPhase = kDu + S;
     static constexpr int kP = 2 * kPhase, kT = 2 * kPhase + 1, kSize = 2 * kPhase + 2;
 };
+
+// Column chunks of the face pass: chunk 0 = {p, T, S_0}; then the x_0 columns and the x_1
+// columns in runs of CW.
+template <int NC>
+struct Chunks {
+    static constexpr int NX = NC - 1;
+    static constexpr int CW = NX <= 6 ? NX : 4;
+    static constexpr int PER = (NX + CW - 1) / CW;
+    static constexpr int N = 1 + 2 * PER;
+    static constexpr int W = CW > 3 ? CW : 3;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so that each XCD
+// walks one contiguous range of cells and the neighbours' records it reads stay in its L2.
+__device__ __forceinline__ unsigned xcd_contiguous_block(unsigned b, unsigned nb) {
+    const unsigned per = nb / 8u, rem = nb % 8u, x = b % 8u, l = b / 8u;
+    return x < rem ? x * (per + 1u) + l : rem * (per + 1u) + (x - rem) * per + l;
+}
+
+// Records are stored in blocks of 64 cells, field-major inside a block: field f of cell c
+// sits at ((c / 64) * kSize + f) * 64 + c % 64. One wave's accesses to a field are one
+// contiguous 512-byte run, and for a given cell every field is a constant offset from
+// one base address.
+template <int NC>
+__device__ __forceinline__ int64_t rix(int field, int64_t cell) {
+    return ((cell >> 6) * Rec<NC>::kSize + field) * 64 + (cell & 63);
+}
 
 // column of q for sub-chart column s of phase p
 template <int NC>
@@ -172,24 +206,25 @@ __global__ __launch_bounds__(64) void k_asm_cell(const double* __restrict__ mixc
             const double dkr = ph == 0 ? 2.0 * sat[0] : -2.0 * sat[1];
             const double mu = muv[ph], mu2 = mu * mu;
             const int b = ph * R::kPhase;
-            rec[(b + R::kLam) * n + c] = kr / mu;
-            for (int k = 0; k < S; ++k) { rec[(b + R::kDlam + k) * n + c] = (0.0 * mu - kr * dmu[ph][k]) / mu2; }
-            rec[(b + R::kDlamS) * n + c] = (dkr * mu - kr * 0.0) / mu2;
-            rec[(b + R::kRho) * n + c] = rhov[ph];
-            rec[(b + R::kC) * n + c] = cval[ph];
-            rec[(b + R::kH) * n + c] = hv[ph];
+            rec[rix<NC>(b + R::kLam, c)] = kr / mu;
+            for (int k = 0; k < S; ++k) { rec[rix<NC>(b + R::kDlam + k, c)] = (0.0 * mu - kr * dmu[ph][k]) / mu2; }
+            rec[rix<NC>(b + R::kDlamS, c)] = (dkr * mu - kr * 0.0) / mu2;
+            rec[rix<NC>(b + R::kRho, c)] = rhov[ph];
+            rec[rix<NC>(b + R::kC, c)] = cval[ph];
+            rec[rix<NC>(b + R::kH, c)] = hv[ph];
+            rec[rix<NC>(b + R::kU, c)] = uv[ph];
             for (int k = 0; k < S; ++k) {
-                rec[(b + R::kDrho + k) * n + c] = drho[ph][k];
-                rec[(b + R::kDc + k) * n + c] = dcv[ph][k];
-                rec[(b + R::kDh + k) * n + c] = dhv[ph][k];
+                rec[rix<NC>(b + R::kDrho + k, c)] = drho[ph][k];
+                rec[rix<NC>(b + R::kDc + k, c)] = dcv[ph][k];
+                rec[rix<NC>(b + R::kDh + k, c)] = dhv[ph][k];
+                rec[rix<NC>(b + R::kDu + k, c)] = duv[ph][k];
             }
-            for (int i = 0; i < NC; ++i) { rec[(b + R::kX + i) * n + c] = x[ph][i]; }
+            for (int i = 0; i < NC; ++i) { rec[rix<NC>(b + R::kX + i, c)] = x[ph][i]; }
         }
-        rec[R::kP * n + c] = P;
-        rec[R::kT * n + c] = T;
-        // accumulation (two_phase_accumulation.hpp:18-160) and backward Euler
-        const double inv_dt = 1.0 / io.dt;
-        auto dsat = [](int ph, int col) { return col != 2 ? 0.0 : (ph == 0 ? 1.0 : -1.0); };
+        rec[rix<NC>(R::kP, c)] = P;
+        rec[rix<NC>(R::kT, c)] = T;
+        // equilibrium rows: ln(x0_i / x1_i) + ln phi0_i - ln phi1_i (+ ln(p/p) = 0); the
+        // component and energy rows are formed by the face pass
         auto dcomp = [](int ph, int i, int col) {
             const int first = 3 + ph * (NC - 1);
             if (col < first || col >= first + NC - 1) { return 0.0; }
@@ -204,37 +239,6 @@ __global__ __launch_bounds__(64) void k_asm_cell(const double* __restrict__ mixc
             return g[2 + (col - first)];
         };
         for (int i = 0; i < NC; ++i) {
-            double acc = 0;
-            for (int ph = 0; ph < 2; ++ph) { acc += phi * sat[ph] * cval[ph] * x[ph][i]; }
-            io.residual[i * n + c] = (acc - io.prev[i * n + c]) * inv_dt;
-        }
-        for (int col = 0; col < Q; ++col) {
-            for (int i = 0; i < NC; ++i) {
-                double g = 0;
-                for (int ph = 0; ph < 2; ++ph) {
-                    const double s = sat[ph], ds = dsat(ph, col), cc = cval[ph], dc = subd(dcv[ph], ph, col);
-                    g += phi * (ds * cc * x[ph][i] + s * dc * x[ph][i] + s * cc * dcomp(ph, i, col));
-                }
-                io.diag[(i * Q + col) * n + c] = g * inv_dt;
-            }
-        }
-        {
-            double fluid = 0;
-            for (int ph = 0; ph < 2; ++ph) { fluid += phi * sat[ph] * rhov[ph] * uv[ph]; }
-            const double e = fluid + (1.0 - phi) * (md.cv_rock * (T - md.t0));
-            io.residual[NC * n + c] = (e - io.prev[NC * n + c]) / io.dt;
-            for (int col = 0; col < Q; ++col) {
-                double g = (1.0 - phi) * (col == 1 ? md.cv_rock : 0.0);
-                for (int ph = 0; ph < 2; ++ph) {
-                    const double s = sat[ph], ds = dsat(ph, col), rho = rhov[ph], dr = subd(drho[ph], ph, col);
-                    const double u = uv[ph], du = subd(duv[ph], ph, col);
-                    g += phi * (ds * rho * u + s * dr * u + s * rho * du);
-                }
-                io.diag[(NC * Q + col) * n + c] = g / io.dt;
-            }
-        }
-        // equilibrium rows: ln(x0_i / x1_i) + ln phi0_i - ln phi1_i (+ ln(p/p) = 0)
-        for (int i = 0; i < NC; ++i) {
             const double r = x[0][i] / x[1][i];
             io.residual[(NC + 1 + i) * n + c] = ((log(r) + lnp[0][i]) - lnp[1][i]) + 0.0;
             for (int col = 0; col < Q; ++col) {
@@ -247,132 +251,190 @@ __global__ __launch_bounds__(64) void k_asm_cell(const double* __restrict__ mixc
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k_asm_rows(AsmGrid G, AsmIo io) {
+__global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md, AsmGrid G, AsmIo io) {
     constexpr int Q = 2 * NC + 1;
-    constexpr int S = NC + 1;
     constexpr int ROWS = NC + 1;
     using R = Rec<NC>;
+    using C = Chunks<NC>;
+    constexpr int W = C::W;
     const int64_t n = G.ncell;
-    const int64_t nadj = 2 * G.nface;
     const double* rec = io.rec;
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        if (io.status[c] != 0) { continue; }
-        double res[ROWS];
-        double dg[ROWS * Q];
-        for (int r = 0; r < ROWS; ++r) {
-            res[r] = io.residual[r * n + c];
-            for (int col = 0; col < Q; ++col) { dg[r * Q + col] = io.diag[(r * Q + col) * n + c]; }
+    // wave-uniform column chunk: first column c0, width cnt, phase cph (-1: both phases)
+    const int chunk = static_cast<int>(threadIdx.x) / 64;
+    int c0 = 0, cnt = 3, cph = -1;
+    if (chunk > 0) {
+        const int k = chunk - 1;
+        cph = k / C::PER;
+        const int j = k % C::PER;
+        c0 = 3 + cph * (NC - 1) + j * C::CW;
+        cnt = min(C::CW, NC - 1 - j * C::CW);
+    }
+    const int64_t blk = xcd_contiguous_block(blockIdx.x, gridDim.x);
+    const int64_t c = blk * 64 + static_cast<int64_t>(threadIdx.x % 64);
+    if (c >= n || io.status[c] != 0) { return; }
+    // sub-chart index of column col in phase ph (-1: the phase does not depend on it)
+    auto sub = [](int ph, int col) {
+        if (col < 2) { return col; }
+        const int first = 3 + ph * (NC - 1);
+        return (col >= first && col < first + NC - 1) ? 2 + (col - first) : -1;
+    };
+    auto dcomp = [](int ph, int i, int col) {
+        const int first = 3 + ph * (NC - 1);
+        if (col < first || col >= first + NC - 1) { return 0.0; }
+        if (i == NC - 1) { return -1.0; }
+        return col - first == i ? 1.0 : 0.0;
+    };
+    const int ph_lo = cph < 0 ? 0 : cph, ph_hi = cph < 0 ? 1 : cph;
+
+    const double phi = G.poro[c];
+    const double s0 = io.q[2 * n + c];
+    const double sat[2] = {s0, 1.0 - s0};
+    const double inv_dt = 1.0 / io.dt;
+    const int64_t ostride = static_cast<int64_t>(G.max_degree) * n;
+    double res[ROWS];
+    if (chunk == 0) {
+        // accumulation residual and backward Euler (two_phase_accumulation.hpp:18-338,
+        // component_accumulation_time.hpp:189-300)
+        for (int i = 0; i < NC; ++i) {
+            double a = 0;
+            for (int ph = 0; ph < 2; ++ph) {
+                const int b = ph * R::kPhase;
+                a += phi * sat[ph] * rec[rix<NC>(b + R::kC, c)] * rec[rix<NC>(b + R::kX + i, c)];
+            }
+            res[i] = (a - io.prev[i * n + c]) * inv_dt;
         }
-        const double vc = G.bulk[c];
-        for (int32_t a = G.adj_off[c]; a < G.adj_off[c + 1]; ++a) {
-            const int32_t f = G.adj_face[a];
+        double fluid = 0;
+        for (int ph = 0; ph < 2; ++ph) {
+            const int b = ph * R::kPhase;
+            fluid += phi * sat[ph] * rec[rix<NC>(b + R::kRho, c)] * rec[rix<NC>(b + R::kU, c)];
+        }
+        const double e = fluid + (1.0 - phi) * (md.cv_rock * (rec[rix<NC>(R::kT, c)] - md.t0));
+        res[NC] = (e - io.prev[NC * n + c]) / io.dt;
+    }
+    // one column at a time: its accumulation derivative, then every face in slot order
+#pragma unroll 1
+    for (int cc = 0; cc < W; ++cc) {
+        if (cc >= cnt) { break; }
+        const int col = c0 + cc;
+        const bool with_residual = chunk == 0 && cc == 0;
+        double acc[ROWS];
+        {
+            const double dsat[2] = {col == 2 ? 1.0 : 0.0, col == 2 ? -1.0 : 0.0};
+            for (int i = 0; i < NC; ++i) {
+                double g = 0;
+                for (int ph = ph_lo; ph <= ph_hi; ++ph) {
+                    const int b = ph * R::kPhase, s = sub(ph, col);
+                    const double cv = rec[rix<NC>(b + R::kC, c)], xi = rec[rix<NC>(b + R::kX + i, c)];
+                    const double dc = s < 0 ? 0.0 : rec[rix<NC>(b + R::kDc + s, c)];
+                    g += phi * (dsat[ph] * cv * xi + sat[ph] * dc * xi + sat[ph] * cv * dcomp(ph, i, col));
+                }
+                acc[i] = g * inv_dt;
+            }
+            double g = (1.0 - phi) * (col == 1 ? md.cv_rock : 0.0);
+            for (int ph = ph_lo; ph <= ph_hi; ++ph) {
+                const int b = ph * R::kPhase, s = sub(ph, col);
+                const double rho = rec[rix<NC>(b + R::kRho, c)], u = rec[rix<NC>(b + R::kU, c)];
+                const double dr = s < 0 ? 0.0 : rec[rix<NC>(b + R::kDrho + s, c)];
+                const double du = s < 0 ? 0.0 : rec[rix<NC>(b + R::kDu + s, c)];
+                g += phi * (dsat[ph] * rho * u + sat[ph] * dr * u + sat[ph] * rho * du);
+            }
+            acc[NC] = g / io.dt;
+        }
+#pragma unroll 1
+        for (int j = 0; j < G.max_degree; ++j) {
+            const int32_t f = G.slot_face[static_cast<int64_t>(j) * n + c];
+            double* od = io.offdiag + static_cast<int64_t>(j) * n + c + static_cast<int64_t>(col) * ostride;
+            if (f < 0) {
+                // padding slot past the cell's degree
+                for (int r = 0; r < ROWS; ++r) { od[static_cast<int64_t>(r * Q) * ostride] = 0.0; }
+                continue;
+            }
             const int64_t o = G.owner[f], nb = G.neighbour[f];
             const bool side_owner = o == c;
-            const int64_t other = side_owner ? nb : o;
-            if (io.status[other] != 0) { continue; }
+            if (io.status[side_owner ? nb : o] != 0) { continue; }
             const double tf = G.trans[f], gd = G.gdot[f], gth = G.gth[f];
             const double vo = G.bulk[o], vn = G.bulk[nb];
-            // per phase: potential, upwind, flux (two_phase_transport.hpp:173-321,
-            // two_phase_tpfa.hpp:231-300)
-            double flux[2], lamu[2], delta[2];
-            bool own[2];
-            for (int ph = 0; ph < 2; ++ph) {
+            double jo[ROWS], jn[ROWS], rate[ROWS];
+            for (int r = 0; r < ROWS; ++r) { jo[r] = 0; jn[r] = 0; rate[r] = 0; }
+            for (int ph = ph_lo; ph <= ph_hi; ++ph) {
                 const int b = ph * R::kPhase;
-                const double rf = 0.5 * (rec[(b + R::kRho) * n + o] + rec[(b + R::kRho) * n + nb]);
-                delta[ph] = (rec[R::kP * n + nb] - rec[R::kP * n + o]) - rf * gd;
-                own[ph] = !(delta[ph] > 0.0);
-                lamu[ph] = rec[(b + R::kLam) * n + (own[ph] ? o : nb)];
-                flux[ph] = -tf * lamu[ph] * delta[ph];
+                // potential, upwind and flux (two_phase_transport.hpp:173-321,
+                // two_phase_tpfa.hpp:231-300)
+                const double rho_o = rec[rix<NC>(b + R::kRho, o)], rho_n = rec[rix<NC>(b + R::kRho, nb)];
+                const double rf = 0.5 * (rho_o + rho_n);
+                const double delta = (rec[rix<NC>(R::kP, nb)] - rec[rix<NC>(R::kP, o)]) - rf * gd;
+                const bool own = !(delta > 0.0);
+                const int64_t up = own ? o : nb;
+                const double lamu = rec[rix<NC>(b + R::kLam, up)];
+                const double flux = -tf * lamu * delta;
+                const double cu = rec[rix<NC>(b + R::kC, up)];
+                const double rhou = own ? rho_o : rho_n;
+                const double hu = rec[rix<NC>(b + R::kH, up)];
+                // derivatives wrt column col of the owner's and the neighbour's q
+                const int s = sub(ph, col);
+                auto sd = [&](int field, int64_t cell) { return s < 0 ? 0.0 : rec[rix<NC>(b + field + s, cell)]; };
+                const double dlo = (col == 2 ? rec[rix<NC>(b + R::kDlamS, o)] : 0.0) + sd(R::kDlam, o);
+                const double dln = (col == 2 ? rec[rix<NC>(b + R::kDlamS, nb)] : 0.0) + sd(R::kDlam, nb);
+                const double dro = sd(R::kDrho, o), drn = sd(R::kDrho, nb);
+                const double dphio = -(col == 0 ? 1.0 : 0.0) - 0.5 * dro * gd;
+                const double dphin = (col == 0 ? 1.0 : 0.0) - 0.5 * drn * gd;
+                const double fo = -tf * (lamu * dphio + delta * (own ? dlo : 0.0));
+                const double fn = -tf * (lamu * dphin + delta * (own ? 0.0 : dln));
+                const double dcu = sd(R::kDc, up);
+                const int first = 3 + ph * (NC - 1);
+                for (int i = 0; i < NC; ++i) {
+                    const double xi = rec[rix<NC>(b + R::kX + i, up)];
+                    const double content = cu * xi;
+                    double dx = 0;
+                    if (col >= first && col < first + NC - 1) { dx = i == NC - 1 ? -1.0 : (col - first == i ? 1.0 : 0.0); }
+                    const double dcontent = dcu * xi + cu * dx;
+                    jo[i] += (own ? dcontent : 0.0) * flux + content * fo;
+                    jn[i] += (own ? 0.0 : dcontent) * flux + content * fn;
+                    if (with_residual) { rate[i] += content * flux; }
+                }
+                const double ed = rhou * hu;
+                const double ded = (own ? dro : drn) * hu + rhou * sd(R::kDh, up);
+                jo[NC] += (own ? ded : 0.0) * flux + ed * fo;
+                jn[NC] += (own ? 0.0 : ded) * flux + ed * fn;
+                if (with_residual) { rate[NC] += ed * flux; }
             }
-            double rate[NC];
-            for (int i = 0; i < NC; ++i) {
-                double r = 0;
-                for (int ph = 0; ph < 2; ++ph) {
-                    const int b = ph * R::kPhase;
-                    const int64_t up = own[ph] ? o : nb;
-                    r += (rec[(b + R::kC) * n + up] * rec[(b + R::kX + i) * n + up]) * flux[ph];
-                }
-                rate[i] = r;
+            if (with_residual) {
+                // this cell's residual rows: owner +rate / V_o, neighbour -rate / V_n
+                for (int i = 0; i < NC; ++i) { res[i] += side_owner ? rate[i] / vo : -rate[i] / vn; }
+                const double etot = rate[NC] + gth * (rec[rix<NC>(R::kT, o)] - rec[rix<NC>(R::kT, nb)]);
+                res[NC] += side_owner ? etot / vo : -etot / vn;
             }
-            double erate = 0;
-            for (int ph = 0; ph < 2; ++ph) {
-                const int b = ph * R::kPhase;
-                const int64_t up = own[ph] ? o : nb;
-                erate += (rec[(b + R::kRho) * n + up] * rec[(b + R::kH) * n + up]) * flux[ph];
+            if (col == 1) {
+                jo[NC] += gth;
+                jn[NC] -= gth;
             }
-            const double etot = erate + gth * (rec[R::kT * n + o] - rec[R::kT * n + nb]);
-            // this cell's rows: owner +rate / V_o, neighbour -rate / V_n
-            for (int i = 0; i < NC; ++i) { res[i] += side_owner ? rate[i] / vo : -rate[i] / vn; }
-            res[NC] += side_owner ? etot / vo : -etot / vn;
-            // Jacobian columns of both cells: derivative of the face rates wrt the owner's
-            // q (jo) and the neighbour's q (jn); this cell's diagonal gets d/d(own q) and its
-            // off-diagonal block d/d(other q)
-            for (int col = 0; col < Q; ++col) {
-                double jo[ROWS], jn[ROWS];
-                for (int r = 0; r < ROWS; ++r) { jo[r] = 0; jn[r] = 0; }
-                for (int ph = 0; ph < 2; ++ph) {
-                    const int b = ph * R::kPhase;
-                    // sub-chart derivative of the phase's record fields wrt q column col
-                    const int first = 3 + ph * (NC - 1);
-                    const int s = col < 2 ? col : ((col >= first && col < first + NC - 1) ? 2 + (col - first) : -1);
-                    auto sd = [&](int field, int64_t cell) { return s < 0 ? 0.0 : rec[(b + field + s) * n + cell]; };
-                    const double dlo = (col == 2 ? rec[(b + R::kDlamS) * n + o] : 0.0) + (s < 0 ? 0.0 : rec[(b + R::kDlam + s) * n + o]);
-                    const double dln = (col == 2 ? rec[(b + R::kDlamS) * n + nb] : 0.0) + (s < 0 ? 0.0 : rec[(b + R::kDlam + s) * n + nb]);
-                    const double dphio = -(col == 0 ? 1.0 : 0.0) - 0.5 * sd(R::kDrho, o) * gd;
-                    const double dphin = (col == 0 ? 1.0 : 0.0) - 0.5 * sd(R::kDrho, nb) * gd;
-                    const double fo = -tf * (lamu[ph] * dphio + delta[ph] * (own[ph] ? dlo : 0.0));
-                    const double fn = -tf * (lamu[ph] * dphin + delta[ph] * (own[ph] ? 0.0 : dln));
-                    const int64_t up = own[ph] ? o : nb;
-                    const double cc = rec[(b + R::kC) * n + up];
-                    const double dcu = sd(R::kDc, up);
-                    for (int i = 0; i < NC; ++i) {
-                        const double xi = rec[(b + R::kX + i) * n + up];
-                        const double content = cc * xi;
-                        double dx = 0;
-                        if (col >= first && col < first + NC - 1) { dx = i == NC - 1 ? -1.0 : (col - first == i ? 1.0 : 0.0); }
-                        const double dcontent = dcu * xi + cc * dx;
-                        jo[i] += (own[ph] ? dcontent : 0.0) * flux[ph] + content * fo;
-                        jn[i] += (own[ph] ? 0.0 : dcontent) * flux[ph] + content * fn;
-                    }
-                    const double rho = rec[(b + R::kRho) * n + up], h = rec[(b + R::kH) * n + up];
-                    const double ed = rho * h;
-                    const double ded = sd(R::kDrho, up) * h + rho * sd(R::kDh, up);
-                    jo[NC] += (own[ph] ? ded : 0.0) * flux[ph] + ed * fo;
-                    jn[NC] += (own[ph] ? 0.0 : ded) * flux[ph] + ed * fn;
+            // this cell's diagonal gets d/d(own q), its slot d/d(other q)
+            for (int r = 0; r < ROWS; ++r) {
+                double dself, doff;
+                if (side_owner) {
+                    dself = jo[r] / vo;
+                    doff = jn[r] / vo;
+                } else if (r < NC) {
+                    dself = -jn[r] / vn;
+                    doff = -jo[r] / vn;
+                } else {
+                    dself = jn[r] * (-1.0 / vn);
+                    doff = jo[r] * (-1.0 / vn);
                 }
-                if (col == 1) {
-                    jo[NC] += gth;
-                    jn[NC] -= gth;
-                }
-                for (int r = 0; r < ROWS; ++r) {
-                    double dself, doff;
-                    if (side_owner) {
-                        dself = jo[r] / vo;
-                        doff = jn[r] / vo;
-                    } else if (r < NC) {
-                        dself = -jn[r] / vn;
-                        doff = -jo[r] / vn;
-                    } else {
-                        dself = jn[r] * (-1.0 / vn);
-                        doff = jo[r] * (-1.0 / vn);
-                    }
-                    dg[r * Q + col] += dself;
-                    io.offdiag[static_cast<int64_t>(r * Q + col) * nadj + a] = doff;
-                }
+                acc[r] += dself;
+                od[static_cast<int64_t>(r * Q) * ostride] = doff;
             }
         }
-        (void)vc;
-        for (int r = 0; r < ROWS; ++r) {
-            io.residual[r * n + c] = res[r];
-            for (int col = 0; col < Q; ++col) { io.diag[(r * Q + col) * n + c] = dg[r * Q + col]; }
-        }
+        for (int r = 0; r < ROWS; ++r) { io.diag[(r * Q + col) * n + c] = acc[r]; }
+    }
+    if (chunk == 0) {
+        for (int r = 0; r < ROWS; ++r) { io.residual[r * n + c] = res[r]; }
     }
 }
 
 template <int NC>
 size_t asm_record_bytes(int64_t ncell) {
-    return static_cast<size_t>(Rec<NC>::kSize) * static_cast<size_t>(ncell) * sizeof(double);
+    return static_cast<size_t>(Rec<NC>::kSize) * static_cast<size_t>((ncell + 63) / 64 * 64) * sizeof(double);
 }
 
 // The domain-error flag is cleared by a kernel rather than a memset so that the launch
@@ -392,7 +454,8 @@ hipError_t launch_assembly(const double* mixc, const AsmModelDev& md, const AsmG
         return hipGetLastError();
     }
     hipLaunchKernelGGL((k_asm_cell<NC, false>), dim3(blocks), dim3(64), 0, s, mixc, md, G, io);
-    hipLaunchKernelGGL(k_asm_rows<NC>, dim3(blocks), dim3(64), 0, s, G, io);
+    const unsigned row_blocks = static_cast<unsigned>((G.ncell + 63) / 64);
+    hipLaunchKernelGGL(k_asm_rows<NC>, dim3(row_blocks), dim3(64 * Chunks<NC>::N), 0, s, md, G, io);
     return hipGetLastError();
 }
 

@@ -379,6 +379,8 @@ struct mpmc_tpfa_grid {
     const int32_t *owner, *neighbour;
     int32_t* adj_off = nullptr;  // device CSR cell -> faces
     int32_t* adj_face = nullptr;
+    int32_t* slot_face = nullptr; // device [max_degree][ncell], -1 past a cell's degree
+    int32_t max_degree = 0;
     void* rec = nullptr;         // per-cell face records
     size_t rec_bytes = 0;
     unsigned long long* flag = nullptr;
@@ -419,14 +421,26 @@ int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk, cons
         adj[fill[ho[f]]++] = static_cast<int32_t>(f);
         adj[fill[hn[f]]++] = static_cast<int32_t>(f);
     }
+    // the same adjacency by (slot, cell), cell fastest: the layout of the off-diagonal blocks
+    int32_t maxdeg = 0;
+    for (int64_t c = 0; c < ncell; ++c) { maxdeg = std::max(maxdeg, off[c + 1] - off[c]); }
+    std::vector<int32_t> slots(static_cast<size_t>(maxdeg) * static_cast<size_t>(ncell), -1);
+    for (int64_t c = 0; c < ncell; ++c) {
+        for (int32_t a = off[c]; a < off[c + 1]; ++a) { slots[static_cast<size_t>(a - off[c]) * ncell + c] = adj[a]; }
+    }
     auto* g = new (std::nothrow) mpmc_tpfa_grid{ncell, nface, bulk, poro, trans, gdot, gth, owner, neighbour};
     if (!g) { return fail(MPMC_E_NO_MEMORY, "tpfa_grid_create: host allocation"); }
+    g->max_degree = maxdeg;
     e = hipMalloc(&g->adj_off, sizeof(int32_t) * off.size());
     if (e == hipSuccess) { e = hipMalloc(&g->adj_face, sizeof(int32_t) * std::max<size_t>(1, adj.size())); }
+    if (e == hipSuccess) { e = hipMalloc(&g->slot_face, sizeof(int32_t) * std::max<size_t>(1, slots.size())); }
     if (e == hipSuccess) { e = hipMalloc(&g->flag, 256); }
     if (e == hipSuccess) { e = hipMemcpy(g->adj_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice); }
     if (e == hipSuccess && !adj.empty()) {
         e = hipMemcpy(g->adj_face, adj.data(), sizeof(int32_t) * adj.size(), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !slots.empty()) {
+        e = hipMemcpy(g->slot_face, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         mpmc_tpfa_grid_destroy(g);
@@ -438,8 +452,8 @@ int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk, cons
 
 int mpmc_tpfa_grid_destroy(mpmc_tpfa_grid_t g) {
     if (!g) { return MPMC_OK; }
-    for (void* ptr : {static_cast<void*>(g->adj_off), static_cast<void*>(g->adj_face), g->rec,
-                      static_cast<void*>(g->flag)}) {
+    for (void* ptr : {static_cast<void*>(g->adj_off), static_cast<void*>(g->adj_face),
+                      static_cast<void*>(g->slot_face), g->rec, static_cast<void*>(g->flag)}) {
         if (ptr) { (void)hipFree(ptr); }
     }
     delete g;
@@ -450,6 +464,12 @@ int mpmc_tpfa_grid_adjacency(mpmc_tpfa_grid_t g, const int32_t** adj_off, const 
     if (!g || !adj_off || !adj_face) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_adjacency: null argument"); }
     *adj_off = g->adj_off;
     *adj_face = g->adj_face;
+    return MPMC_OK;
+}
+
+int mpmc_tpfa_grid_max_degree(mpmc_tpfa_grid_t g, int32_t* max_degree) {
+    if (!g || !max_degree) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_max_degree: null argument"); }
+    *max_degree = g->max_degree;
     return MPMC_OK;
 }
 
@@ -489,7 +509,7 @@ static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assemb
         for (const void* ptr : more) {
             if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: buffers must be device memory"); }
         }
-        if (g->nface > 0 && !is_device_ptr(out->offdiag)) {
+        if (g->max_degree > 0 && !is_device_ptr(out->offdiag)) {
             return fail(MPMC_E_INVALID_ARGUMENT, "assembly: offdiag must be device memory");
         }
     }
@@ -517,8 +537,9 @@ static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assemb
     mpmc::dev::AsmModelDev dm{};
     dm.mu_a0 = md->mu_a0; dm.mu_a1 = md->mu_a1; dm.cp = md->cp; dm.t0 = md->t0; dm.cv_rock = md->cv_rock;
     for (int i = 0; i < MPMC_MAX_COMPONENTS; ++i) { dm.molar_mass[i] = md->molar_mass[i]; }
-    const mpmc::dev::AsmGrid G{g->ncell, g->nface, g->bulk, g->poro, g->owner, g->neighbour, g->trans, g->gdot,
-                               g->gth, g->adj_off, g->adj_face};
+    const mpmc::dev::AsmGrid G{g->ncell, g->nface,    g->bulk,     g->poro,      g->owner,     g->neighbour,
+                               g->trans, g->gdot,     g->gth,      g->adj_off,   g->adj_face,  g->slot_face,
+                               g->max_degree};
     const mpmc::dev::AsmIo io{q, branch, prev, dt, out->cell_status, out->residual, out->diag, out->offdiag,
                               static_cast<double*>(g->rec), g->flag};
     e = hipErrorInvalidValue;

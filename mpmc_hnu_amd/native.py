@@ -31,6 +31,7 @@ EXPORTED = (
     "mpmc_tpfa_grid_create",
     "mpmc_tpfa_grid_destroy",
     "mpmc_tpfa_grid_adjacency",
+    "mpmc_tpfa_grid_max_degree",
     "mpmc_assembly_default_model",
     "mpmc_assembly_2p",
     "mpmc_inventory_2p",
@@ -161,6 +162,7 @@ def library() -> ctypes.CDLL:
     lib.mpmc_tpfa_grid_create.argtypes = [ctypes.c_int64, ctypes.c_int64] + [P] * 7 + [ctypes.POINTER(P)]
     lib.mpmc_tpfa_grid_destroy.argtypes = [P]
     lib.mpmc_tpfa_grid_adjacency.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
+    lib.mpmc_tpfa_grid_max_degree.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     lib.mpmc_assembly_default_model.argtypes = [ctypes.c_int, ctypes.POINTER(AssemblyModel)]
     lib.mpmc_assembly_default_model.restype = None
     lib.mpmc_assembly_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, ctypes.c_double,

@@ -76,10 +76,15 @@ def test_gpu_assembly_matches_oracle(gpu, nc, dims, low_p):
     ok, worst = _close(gd, od)
     print(f"  diagonal blocks worst rel {worst:.3e}", flush=True)
     assert ok, worst
-    # off-diagonal blocks: CSR position a of cell c <-> (face, side)
+    # off-diagonal blocks: slot j of cell c is CSR position a = off[c] + j <-> (face, side);
+    # padding slots past a cell's degree are zero
     off, adj = A.adjacency_host(grid["owner"], grid["neighbour"], n)
     rows = nc + 1
-    go = g["offdiag"].reshape(rows, Q, -1)
+    deg = np.diff(off)
+    assert g["offdiag"].shape[1:] == (deg.max(), n)
+    for j in range(deg.max()):
+        assert np.all(g["offdiag"][:, j, deg <= j] == 0.0)
+    go = A.ell_to_csr(g["offdiag"], off, grid["owner"].size).reshape(rows, Q, -1)
     worst_all = 0.0
     for c in range(n):
         for a in range(off[c], off[c + 1]):
