@@ -116,6 +116,9 @@ __device__ __forceinline__ int qcol(int phase, int s) {
     return s < 2 ? s : 3 + phase * (NC - 1) + (s - 2);
 }
 
+// One wave = 32 cells x 2 phases: lane l evaluates phase l / 32 of cell l % 32, so each lane
+// holds one phase's Dual state. The two lanes of a cell meet through cross-lane shuffles
+// for what needs both phases (status, inventories, equilibrium rows).
 template <int NC, bool kInventory>
 __global__ __launch_bounds__(64) void k_asm_cell(const double* __restrict__ mixc, AsmModelDev md, AsmGrid G, AsmIo io) {
     constexpr int Q = 2 * NC + 1;
@@ -126,126 +129,143 @@ __global__ __launch_bounds__(64) void k_asm_cell(const double* __restrict__ mixc
     load_mix<NC>(mixc, smix);
     __syncthreads();
     const int64_t n = G.ncell;
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        double qv[Q];
-        for (int k = 0; k < Q; ++k) { qv[k] = io.q[k * n + c]; }
-        const double P = qv[0], T = qv[1];
-        const double phi = G.poro[c];
-        double sat[2] = {qv[2], 1.0 - qv[2]};
-        double x[2][NC];
-        double lnp[2][NC], J[2][NC * S];
-        double cval[2], rhov[2], muv[2], hv[2], uv[2];
-        double dcv[2][S], drho[2][S], dmu[2][S], dhv[2][S], duv[2][S];
-        bool ok = true;
-        for (int ph = 0; ph < 2 && ok; ++ph) {
-            // reconstruct_composition (pr76_selected_phase_property_closure.hpp:115-151)
+    const int ph = static_cast<int>(threadIdx.x) >> 5;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * 32 + (threadIdx.x & 31);
+    const bool live = c < n;
+    // both phases' compositions (the equilibrium rows need the other phase's x as well)
+    double P = 1.0, T = 1.0, s0 = 0.0;
+    double x[2][NC];
+    bool ok = live;
+    for (int q = 0; q < 2; ++q) {
+        for (int k = 0; k < NC; ++k) { x[q][k] = 0.5; }
+    }
+    if (live) {
+        P = io.q[c];
+        T = io.q[n + c];
+        s0 = io.q[2 * n + c];
+        // reconstruct_composition (pr76_selected_phase_property_closure.hpp:115-151)
+        for (int q = 0; q < 2; ++q) {
             double sum = 0;
             for (int k = 0; k + 1 < NC; ++k) {
-                x[ph][k] = qv[3 + ph * (NC - 1) + k];
-                ok = ok && finite(x[ph][k]) && x[ph][k] > 0;
-                sum += x[ph][k];
+                x[q][k] = io.q[(3 + q * (NC - 1) + k) * n + c];
+                ok = ok && finite(x[q][k]) && x[q][k] > 0;
+                sum += x[q][k];
             }
-            x[ph][NC - 1] = 1.0 - sum;
-            ok = ok && finite(x[ph][NC - 1]) && x[ph][NC - 1] > 0 && finite(P) && P > 0 && finite(T) && T > 0;
-            if (!ok) { break; }
-            double zv, zg[S];
-            if (phase_dual<NC>(smix, P, T, x[ph], io.branch[ph * n + c], zv, zg, lnp[ph], J[ph]) != PF_OK) {
-                ok = false;
-                break;
-            }
-            // molar mass, molar density p / (Z R T), mass density, mu, h, u in dual arithmetic
-            D Pd = D::seed(P, 0), Td = D::seed(T, 1);
-            D mm(0.0), isum(0.0);
-            for (int k = 0; k + 1 < NC; ++k) {
-                const D xk = D::seed(x[ph][k], 2 + k);
-                isum = isum + xk;
-                mm = mm + xk * md.molar_mass[k];
-            }
-            const D xl = D(1.0) - isum;
-            mm = mm + xl * md.molar_mass[NC - 1];
-            D z;
-            z.v = zv;
-            for (int k = 0; k < S; ++k) { z.d[k] = zg[k]; }
-            const D den = (z * smix.r_gas) * Td;
-            const D cd = Pd / den;
-            const D rho = cd * mm;
-            const D mu = md.mu_a0 + md.mu_a1 * (rho * rho);
-            const D h = md.cp * (Td - md.t0);
-            const D u = h - Pd / rho;
-            ok = ok && mm.finite() && mm.v > 0 && cd.finite() && cd.v > 0 && rho.finite() && rho.v > 0 &&
-                 mu.finite() && mu.v > 0 && h.finite() && u.finite();
-            cval[ph] = cd.v; rhov[ph] = rho.v; muv[ph] = mu.v; hv[ph] = h.v; uv[ph] = u.v;
-            for (int k = 0; k < S; ++k) {
-                dcv[ph][k] = cd.d[k]; drho[ph][k] = rho.d[k]; dmu[ph][k] = mu.d[k]; dhv[ph][k] = h.d[k];
-                duv[ph][k] = u.d[k];
-            }
+            x[q][NC - 1] = 1.0 - sum;
+            ok = ok && finite(x[q][NC - 1]) && x[q][NC - 1] > 0;
         }
-        if (!ok) {
-            io.status[c] = 1;
-            atomicOr(io.flag, 1ull);
-            continue;
+        ok = ok && finite(P) && P > 0 && finite(T) && T > 0;
+    }
+    const double sat = ph == 0 ? s0 : 1.0 - s0;
+    // this lane's phase composition, selected without a run-time array index
+    double xp[NC];
+    for (int k = 0; k < NC; ++k) { xp[k] = ph == 0 ? x[0][k] : x[1][k]; }
+    double zv = 0, zg[S], lnp[NC], J[NC * S];
+    double cval = 0, rhov = 0, muv = 0, hv = 0, uv = 0;
+    double dcv[S], drho[S], dmu[S], dhv[S], duv[S];
+    if (ok) {
+        ok = phase_dual_body<NC>(smix, P, T, xp, io.branch[ph * n + c], zv, zg, lnp, J) == PF_OK;
+    }
+    if (ok) {
+        // molar mass, molar density p / (Z R T), mass density, mu, h, u in dual arithmetic
+        D Pd = D::seed(P, 0), Td = D::seed(T, 1);
+        D mm(0.0), isum(0.0);
+        for (int k = 0; k + 1 < NC; ++k) {
+            const D xk = D::seed(xp[k], 2 + k);
+            isum = isum + xk;
+            mm = mm + xk * md.molar_mass[k];
         }
-        io.status[c] = 0;
-        if constexpr (kInventory) {
-            // N_i and E at this state (previous-time inventories), into residual rows 0..nc
-            for (int i = 0; i < NC; ++i) {
-                double acc = 0;
-                for (int ph = 0; ph < 2; ++ph) { acc += phi * sat[ph] * cval[ph] * x[ph][i]; }
-                io.residual[i * n + c] = acc;
-            }
-            double fluid = 0;
-            for (int ph = 0; ph < 2; ++ph) { fluid += phi * sat[ph] * rhov[ph] * uv[ph]; }
-            io.residual[NC * n + c] = fluid + (1.0 - phi) * (md.cv_rock * (T - md.t0));
-            continue;
+        const D xl = D(1.0) - isum;
+        mm = mm + xl * md.molar_mass[NC - 1];
+        D z;
+        z.v = zv;
+        for (int k = 0; k < S; ++k) { z.d[k] = zg[k]; }
+        const D den = (z * smix.r_gas) * Td;
+        const D cd = Pd / den;
+        const D rho = cd * mm;
+        const D mu = md.mu_a0 + md.mu_a1 * (rho * rho);
+        const D h = md.cp * (Td - md.t0);
+        const D u = h - Pd / rho;
+        ok = mm.finite() && mm.v > 0 && cd.finite() && cd.v > 0 && rho.finite() && rho.v > 0 && mu.finite() &&
+             mu.v > 0 && h.finite() && u.finite();
+        cval = cd.v; rhov = rho.v; muv = mu.v; hv = h.v; uv = u.v;
+        for (int k = 0; k < S; ++k) {
+            dcv[k] = cd.d[k]; drho[k] = rho.d[k]; dmu[k] = mu.d[k]; dhv[k] = h.d[k]; duv[k] = u.d[k];
         }
-        double* rec = io.rec;
-        // mobility with Corey kr = S^2 (two_phase_transport.hpp:92-125)
-        for (int ph = 0; ph < 2; ++ph) {
-            const double kr = sat[ph] * sat[ph];
-            const double dkr = ph == 0 ? 2.0 * sat[0] : -2.0 * sat[1];
-            const double mu = muv[ph], mu2 = mu * mu;
-            const int b = ph * R::kPhase;
-            rec[rix<NC>(b + R::kLam, c)] = kr / mu;
-            for (int k = 0; k < S; ++k) { rec[rix<NC>(b + R::kDlam + k, c)] = (0.0 * mu - kr * dmu[ph][k]) / mu2; }
-            rec[rix<NC>(b + R::kDlamS, c)] = (dkr * mu - kr * 0.0) / mu2;
-            rec[rix<NC>(b + R::kRho, c)] = rhov[ph];
-            rec[rix<NC>(b + R::kC, c)] = cval[ph];
-            rec[rix<NC>(b + R::kH, c)] = hv[ph];
-            rec[rix<NC>(b + R::kU, c)] = uv[ph];
-            for (int k = 0; k < S; ++k) {
-                rec[rix<NC>(b + R::kDrho + k, c)] = drho[ph][k];
-                rec[rix<NC>(b + R::kDc + k, c)] = dcv[ph][k];
-                rec[rix<NC>(b + R::kDh + k, c)] = dhv[ph][k];
-                rec[rix<NC>(b + R::kDu + k, c)] = duv[ph][k];
-            }
-            for (int i = 0; i < NC; ++i) { rec[rix<NC>(b + R::kX + i, c)] = x[ph][i]; }
-        }
-        rec[rix<NC>(R::kP, c)] = P;
-        rec[rix<NC>(R::kT, c)] = T;
-        // equilibrium rows: ln(x0_i / x1_i) + ln phi0_i - ln phi1_i (+ ln(p/p) = 0); the
-        // component and energy rows are formed by the face pass
-        auto dcomp = [](int ph, int i, int col) {
-            const int first = 3 + ph * (NC - 1);
-            if (col < first || col >= first + NC - 1) { return 0.0; }
-            if (i == NC - 1) { return -1.0; }
-            return col - first == i ? 1.0 : 0.0;
-        };
-        auto subd = [](const double* g, int ph, int col) {
-            // q column -> sub-chart derivative of phase ph (zero outside its chart)
-            if (col < 2) { return g[col]; }
-            const int first = 3 + ph * (NC - 1);
-            if (col < first || col >= first + NC - 1) { return 0.0; }
-            return g[2 + (col - first)];
-        };
+    }
+    // a cell is evaluated when both of its phases are
+    const int ok_other = __shfl_xor(ok ? 1 : 0, 32);
+    const bool cell_ok = ok && ok_other != 0;
+    if (live && ph == 0) {
+        io.status[c] = cell_ok ? 0 : 1;
+        if (!cell_ok) { atomicOr(io.flag, 1ull); }
+    }
+    if constexpr (kInventory) {
+        // N_i and E at this state (previous-time inventories), into residual rows 0..nc
+        const double phi = live ? G.poro[c] : 0.0;
         for (int i = 0; i < NC; ++i) {
-            const double r = x[0][i] / x[1][i];
-            io.residual[(NC + 1 + i) * n + c] = ((log(r) + lnp[0][i]) - lnp[1][i]) + 0.0;
-            for (int col = 0; col < Q; ++col) {
-                const double dr = (dcomp(0, i, col) - r * dcomp(1, i, col)) / x[1][i];
-                const double g = ((dr / r + subd(&J[0][i * S], 0, col)) - subd(&J[1][i * S], 1, col)) + 0.0;
-                io.diag[((NC + 1 + i) * Q + col) * n + c] = g;
-            }
+            const double mine = phi * sat * cval * xp[i];
+            const double theirs = __shfl_down(mine, 32);
+            if (live && cell_ok && ph == 0) { io.residual[i * n + c] = (0.0 + mine) + theirs; }
+        }
+        const double mine = phi * sat * rhov * uv;
+        const double theirs = __shfl_down(mine, 32);
+        if (live && cell_ok && ph == 0) {
+            io.residual[NC * n + c] = ((0.0 + mine) + theirs) + (1.0 - phi) * (md.cv_rock * (T - md.t0));
+        }
+        return;
+    }
+    double* rec = io.rec;
+    if (live && cell_ok) {
+        // mobility with Corey kr = S^2 (two_phase_transport.hpp:92-125)
+        const double kr = sat * sat;
+        const double dkr = ph == 0 ? 2.0 * sat : -2.0 * sat;
+        const double mu = muv, mu2 = mu * mu;
+        const int b = ph * R::kPhase;
+        rec[rix<NC>(b + R::kLam, c)] = kr / mu;
+        for (int k = 0; k < S; ++k) { rec[rix<NC>(b + R::kDlam + k, c)] = (0.0 * mu - kr * dmu[k]) / mu2; }
+        rec[rix<NC>(b + R::kDlamS, c)] = (dkr * mu - kr * 0.0) / mu2;
+        rec[rix<NC>(b + R::kRho, c)] = rhov;
+        rec[rix<NC>(b + R::kC, c)] = cval;
+        rec[rix<NC>(b + R::kH, c)] = hv;
+        rec[rix<NC>(b + R::kU, c)] = uv;
+        for (int k = 0; k < S; ++k) {
+            rec[rix<NC>(b + R::kDrho + k, c)] = drho[k];
+            rec[rix<NC>(b + R::kDc + k, c)] = dcv[k];
+            rec[rix<NC>(b + R::kDh + k, c)] = dhv[k];
+            rec[rix<NC>(b + R::kDu + k, c)] = duv[k];
+        }
+        for (int i = 0; i < NC; ++i) { rec[rix<NC>(b + R::kX + i, c)] = xp[i]; }
+        if (ph == 0) {
+            rec[rix<NC>(R::kP, c)] = P;
+            rec[rix<NC>(R::kT, c)] = T;
+        }
+    }
+    // equilibrium rows ln(x0_i / x1_i) + ln phi0_i - ln phi1_i (+ ln(p/p) = 0): phase-0
+    // lanes write the residual and the p, T, S_0 and x_0 columns (with phase 1's ln phi
+    // and its p, T derivatives from the partner lane), phase-1 lanes the x_1 columns.
+    // The component and energy rows are formed by the face pass.
+    for (int i = 0; i < NC; ++i) {
+        const double lnp1 = __shfl_down(lnp[i], 32);
+        const double j1p = __shfl_down(J[i * S + 0], 32);
+        const double j1t = __shfl_down(J[i * S + 1], 32);
+        if (!(live && cell_ok)) { continue; }
+        const double r = x[0][i] / x[1][i];
+        const int first = 3 + ph * (NC - 1);
+        if (ph == 0) {
+            io.residual[(NC + 1 + i) * n + c] = ((log(r) + lnp[i]) - lnp1) + 0.0;
+            // p and T: both phases depend on them; S_0: neither
+            io.diag[((NC + 1 + i) * Q + 0) * n + c] = ((0.0 / r + J[i * S + 0]) - j1p) + 0.0;
+            io.diag[((NC + 1 + i) * Q + 1) * n + c] = ((0.0 / r + J[i * S + 1]) - j1t) + 0.0;
+            io.diag[((NC + 1 + i) * Q + 2) * n + c] = ((0.0 / r + 0.0) - 0.0) + 0.0;
+        }
+        for (int k = 0; k + 1 < NC; ++k) {
+            // column first + k: x_ph,k (and x_ph,last = 1 - sum through the dependent entry)
+            const double dcomp = i == NC - 1 ? -1.0 : (k == i ? 1.0 : 0.0);
+            const double dr = ph == 0 ? (dcomp - r * 0.0) / x[1][i] : (0.0 - r * dcomp) / x[1][i];
+            const double g = ph == 0 ? ((dr / r + J[i * S + 2 + k]) - 0.0) + 0.0
+                                     : ((dr / r + 0.0) - J[i * S + 2 + k]) + 0.0;
+            io.diag[((NC + 1 + i) * Q + first + k) * n + c] = g;
         }
     }
 }
@@ -447,7 +467,7 @@ static __global__ void k_asm_clear(unsigned long long* flag) {
 template <int NC>
 hipError_t launch_assembly(const double* mixc, const AsmModelDev& md, const AsmGrid& G, const AsmIo& io, bool inventory,
                            hipStream_t s) {
-    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((G.ncell + 63) / 64, 65536));
+    const unsigned blocks = static_cast<unsigned>((G.ncell + 31) / 32);
     hipLaunchKernelGGL(k_asm_clear, dim3(1), dim3(64), 0, s, io.flag);
     if (inventory) {
         hipLaunchKernelGGL((k_asm_cell<NC, true>), dim3(blocks), dim3(64), 0, s, mixc, md, G, io);

@@ -72,8 +72,8 @@ enum PhaseFailure : int { PF_OK = 0, PF_PHASE, PF_RANGE, PF_DOMAIN, PF_INDEX };
 // implicit-function step of pr76_phase.hpp:209-222. Writes Z and its gradient,
 // ln(phi) and its Jacobian J[i * (NC+1) + k].
 template <int NC>
-__device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, const double* xr, int root_index,
-                                       double& zval, double* zgrad, double* lnphi, double* J) {
+__device__ __forceinline__ int phase_dual_body(const Mix<NC>& m, double p, double t, const double* xr, int root_index,
+                                               double& zval, double* zgrad, double* lnphi, double* J) {
     constexpr int Q = NC + 1;
     using D = Dual<Q>;
     if (!finite(p) || !(p > 0)) { return PF_DOMAIN; }
@@ -90,6 +90,7 @@ __device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, con
     D roots[NC], sums[NC];
     D att(0.0), cov(0.0);
     const D st = dsqrt(T);
+#pragma unroll
     for (int i = 0; i < NC; ++i) {
         const D xi = i == NC - 1 ? last : D::seed(xr[i], 2 + i);
         const D rr = st / m.sqrt_tc[i];
@@ -106,8 +107,10 @@ __device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, con
             if (!roots[i].finite()) { return PF_RANGE; }
         }
     }
+#pragma unroll
     for (int i = 0; i < NC; ++i) {
         const D xi = i == NC - 1 ? last : D::seed(xr[i], 2 + i);
+#pragma unroll
         for (int j = i + 1; j < NC; ++j) {
             const D xj = j == NC - 1 ? last : D::seed(xr[j], 2 + j);
             const double f = m.factor[i * NC + j];
@@ -131,7 +134,7 @@ __device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, con
     const RootSet rs = pr76_roots<false>(A.v, B.v);
     if (rs.status != RS_SUCCESS) { return PF_PHASE; }
     if (root_index < 0 || root_index >= rs.count) { return PF_INDEX; }
-    const Root r = rs.r[root_index];
+    const Root r = root_index == 0 ? rs.r[0] : (root_index == 1 ? rs.r[1] : rs.r[2]);
     if (!r.dvalid) { return PF_PHASE; }
     const double scale = smax(smax(1.0, B.v), __builtin_sqrt(__builtin_fabs(A.v)));
     const double y0 = r.x / scale;
@@ -164,6 +167,7 @@ __device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, con
         zm1 = B / fv - (A / z) / ((1.0 + 2.0 * bz) - bz * bz);
         lfree = dlog1p(zm1 - B);
     }
+#pragma unroll
     for (int i = 0; i < NC; ++i) {
         const D ratio = m.b[i] / cov;
         const D coef = ((2.0 * sums[i] - att * ratio) / rt) * prt;
@@ -176,6 +180,13 @@ __device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, con
     zval = z.v;
     for (int k = 0; k < Q; ++k) { zgrad[k] = z.d[k]; }
     return PF_OK;
+}
+
+// Out-of-line copy for callers that evaluate several phases per lane.
+template <int NC>
+__device__ __noinline__ int phase_dual(const Mix<NC>& m, double p, double t, const double* xr, int root_index,
+                                       double& zval, double* zgrad, double* lnphi, double* J) {
+    return phase_dual_body<NC>(m, p, t, xr, root_index, zval, zgrad, lnphi, J);
 }
 
 __device__ __forceinline__ bool closure_same_z(double v, double accepted) { // pr76_thermodynamic_closure.hpp:70-79
