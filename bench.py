@@ -67,6 +67,12 @@ def cpu_baseline(mix, p, t, z, sample: int):
     }
 
 
+# HBM bytes of one Newton evaluation at nc=6 on 100x100x50, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counters (k_asm_cell + k_asm_rows), each scaled by what the counter reads for
+# a plain 1 GiB streaming kernel (profiles/assembly_2p_nc6_100x100x50_kernel_stats.md).
+ASSEMBLY_TRAFFIC_BYTES = (36615 + 1072624) * 2.0 * 1024 + (728516 + 2578166) * 1.0 * 1024
+
+
 def assembly_leg(dims, steps: int, warmup: int) -> dict:
     """One Newton evaluation of the two-phase natural-variable system (residual + block
     Jacobian) on a structured grid, nc=6, inputs resident in HBM."""
@@ -113,10 +119,39 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
           f"({ncell} cells, {nface} faces)", file=sys.stderr, flush=True)
     g.close()
     m.close()
-    return {"grid": list(dims), "cells": ncell, "faces": int(nface), "unknowns_per_cell": Q, "ms": ms,
-            "algorithmic_bytes": algo_bytes, "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS,
-            "frac": gbs / PEAK_HBM_GBS,
-            "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables"}
+    out = {"grid": list(dims), "cells": ncell, "faces": int(nface), "unknowns_per_cell": Q, "ms": ms,
+           "algorithmic_bytes": algo_bytes, "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS,
+           "frac": gbs / PEAK_HBM_GBS,
+           "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables"}
+    if tuple(dims) == (100, 100, 50):
+        out["roofline"] = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": gbs / PEAK_HBM_GBS, "traffic": ASSEMBLY_TRAFFIC_BYTES,
+                           "traffic_source": "profiles/assembly_2p_nc6_100x100x50_kernel_stats.md"}
+    return out
+
+
+def copy_leg(steps: int) -> dict:
+    """Plain streaming kernel over 1 GiB (2^27 doubles, dst = -src: one elementwise read
+    and one write per element): the calibration kernel for HBM byte counters
+    (FETCH_SIZE / WRITE_SIZE read 1 GiB each per launch when exact)."""
+    import torch
+
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        torch.neg(src, out=dst)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    gbs = 2.0 * n * 8 / (ms * 1e-3) / 1e9
+    print(f"[bench] copy: {ms:.3f} ms per 1 GiB copy, {gbs:.0f} GB/s read+write", file=sys.stderr, flush=True)
+    return {"bytes_each_way": n * 8, "ms": ms, "gbs_read_plus_write": gbs}
 
 
 def serve_leg(seconds: float) -> dict:
@@ -167,7 +202,8 @@ def main(argv=None):
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals for the serving leg (0: skip)")
     ap.add_argument("--legs", type=str, default="flash,assembly,serving",
-                    help="comma list of legs to run; the JSON line is complete only with all three")
+                    help="comma list of legs to run (flash, assembly, serving; copy: a 1 GiB device copy that "
+                         "calibrates HBM byte counters); the JSON line is complete only with the first three")
     a = ap.parse_args(argv)
     legs = set(a.legs.split(","))
     if a.gpus != 1:
@@ -183,6 +219,8 @@ def main(argv=None):
     if "flash" not in legs:
         # a partial run (profiling one leg): print that leg's object only
         part = {}
+        if "copy" in legs:
+            part["copy"] = copy_leg(a.steps)
         if "assembly" in legs and a.assembly_dims:
             part["assembly"] = assembly_leg(tuple(int(v) for v in a.assembly_dims.split(",")), a.steps, a.warmup)
         if "serving" in legs and a.serve_seconds > 0:

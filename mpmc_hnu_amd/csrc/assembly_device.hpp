@@ -7,24 +7,26 @@
 // unknowns (two_phase_cell_state.hpp:120-231). Rows per cell: nc component balances,
 // one energy balance, nc fugacity-equilibrium rows.
 //
-// k_asm_cell, one lane per cell:
-//   * each phase's selected-branch properties over its own sub-chart (p, T, x_p independent)
-//     in one Dual<nc+1> pass (the closure's phase_dual, then molar mass, molar density,
-//     mass density, viscosity, enthalpy, internal energy: the selected-phase property
-//     closure of pr76_selected_phase_property_closure.hpp:336-465 and its chart, :533-820);
-//   * equilibrium rows ln(x0/x1) + ln phi0 - ln phi1 and their Jacobian;
+// k_asm_cell, one lane per (cell, phase), the two phases of a cell in the two halves of
+// one wave:
+//   * the phase's selected-branch properties over its own sub-chart (p, T, x_p independent)
+//     in one Dual<nc+1> pass (the closure's phase evaluation, then molar mass, molar
+//     density, mass density, viscosity, enthalpy, internal energy: the selected-phase
+//     property closure of pr76_selected_phase_property_closure.hpp:336-465 and its chart,
+//     :533-820);
 //   * Corey relative permeability and mobility (two_phase_transport.hpp:38-137);
-//   * component and energy accumulation with backward Euler (two_phase_accumulation.hpp:
-//     18-338, component_accumulation_time.hpp:189-300, energy_accumulation.hpp:511-580);
-//   * a compact record per phase for the face pass.
+//   * a record of the phase for the face pass;
+//   * the equilibrium rows ln(x0/x1) + ln phi0 - ln phi1 and their Jacobian, the other
+//     phase's ln phi and its p, T derivatives taken across the wave by shuffles.
 // k_asm_rows, one wave per (64 cells, column chunk): the cell's columns are split into
-// chunks {p, T, S_0}, then runs of x_0 columns and of x_1 columns, so that one lane keeps
-// only (nc+1) x (chunk width) Jacobian sums in registers. Per lane: the accumulation rows
-// and backward Euler of its columns (and, for the first chunk, the residual), then every
-// face of the cell in slot order: potential upwind and TPFA linearization
-// (two_phase_transport.hpp:173-321, two_phase_tpfa.hpp:129-667) from both cells'
-// records, added to this cell's rows, and the off-diagonal block of that face written
-// to the cell's slot. Each face is evaluated by both its cells, so there are no atomics
+// chunks {p, T, S_0}, then runs of x_0 columns and of x_1 columns. Per lane, one column at
+// a time: the component and energy accumulation with backward Euler
+// (two_phase_accumulation.hpp:18-338, component_accumulation_time.hpp:189-300,
+// energy_accumulation.hpp:511-580), then every face of the cell in slot order: potential
+// upwind and TPFA linearization (two_phase_transport.hpp:173-321,
+// two_phase_tpfa.hpp:129-667) from both cells' records, added to this cell's rows, and
+// the off-diagonal entries of that face written to the cell's slot; the first chunk also
+// forms the residual. Each face is evaluated by both its cells, so there are no atomics
 // and the sums are deterministic. Off-diagonal blocks are stored by (slot, cell), cell
 // fastest, so every store of a wave is one contiguous 512-byte run.
 #pragma once
@@ -53,8 +55,13 @@ struct AsmGrid {
     const int32_t* adj_off;    // [ncell + 1] CSR of cell -> adjacent faces
     const int32_t* adj_face;   // [2 nface]
     const int32_t* slot_face;  // [max_degree][ncell]: face of slot j of cell c (CSR order), -1 past the degree
+    const int32_t* slot_other; // [max_degree][ncell]: the cell across that face, + kSlotNeighbour when c is
+                               // the face's neighbour; -1 past the degree
     int32_t max_degree;
+    int32_t block_stride;      // largest |owner - neighbour| of a face, in 64-cell blocks (>= 1)
 };
+
+constexpr int32_t kSlotNeighbour = 1 << 30;  // cell counts stay below 2^30
 
 struct AsmIo {
     const double* q;        // [Q][ncell]
@@ -83,16 +90,59 @@ struct Rec {
     static constexpr int kP = 2 * kPhase, kT = 2 * kPhase + 1, kSize = 2 * kPhase + 2;
 };
 
-// Column chunks of the face pass: chunk 0 = {p, T, S_0}; then the x_0 columns and the x_1
-// columns in runs of CW.
+// Column chunks of the face pass: the p, T, S_0 columns in runs of C0, then the x_0
+// columns and the x_1 columns in runs of CW. One wave works on one chunk of 64 cells; a
+// workgroup holds up to kWaves chunks of the same 64 cells. (Measured at nc = 6 on
+// 500,000 cells: single-column chunks are 15% slower, three-column x chunks 40%.)
 template <int NC>
 struct Chunks {
     static constexpr int NX = NC - 1;
+    static constexpr int C0 = 3;
     static constexpr int CW = NX <= 6 ? NX : 4;
+    static constexpr int N0 = 3 / C0;
     static constexpr int PER = (NX + CW - 1) / CW;
-    static constexpr int N = 1 + 2 * PER;
-    static constexpr int W = CW > 3 ? CW : 3;
+    static constexpr int N = N0 + 2 * PER;
+    static constexpr int W = CW > C0 ? CW : C0;
+    static constexpr int kWaves = N < 8 ? N : 8;
+    static constexpr int kGroups = (N + kWaves - 1) / kWaves;
+    // chunk k -> first column c0, width cnt, phase cph (-1: both phases)
+    __device__ static void decode(int k, int& c0, int& cnt, int& cph) {
+        if (k < N0) {
+            c0 = k * C0;
+            cnt = C0;
+            cph = -1;
+            return;
+        }
+        k -= N0;
+        cph = k / PER;
+        const int j = k % PER;
+        c0 = 3 + cph * NX + j * CW;
+        cnt = NX - j * CW < CW ? NX - j * CW : CW;
+    }
 };
+// Visiting order of the 64-cell blocks in the face pass: the block indices read as rows of
+// `stride` blocks, walked column by column. Cells a face apart are at most `stride` blocks
+// apart, so the blocks visited close together in time include each other's far
+// neighbours (the cells one layer up and down on a structured grid), whose records are
+// then still in L2 when they are read. A bijection of [0, nblk) for any stride >= 1.
+__device__ __forceinline__ int64_t strided_block(int64_t t, int64_t nblk, int64_t stride) {
+    if (stride <= 1 || stride >= nblk) { return t; }
+    const int64_t full = nblk / stride, rem = nblk % stride;
+    int64_t r, k;
+    if (t < rem * (full + 1)) {
+        r = t / (full + 1);
+        k = t % (full + 1);
+    } else {
+        const int64_t u = t - rem * (full + 1);
+        r = rem + u / full;
+        k = u % full;
+    }
+    return k * stride + r;
+}
+
+// Streaming (non-temporal) store of an output that no kernel of this pass reads back, so
+// that the Jacobian stream does not evict the records the face pass re-reads from L2.
+__device__ __forceinline__ void put(double* p, double v) { __builtin_nontemporal_store(v, p); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so that each XCD
 // walks one contiguous range of cells and the neighbours' records it reads stay in its L2.
@@ -271,7 +321,7 @@ __global__ __launch_bounds__(64) void k_asm_cell(const double* __restrict__ mixc
 }
 
 template <int NC>
-__global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md, AsmGrid G, AsmIo io) {
+__global__ __launch_bounds__(64 * Chunks<NC>::kWaves) void k_asm_rows(AsmModelDev md, AsmGrid G, AsmIo io) {
     constexpr int Q = 2 * NC + 1;
     constexpr int ROWS = NC + 1;
     using R = Rec<NC>;
@@ -279,17 +329,15 @@ __global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md,
     constexpr int W = C::W;
     const int64_t n = G.ncell;
     const double* rec = io.rec;
-    // wave-uniform column chunk: first column c0, width cnt, phase cph (-1: both phases)
-    const int chunk = static_cast<int>(threadIdx.x) / 64;
-    int c0 = 0, cnt = 3, cph = -1;
-    if (chunk > 0) {
-        const int k = chunk - 1;
-        cph = k / C::PER;
-        const int j = k % C::PER;
-        c0 = 3 + cph * (NC - 1) + j * C::CW;
-        cnt = min(C::CW, NC - 1 - j * C::CW);
-    }
-    const int64_t blk = xcd_contiguous_block(blockIdx.x, gridDim.x);
+    // wave-uniform column chunk: first column c0, width cnt, phase cph (-1: both phases);
+    // consecutive workgroups (after the XCD renumbering) are the chunk groups of one block
+    // of 64 cells
+    const int64_t w = xcd_contiguous_block(blockIdx.x, gridDim.x);
+    const int chunk = static_cast<int>(w % C::kGroups) * C::kWaves + static_cast<int>(threadIdx.x) / 64;
+    if (chunk >= C::N) { return; }
+    int c0 = 0, cnt = 0, cph = -1;
+    C::decode(chunk, c0, cnt, cph);
+    const int64_t blk = strided_block(w / C::kGroups, (n + 63) / 64, G.block_stride);
     const int64_t c = blk * 64 + static_cast<int64_t>(threadIdx.x % 64);
     if (c >= n || io.status[c] != 0) { return; }
     // sub-chart index of column col in phase ph (-1: the phase does not depend on it)
@@ -311,6 +359,7 @@ __global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md,
     const double sat[2] = {s0, 1.0 - s0};
     const double inv_dt = 1.0 / io.dt;
     const int64_t ostride = static_cast<int64_t>(G.max_degree) * n;
+    const double inv_v = 1.0 / G.bulk[c];
     double res[ROWS];
     if (chunk == 0) {
         // accumulation residual and backward Euler (two_phase_accumulation.hpp:18-338,
@@ -362,37 +411,48 @@ __global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md,
         }
 #pragma unroll 1
         for (int j = 0; j < G.max_degree; ++j) {
+            const int32_t v = G.slot_other[static_cast<int64_t>(j) * n + c];
             const int32_t f = G.slot_face[static_cast<int64_t>(j) * n + c];
             double* od = io.offdiag + static_cast<int64_t>(j) * n + c + static_cast<int64_t>(col) * ostride;
-            if (f < 0) {
+            if (v < 0) {
                 // padding slot past the cell's degree
-                for (int r = 0; r < ROWS; ++r) { od[static_cast<int64_t>(r * Q) * ostride] = 0.0; }
+                for (int r = 0; r < ROWS; ++r) { put(od + static_cast<int64_t>(r * Q) * ostride, 0.0); }
                 continue;
             }
-            const int64_t o = G.owner[f], nb = G.neighbour[f];
-            const bool side_owner = o == c;
-            if (io.status[side_owner ? nb : o] != 0) { continue; }
+            const bool side_owner = (v & kSlotNeighbour) == 0;
+            const int64_t other = v & (kSlotNeighbour - 1);
+            const int64_t o = side_owner ? c : other, nb = side_owner ? other : c;
+            if (io.status[other] != 0) { continue; }
             const double tf = G.trans[f], gd = G.gdot[f], gth = G.gth[f];
-            const double vo = G.bulk[o], vn = G.bulk[nb];
+            // rows are scaled by this cell's bulk volume: +1/V_o as owner, -1/V_n as neighbour
+            const double sv = side_owner ? inv_v : -inv_v;
             double jo[ROWS], jn[ROWS], rate[ROWS];
             for (int r = 0; r < ROWS; ++r) { jo[r] = 0; jn[r] = 0; rate[r] = 0; }
             for (int ph = ph_lo; ph <= ph_hi; ++ph) {
                 const int b = ph * R::kPhase;
                 // potential, upwind and flux (two_phase_transport.hpp:173-321,
                 // two_phase_tpfa.hpp:231-300)
+                // both cells' fields are loaded, and the upwind ones selected afterwards, so
+                // that no load waits for the upwind decision
                 const double rho_o = rec[rix<NC>(b + R::kRho, o)], rho_n = rec[rix<NC>(b + R::kRho, nb)];
-                const double rf = 0.5 * (rho_o + rho_n);
-                const double delta = (rec[rix<NC>(R::kP, nb)] - rec[rix<NC>(R::kP, o)]) - rf * gd;
-                const bool own = !(delta > 0.0);
-                const int64_t up = own ? o : nb;
-                const double lamu = rec[rix<NC>(b + R::kLam, up)];
-                const double flux = -tf * lamu * delta;
-                const double cu = rec[rix<NC>(b + R::kC, up)];
-                const double rhou = own ? rho_o : rho_n;
-                const double hu = rec[rix<NC>(b + R::kH, up)];
-                // derivatives wrt column col of the owner's and the neighbour's q
+                const double p_o = rec[rix<NC>(R::kP, o)], p_n = rec[rix<NC>(R::kP, nb)];
+                const double lam_o = rec[rix<NC>(b + R::kLam, o)], lam_n = rec[rix<NC>(b + R::kLam, nb)];
+                const double c_o = rec[rix<NC>(b + R::kC, o)], c_n = rec[rix<NC>(b + R::kC, nb)];
+                const double h_o = rec[rix<NC>(b + R::kH, o)], h_n = rec[rix<NC>(b + R::kH, nb)];
                 const int s = sub(ph, col);
                 auto sd = [&](int field, int64_t cell) { return s < 0 ? 0.0 : rec[rix<NC>(b + field + s, cell)]; };
+                const double dc_o = sd(R::kDc, o), dc_n = sd(R::kDc, nb);
+                const double dh_o = sd(R::kDh, o), dh_n = sd(R::kDh, nb);
+                const double rf = 0.5 * (rho_o + rho_n);
+                const double delta = (p_n - p_o) - rf * gd;
+                const bool own = !(delta > 0.0);
+                const int64_t up = own ? o : nb;
+                const double lamu = own ? lam_o : lam_n;
+                const double flux = -tf * lamu * delta;
+                const double cu = own ? c_o : c_n;
+                const double rhou = own ? rho_o : rho_n;
+                const double hu = own ? h_o : h_n;
+                // derivatives wrt column col of the owner's and the neighbour's q
                 const double dlo = (col == 2 ? rec[rix<NC>(b + R::kDlamS, o)] : 0.0) + sd(R::kDlam, o);
                 const double dln = (col == 2 ? rec[rix<NC>(b + R::kDlamS, nb)] : 0.0) + sd(R::kDlam, nb);
                 const double dro = sd(R::kDrho, o), drn = sd(R::kDrho, nb);
@@ -400,7 +460,7 @@ __global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md,
                 const double dphin = (col == 0 ? 1.0 : 0.0) - 0.5 * drn * gd;
                 const double fo = -tf * (lamu * dphio + delta * (own ? dlo : 0.0));
                 const double fn = -tf * (lamu * dphin + delta * (own ? 0.0 : dln));
-                const double dcu = sd(R::kDc, up);
+                const double dcu = own ? dc_o : dc_n;
                 const int first = 3 + ph * (NC - 1);
                 for (int i = 0; i < NC; ++i) {
                     const double xi = rec[rix<NC>(b + R::kX + i, up)];
@@ -413,16 +473,16 @@ __global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md,
                     if (with_residual) { rate[i] += content * flux; }
                 }
                 const double ed = rhou * hu;
-                const double ded = (own ? dro : drn) * hu + rhou * sd(R::kDh, up);
+                const double ded = (own ? dro : drn) * hu + rhou * (own ? dh_o : dh_n);
                 jo[NC] += (own ? ded : 0.0) * flux + ed * fo;
                 jn[NC] += (own ? 0.0 : ded) * flux + ed * fn;
                 if (with_residual) { rate[NC] += ed * flux; }
             }
             if (with_residual) {
                 // this cell's residual rows: owner +rate / V_o, neighbour -rate / V_n
-                for (int i = 0; i < NC; ++i) { res[i] += side_owner ? rate[i] / vo : -rate[i] / vn; }
+                for (int i = 0; i < NC; ++i) { res[i] += rate[i] * sv; }
                 const double etot = rate[NC] + gth * (rec[rix<NC>(R::kT, o)] - rec[rix<NC>(R::kT, nb)]);
-                res[NC] += side_owner ? etot / vo : -etot / vn;
+                res[NC] += etot * sv;
             }
             if (col == 1) {
                 jo[NC] += gth;
@@ -430,22 +490,11 @@ __global__ __launch_bounds__(64 * Chunks<NC>::N) void k_asm_rows(AsmModelDev md,
             }
             // this cell's diagonal gets d/d(own q), its slot d/d(other q)
             for (int r = 0; r < ROWS; ++r) {
-                double dself, doff;
-                if (side_owner) {
-                    dself = jo[r] / vo;
-                    doff = jn[r] / vo;
-                } else if (r < NC) {
-                    dself = -jn[r] / vn;
-                    doff = -jo[r] / vn;
-                } else {
-                    dself = jn[r] * (-1.0 / vn);
-                    doff = jo[r] * (-1.0 / vn);
-                }
-                acc[r] += dself;
-                od[static_cast<int64_t>(r * Q) * ostride] = doff;
+                acc[r] += (side_owner ? jo[r] : jn[r]) * sv;
+                put(od + static_cast<int64_t>(r * Q) * ostride, (side_owner ? jn[r] : jo[r]) * sv);
             }
         }
-        for (int r = 0; r < ROWS; ++r) { io.diag[(r * Q + col) * n + c] = acc[r]; }
+        for (int r = 0; r < ROWS; ++r) { put(io.diag + (r * Q + col) * n + c, acc[r]); }
     }
     if (chunk == 0) {
         for (int r = 0; r < ROWS; ++r) { io.residual[r * n + c] = res[r]; }
@@ -474,8 +523,8 @@ hipError_t launch_assembly(const double* mixc, const AsmModelDev& md, const AsmG
         return hipGetLastError();
     }
     hipLaunchKernelGGL((k_asm_cell<NC, false>), dim3(blocks), dim3(64), 0, s, mixc, md, G, io);
-    const unsigned row_blocks = static_cast<unsigned>((G.ncell + 63) / 64);
-    hipLaunchKernelGGL(k_asm_rows<NC>, dim3(row_blocks), dim3(64 * Chunks<NC>::N), 0, s, md, G, io);
+    const unsigned row_blocks = static_cast<unsigned>((G.ncell + 63) / 64 * Chunks<NC>::kGroups);
+    hipLaunchKernelGGL(k_asm_rows<NC>, dim3(row_blocks), dim3(64 * Chunks<NC>::kWaves), 0, s, md, G, io);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -380,7 +381,9 @@ struct mpmc_tpfa_grid {
     int32_t* adj_off = nullptr;  // device CSR cell -> faces
     int32_t* adj_face = nullptr;
     int32_t* slot_face = nullptr; // device [max_degree][ncell], -1 past a cell's degree
+    int32_t* slot_other = nullptr; // device [max_degree][ncell], cell across (+ 2^30 as neighbour), -1 past
     int32_t max_degree = 0;
+    int32_t block_stride = 1;     // largest face span |owner - neighbour| in 64-cell blocks
     void* rec = nullptr;         // per-cell face records
     size_t rec_bytes = 0;
     unsigned long long* flag = nullptr;
@@ -424,16 +427,26 @@ int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk, cons
     // the same adjacency by (slot, cell), cell fastest: the layout of the off-diagonal blocks
     int32_t maxdeg = 0;
     for (int64_t c = 0; c < ncell; ++c) { maxdeg = std::max(maxdeg, off[c + 1] - off[c]); }
+    int64_t span = 0;
+    for (int64_t f = 0; f < nface; ++f) { span = std::max<int64_t>(span, std::abs(int64_t{ho[f]} - int64_t{hn[f]})); }
     std::vector<int32_t> slots(static_cast<size_t>(maxdeg) * static_cast<size_t>(ncell), -1);
+    std::vector<int32_t> others(slots.size(), -1);
     for (int64_t c = 0; c < ncell; ++c) {
-        for (int32_t a = off[c]; a < off[c + 1]; ++a) { slots[static_cast<size_t>(a - off[c]) * ncell + c] = adj[a]; }
+        for (int32_t a = off[c]; a < off[c + 1]; ++a) {
+            const size_t k = static_cast<size_t>(a - off[c]) * ncell + c;
+            const int32_t f = adj[a];
+            slots[k] = f;
+            others[k] = ho[f] == c ? hn[f] : ho[f] + mpmc::dev::kSlotNeighbour;
+        }
     }
     auto* g = new (std::nothrow) mpmc_tpfa_grid{ncell, nface, bulk, poro, trans, gdot, gth, owner, neighbour};
     if (!g) { return fail(MPMC_E_NO_MEMORY, "tpfa_grid_create: host allocation"); }
     g->max_degree = maxdeg;
+    g->block_stride = static_cast<int32_t>(std::max<int64_t>(1, span / 64));
     e = hipMalloc(&g->adj_off, sizeof(int32_t) * off.size());
     if (e == hipSuccess) { e = hipMalloc(&g->adj_face, sizeof(int32_t) * std::max<size_t>(1, adj.size())); }
     if (e == hipSuccess) { e = hipMalloc(&g->slot_face, sizeof(int32_t) * std::max<size_t>(1, slots.size())); }
+    if (e == hipSuccess) { e = hipMalloc(&g->slot_other, sizeof(int32_t) * std::max<size_t>(1, others.size())); }
     if (e == hipSuccess) { e = hipMalloc(&g->flag, 256); }
     if (e == hipSuccess) { e = hipMemcpy(g->adj_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice); }
     if (e == hipSuccess && !adj.empty()) {
@@ -441,6 +454,9 @@ int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk, cons
     }
     if (e == hipSuccess && !slots.empty()) {
         e = hipMemcpy(g->slot_face, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !others.empty()) {
+        e = hipMemcpy(g->slot_other, others.data(), sizeof(int32_t) * others.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         mpmc_tpfa_grid_destroy(g);
@@ -453,7 +469,8 @@ int mpmc_tpfa_grid_create(int64_t ncell, int64_t nface, const double* bulk, cons
 int mpmc_tpfa_grid_destroy(mpmc_tpfa_grid_t g) {
     if (!g) { return MPMC_OK; }
     for (void* ptr : {static_cast<void*>(g->adj_off), static_cast<void*>(g->adj_face),
-                      static_cast<void*>(g->slot_face), g->rec, static_cast<void*>(g->flag)}) {
+                      static_cast<void*>(g->slot_face), static_cast<void*>(g->slot_other), g->rec,
+                      static_cast<void*>(g->flag)}) {
         if (ptr) { (void)hipFree(ptr); }
     }
     delete g;
@@ -539,7 +556,7 @@ static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assemb
     for (int i = 0; i < MPMC_MAX_COMPONENTS; ++i) { dm.molar_mass[i] = md->molar_mass[i]; }
     const mpmc::dev::AsmGrid G{g->ncell, g->nface,    g->bulk,     g->poro,      g->owner,     g->neighbour,
                                g->trans, g->gdot,     g->gth,      g->adj_off,   g->adj_face,  g->slot_face,
-                               g->max_degree};
+                               g->slot_other, g->max_degree, g->block_stride};
     const mpmc::dev::AsmIo io{q, branch, prev, dt, out->cell_status, out->residual, out->diag, out->offdiag,
                               static_cast<double*>(g->rec), g->flag};
     e = hipErrorInvalidValue;
