@@ -130,6 +130,97 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     return out
 
 
+def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
+    """The uneven multi-mixture batch: 256 mixtures, nc ~ U{2..12}, cells per mixture
+    log-uniform on [1e3, 2e5] (about 9.7M cells), 30% of each mixture's cells within +-2%
+    in P of a phase boundary found by scan and bisection on P with the GPU flash. Timed as
+    one flash plan (ragged launches grouped by nc, captured in a HIP graph) against the same
+    segments flashed one mixture at a time; both must give identical records."""
+    import torch
+
+    from mpmc_hnu_amd import flash, workloads as W
+
+    t0 = time.perf_counter()
+    layout = W.uneven_layout(scale=scale)
+    mixes = [flash.Mixture.from_dict(m["mixture"]) for m in layout]
+    base = W.uneven_base_points(layout)
+
+    def phase_count(reqs):
+        res = [None] * len(reqs)
+        by_nc = {}
+        for i, (k, _, _, _) in enumerate(reqs):
+            by_nc.setdefault(layout[k]["nc"], []).append(i)
+        for nc, idx in by_nc.items():
+            cap = flash.multi_max_mixtures(nc)
+            for s0 in range(0, len(idx), cap):
+                chunk = idx[s0:s0 + cap]
+                sizes = [reqs[i][1].size for i in chunk]
+                off = np.concatenate([[0], np.cumsum(sizes)])
+                out = flash.flash_multi([mixes[reqs[i][0]] for i in chunk], off,
+                                        np.concatenate([reqs[i][1] for i in chunk]),
+                                        np.concatenate([reqs[i][2] for i in chunk]),
+                                        np.concatenate([reqs[i][3] for i in chunk], axis=1), counters=False)
+                pc = out["phase_count"].cpu().numpy()
+                for j, i in enumerate(chunk):
+                    res[i] = pc[off[j]:off[j + 1]]
+        return res
+
+    pb = W.boundary_pressures(phase_count, base)
+    cells = W.uneven_cells(layout, base, pb)
+    dev = torch.device("cuda", 0)
+    segments = [(mixes[k], torch.as_tensor(p, device=dev), torch.as_tensor(t, device=dev),
+                 torch.as_tensor(z, device=dev)) for k, (p, t, z) in enumerate(cells)]
+    ncell = int(sum(m["ncell"] for m in layout))
+    near = int(sum(round(0.3 * m["ncell"]) if np.isfinite(b).any() else 0 for m, b in zip(layout, pb)))
+    print(f"[bench] uneven: {len(layout)} mixtures, {ncell} cells ({near} near a boundary), "
+          f"built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    plan = flash.Plan(segments, counters=False)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        plan.launch(stream)
+    torch.cuda.synchronize()
+    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    e1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for k in range(steps):
+        e0[k].record(stream)
+        plan.launch(stream)
+        e1[k].record(stream)
+    torch.cuda.synchronize()
+    plan_ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
+    # the same segments one mixture at a time, sharing one workspace
+    nmax = max(m["ncell"] for m in layout)
+    ws = torch.empty(max(flash.workspace_bytes(nc, nmax) for nc in range(2, 13)), dtype=torch.uint8, device=dev)
+    alone = [flash.allocate_outputs(mix.nc, p.numel(), dev, counters=False) for mix, p, _, _ in segments]
+    a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a0.record(stream)
+    for (mix, p, t, z), out in zip(segments, alone):
+        flash.flash_batch(mix, p, t, z, out=out, counters=False, workspace=ws, stream=stream)
+    a1.record(stream)
+    torch.cuda.synchronize()
+    alone_ms = a0.elapsed_time(a1)
+    identical = all(torch.equal(o1[k], o2[k]) if k not in ("beta", "zfactor", "x", "ln_phi") else
+                    torch.equal(torch.nan_to_num(o1[k], nan=7.0), torch.nan_to_num(o2[k], nan=7.0))
+                    for o1, o2 in zip(plan.outputs, alone)
+                    for k in ("status", "phase_count", "beta", "zfactor", "branch", "x", "ln_phi"))
+    counts = {}
+    for o in plan.outputs:
+        v, c = np.unique(o["phase_count"].cpu().numpy(), return_counts=True)
+        for a, b in zip(v, c):
+            counts[int(a)] = counts.get(int(a), 0) + int(b)
+    groups = plan.groups
+    plan.close()
+    for m in mixes:
+        m.close()
+    print(f"[bench] uneven: plan {plan_ms:.1f} ms ({ncell / (plan_ms * 1e-3) / 1e6:.2f} M cells/s, {groups} launch "
+          f"groups); one mixture at a time {alone_ms:.1f} ms; identical records: {identical}",
+          file=sys.stderr, flush=True)
+    return {"mixtures": len(layout), "cells": ncell, "near_boundary_cells": near, "launch_groups": groups,
+            "plan_ms": plan_ms, "cells_per_s": ncell / (plan_ms * 1e-3), "one_mixture_at_a_time_ms": alone_ms,
+            "speedup_vs_one_at_a_time": alone_ms / plan_ms, "identical_records": bool(identical),
+            "phase_counts": counts, "precision": "fp64",
+            "what": "max-3-phase flash of every cell; ragged launches grouped by nc, one HIP graph"}
+
+
 def copy_leg(steps: int) -> dict:
     """Plain streaming kernel over 1 GiB (2^27 doubles, dst = -src: one elementwise read
     and one write per element): the calibration kernel for HBM byte counters
@@ -201,9 +292,12 @@ def main(argv=None):
                     help="grid of the assembly leg (nx,ny,nz; empty: skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals for the serving leg (0: skip)")
-    ap.add_argument("--legs", type=str, default="flash,assembly,serving",
-                    help="comma list of legs to run (flash, assembly, serving; copy: a 1 GiB device copy that "
-                         "calibrates HBM byte counters); the JSON line is complete only with the first three")
+    ap.add_argument("--uneven-scale", type=float, default=1.0,
+                    help="cells per mixture of the uneven leg, as a multiple of the 9.7M-cell batch")
+    ap.add_argument("--legs", type=str, default="flash,assembly,uneven,serving",
+                    help="comma list of legs to run (flash, assembly, uneven, serving; copy: a 1 GiB streaming "
+                         "kernel that calibrates HBM byte counters); the JSON line is complete only with the "
+                         "first four")
     a = ap.parse_args(argv)
     legs = set(a.legs.split(","))
     if a.gpus != 1:
@@ -223,6 +317,8 @@ def main(argv=None):
             part["copy"] = copy_leg(a.steps)
         if "assembly" in legs and a.assembly_dims:
             part["assembly"] = assembly_leg(tuple(int(v) for v in a.assembly_dims.split(",")), a.steps, a.warmup)
+        if "uneven" in legs:
+            part["uneven"] = uneven_leg(a.steps, a.warmup, a.uneven_scale)
         if "serving" in legs and a.serve_seconds > 0:
             part["serving"] = serve_leg(a.serve_seconds)
         print(json.dumps(part), flush=True)
@@ -302,6 +398,9 @@ def main(argv=None):
     asm = None
     if a.assembly_dims and "assembly" in legs:
         asm = assembly_leg(tuple(int(v) for v in a.assembly_dims.split(",")), a.steps, a.warmup)
+    uneven = None
+    if "uneven" in legs:
+        uneven = uneven_leg(a.steps, a.warmup, a.uneven_scale)
     serving = None
     if a.serve_seconds > 0 and "serving" in legs:
         serving = serve_leg(a.serve_seconds)
@@ -356,6 +455,7 @@ def main(argv=None):
             "algorithmic_bytes_per_launch": algorithmic_bytes(nc, ncell),
         },
         "assembly": asm,
+        "uneven": uneven,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -136,6 +136,58 @@ int mpmc_flash_batch_ws(mpmc_mixture_t mixture, int64_t ncell, const double* p, 
                         void* stream);
 
 /*
+ * Ragged multi-mixture batch: the cells of several mixtures with the same component count
+ * in one staged launch sequence. Mixture k owns cells [cell_offset[k], cell_offset[k+1])
+ * of the batch (cell_offset: host array of nmix + 1 entries, cell_offset[0] = 0,
+ * cell_offset[nmix] = ncell); p, t, z and the outputs cover all ncell cells in the
+ * layouts of mpmc_flash_batch. The mixture table is staged in LDS and every cell reads its
+ * own mixture's constants, so one persistent pipeline serves the whole uneven batch.
+ * Results are bit-identical to mpmc_flash_batch on each mixture's cells. The workspace is
+ * caller-owned (mpmc_flash_multi_workspace_bytes), so the call allocates nothing and can
+ * be captured in a graph. Staged schedule only (options->schedule = 0).
+ * The reference solves one request at a time (PtFlashBackend::solve,
+ * modules/flash/include/mpmc/flash/pt_flash_backend.hpp:187-195); this is the batch form of
+ * that call for many mixtures at once. */
+int mpmc_flash_multi_max_mixtures(int nc);
+size_t mpmc_flash_multi_workspace_bytes(int nc, int nmix, int64_t ncell);
+int mpmc_flash_multi(int nmix, const mpmc_mixture_t* mixtures, const int64_t* cell_offset, const double* p,
+                     const double* t, const double* z, const mpmc_flash_options* options,
+                     const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Flash plan: a fixed uneven batch (segments of any mixtures and component counts, each
+ * with its own device inputs and outputs in the layouts of mpmc_flash_batch) prepared
+ * once and replayed. Creation groups the segments by component count into ragged
+ * launches (mpmc_flash_multi: one persistent pipeline per group of up to
+ * mpmc_flash_multi_max_mixtures(nc) mixtures), allocates every buffer, uploads the
+ * mixture tables, runs the batch once and captures the whole launch sequence (gather,
+ * pipeline, scatter per group) into one HIP graph. mpmc_flash_plan_launch replays the
+ * graph on the caller's stream: the segments' current input contents are flashed and their
+ * outputs written, with no allocation and one launch call. Results are those of
+ * mpmc_flash_batch on each segment. A plan is bound to the device pointers it was made
+ * with; replays of one plan must not overlap. No counterpart in the reference: it is the
+ * repeated batch caller of PtFlashBackend::solve (pt_flash_backend.hpp:187-195), e.g. a
+ * scan of every cell after each Newton solve. */
+typedef struct mpmc_flash_segment {
+    mpmc_mixture_t mixture;
+    int64_t ncell;
+    const double* p;
+    const double* t;
+    const double* z;          /* [nc][ncell] */
+    mpmc_flash_outputs out;   /* counters: given for every segment or for none */
+} mpmc_flash_segment;
+
+typedef struct mpmc_flash_plan* mpmc_flash_plan_t;
+
+int mpmc_flash_plan_create(int nseg, const mpmc_flash_segment* segments, const mpmc_flash_options* options,
+                           mpmc_flash_plan_t* out);
+int mpmc_flash_plan_launch(mpmc_flash_plan_t plan, void* stream);
+/* Launch groups and device bytes held by the plan. */
+int mpmc_flash_plan_info(mpmc_flash_plan_t plan, int32_t* groups, size_t* device_bytes);
+/* Waits for the device, then frees the plan. */
+int mpmc_flash_plan_destroy(mpmc_flash_plan_t plan);
+
+/*
  * Candidate-phase PR76 evaluation, one composition per cell: all roots, then ln(phi) of
  * every root (admissibility is in root_flags). Replaces Pr76Phase::roots_full + evaluate_full
  * (modules/thermodynamics/include/mpmc/thermodynamics/pr76_phase.hpp:125-148).

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <algorithm>
@@ -159,12 +160,8 @@ size_t mpmc_flash_workspace_bytes(int nc, int64_t ncell) {
     return workspace_bytes_for(nc, ncell);
 }
 
-static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
-                            const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
-                            size_t workspace_bytes, bool own_workspace, void* stream) {
-    if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: null mixture or outputs"); }
-    if (ncell < 0 || ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
-    mpmc_flash_options o;
+// Options of a flash call with defaults filled in; MPMC_OK or an error code.
+static int flash_options(const mpmc_flash_options* options, mpmc_flash_options& o) {
     mpmc_flash_default_options(&o);
     if (options) { o = *options; }
     if (o.max_phases != 2 && o.max_phases != 3) {
@@ -178,7 +175,10 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
     if (o.schedule == 1 && o.precision != 0) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: the serial schedule runs FP64 only");
     }
-    if (ncell == 0) { return MPMC_OK; }
+    return MPMC_OK;
+}
+
+static int flash_buffers(const double* p, const double* t, const double* z, const mpmc_flash_outputs* out) {
     const void* ptrs[] = {p, t, z, out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
                           out->ln_phi};
     for (const void* ptr : ptrs) {
@@ -187,6 +187,18 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
     if (out->counters && !is_device_ptr(out->counters)) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: counters must be device memory");
     }
+    return MPMC_OK;
+}
+
+static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                            const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
+                            size_t workspace_bytes, bool own_workspace, void* stream) {
+    if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: null mixture or outputs"); }
+    if (ncell < 0 || ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
+    mpmc_flash_options o;
+    if (const int rc = flash_options(options, o); rc != MPMC_OK) { return rc; }
+    if (ncell == 0) { return MPMC_OK; }
+    if (const int rc = flash_buffers(p, t, z, out); rc != MPMC_OK) { return rc; }
     const mpmc::dev::FlashOut fo{out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
                                  out->ln_phi, out->counters};
     const hipStream_t s = static_cast<hipStream_t>(stream);
@@ -233,8 +245,8 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
     switch (m->nc) {
 #define MPMC_CASE(N)                                                                                             \
     case N:                                                                                                      \
-        e = mpmc::dev::launch_pipeline<N>(m->dev_constants, workspace, workspace_bytes, ncell, p, t, z,          \
-                                          o.max_phases, o.fugacity_tolerance, o.precision, fo, s);               \
+        e = mpmc::dev::launch_pipeline<N>(m->dev_constants, 1, nullptr, workspace, workspace_bytes, ncell, p, t,  \
+                                          z, o.max_phases, o.fugacity_tolerance, o.precision, fo, s);            \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE
@@ -252,6 +264,381 @@ int mpmc_flash_batch_ws(mpmc_mixture_t m, int64_t ncell, const double* p, const 
                         const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                         size_t workspace_bytes, void* stream) {
     return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, false, stream);
+}
+
+// ---------------------------------------------------------------- ragged multi-mixture batch
+namespace {
+constexpr size_t kMixTableLds = 48 * 1024; // largest mixture table one launch stages in LDS
+size_t mix_doubles(int nc) { return static_cast<size_t>(4 * nc + nc * nc + 1); }
+size_t align256(size_t v) { return (v + 255) & ~size_t{255}; }
+} // namespace
+
+static __global__ void k_fill_mix(int32_t* mix_of, int64_t n, int32_t value) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        mix_of[i] = value;
+    }
+}
+
+int mpmc_flash_multi_max_mixtures(int nc) {
+    if (nc < 1 || nc > MPMC_MAX_COMPONENTS) { return 0; }
+    return static_cast<int>(kMixTableLds / (mix_doubles(nc) * sizeof(double)));
+}
+
+size_t mpmc_flash_multi_workspace_bytes(int nc, int nmix, int64_t ncell) {
+    if (nc < 1 || nc > MPMC_MAX_COMPONENTS || nmix < 1 || ncell < 0) { return 0; }
+    return align256(workspace_bytes_for(nc, ncell)) + align256(static_cast<size_t>(nmix) * mix_doubles(nc) * 8) +
+           align256(static_cast<size_t>(ncell) * sizeof(int32_t));
+}
+
+int mpmc_flash_multi(int nmix, const mpmc_mixture_t* mixtures, const int64_t* cell_offset, const double* p,
+                     const double* t, const double* z, const mpmc_flash_options* options,
+                     const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes, void* stream) {
+    if (nmix < 1 || !mixtures || !cell_offset || !out) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: null argument or no mixture");
+    }
+    const int nc = mixtures[0] ? mixtures[0]->nc : 0;
+    for (int k = 0; k < nmix; ++k) {
+        if (!mixtures[k]) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: null mixture"); }
+        if (mixtures[k]->nc != nc) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: all mixtures of one call must have the same nc");
+        }
+    }
+    if (nmix > mpmc_flash_multi_max_mixtures(nc)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: more mixtures than mpmc_flash_multi_max_mixtures(nc)");
+    }
+    if (cell_offset[0] != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: cell_offset[0] must be 0"); }
+    for (int k = 0; k < nmix; ++k) {
+        if (cell_offset[k + 1] < cell_offset[k]) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: cell offsets must not decrease");
+        }
+    }
+    const int64_t ncell = cell_offset[nmix];
+    if (ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: bad ncell"); }
+    mpmc_flash_options o;
+    if (const int rc = flash_options(options, o); rc != MPMC_OK) { return rc; }
+    if (o.schedule != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: runs the staged schedule only"); }
+    if (ncell == 0) { return MPMC_OK; }
+    if (const int rc = flash_buffers(p, t, z, out); rc != MPMC_OK) { return rc; }
+    const size_t need = mpmc_flash_multi_workspace_bytes(nc, nmix, ncell);
+    if (!workspace || workspace_bytes < need || !is_device_ptr(workspace)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: workspace must be device memory of "
+                                             "mpmc_flash_multi_workspace_bytes(nc, nmix, ncell) bytes");
+    }
+    char* base = static_cast<char*>(workspace);
+    const size_t pipe_bytes = align256(workspace_bytes_for(nc, ncell));
+    double* table = reinterpret_cast<double*>(base + pipe_bytes);
+    int32_t* mix_of = reinterpret_cast<int32_t*>(base + pipe_bytes +
+                                                 align256(static_cast<size_t>(nmix) * mix_doubles(nc) * 8));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nmix && e == hipSuccess; ++k) {
+        e = hipMemcpyAsync(table + k * mix_doubles(nc), mixtures[k]->dev_constants, mix_doubles(nc) * sizeof(double),
+                           hipMemcpyDeviceToDevice, s);
+        const int64_t len = cell_offset[k + 1] - cell_offset[k];
+        if (e == hipSuccess && len > 0) {
+            const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((len + 255) / 256, 4096));
+            hipLaunchKernelGGL(k_fill_mix, dim3(blocks), dim3(256), 0, s, mix_of + cell_offset[k], len, k);
+            e = hipGetLastError();
+        }
+    }
+    if (e != hipSuccess) { return hip_fail(e, "flash_multi setup"); }
+    const mpmc::dev::FlashOut fo{out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
+                                 out->ln_phi, out->counters};
+    e = hipErrorInvalidValue;
+    switch (nc) {
+#define MPMC_CASE(N)                                                                                             \
+    case N:                                                                                                      \
+        e = mpmc::dev::launch_pipeline<N>(table, nmix, nmix > 1 ? mix_of : nullptr, workspace, pipe_bytes, ncell, \
+                                          p, t, z, o.max_phases, o.fugacity_tolerance, o.precision, fo, s);     \
+        break;
+        MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "flash_multi: unsupported nc");
+    }
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_multi launch");
+}
+
+// ---------------------------------------------------------------- flash plans
+} // extern "C"
+
+namespace {
+
+// One strided copy: rows x n elements of `bytes` each, from src (row pitch src_pitch
+// elements) to dst (row pitch dst_pitch elements).
+struct SegCopy {
+    const void* src;
+    void* dst;
+    int64_t n, src_pitch, dst_pitch;
+    int32_t rows, bytes;
+};
+
+__global__ void k_seg_copy(const SegCopy* __restrict__ list) {
+    const SegCopy d = list[blockIdx.y];
+    const int64_t total = d.n * d.rows;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t r = e / d.n, i = e % d.n;
+        if (d.bytes == 8) {
+            static_cast<double*>(d.dst)[r * d.dst_pitch + i] = static_cast<const double*>(d.src)[r * d.src_pitch + i];
+        } else {
+            static_cast<int32_t*>(d.dst)[r * d.dst_pitch + i] = static_cast<const int32_t*>(d.src)[r * d.src_pitch + i];
+        }
+    }
+}
+
+// A launch group: segments of one nc whose mixtures fit one LDS table, packed into one
+// ragged batch of n cells.
+struct PlanGroup {
+    int nc = 0;
+    int64_t n = 0;
+    std::vector<mpmc_mixture_t> mixtures;  // table order
+    std::vector<int64_t> offsets;          // per table entry: first cell (cells are grouped by mixture)
+    double *p = nullptr, *t = nullptr, *z = nullptr;
+    mpmc_flash_outputs out{};
+    double* table = nullptr;
+    int32_t* mix_of = nullptr;
+    SegCopy* gather = nullptr;             // device descriptor lists
+    SegCopy* scatter = nullptr;
+    int ngather = 0, nscatter = 0;
+};
+
+} // namespace
+
+struct mpmc_flash_plan {
+    mpmc_flash_options options{};
+    std::vector<PlanGroup> groups;
+    std::vector<void*> allocations;
+    void* workspace = nullptr;
+    size_t workspace_bytes = 0;
+    size_t device_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+hipError_t plan_alloc(mpmc_flash_plan* pl, void** ptr, size_t bytes) {
+    *ptr = nullptr;
+    const hipError_t e = hipMalloc(ptr, bytes > 0 ? bytes : 1);
+    if (e == hipSuccess) {
+        pl->allocations.push_back(*ptr);
+        pl->device_bytes += bytes;
+    }
+    return e;
+}
+
+// The launch sequence of every group on stream s: gather the segments' inputs, run the
+// ragged pipeline, scatter the outputs back to the segments.
+hipError_t plan_record(mpmc_flash_plan* pl, hipStream_t s) {
+    for (PlanGroup& g : pl->groups) {
+        hipLaunchKernelGGL(k_seg_copy, dim3(256, static_cast<unsigned>(g.ngather)), dim3(256), 0, s, g.gather);
+        const mpmc::dev::FlashOut fo{g.out.status, g.out.phase_count, g.out.beta, g.out.zfactor, g.out.branch,
+                                     g.out.x, g.out.ln_phi, g.out.counters};
+        const int nmix = static_cast<int>(g.mixtures.size());
+        hipError_t e = hipErrorInvalidValue;
+        switch (g.nc) {
+#define MPMC_CASE(N)                                                                                              \
+    case N:                                                                                                       \
+        e = mpmc::dev::launch_pipeline<N>(g.table, nmix, nmix > 1 ? g.mix_of : nullptr, pl->workspace,            \
+                                          pl->workspace_bytes, g.n, g.p, g.t, g.z, pl->options.max_phases,        \
+                                          pl->options.fugacity_tolerance, pl->options.precision, fo, s);          \
+        break;
+            MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+        default: break;
+        }
+        if (e != hipSuccess) { return e; }
+        hipLaunchKernelGGL(k_seg_copy, dim3(256, static_cast<unsigned>(g.nscatter)), dim3(256), 0, s, g.scatter);
+    }
+    return hipGetLastError();
+}
+
+} // namespace
+
+extern "C" {
+
+int mpmc_flash_plan_create(int nseg, const mpmc_flash_segment* segs, const mpmc_flash_options* options,
+                           mpmc_flash_plan_t* out) {
+    if (!out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_create: null output handle"); }
+    *out = nullptr;
+    if (nseg < 1 || !segs) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_create: no segments"); }
+    mpmc_flash_options o;
+    if (const int rc = flash_options(options, o); rc != MPMC_OK) { return rc; }
+    if (o.schedule != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_create: staged schedule only"); }
+    const bool counters = segs[0].out.counters != nullptr;
+    for (int k = 0; k < nseg; ++k) {
+        const mpmc_flash_segment& g = segs[k];
+        if (!g.mixture || g.ncell < 0 || g.ncell > (int64_t{1} << 30)) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_create: bad segment");
+        }
+        if (g.ncell > 0) {
+            if (const int rc = flash_buffers(g.p, g.t, g.z, &g.out); rc != MPMC_OK) { return rc; }
+        }
+        if ((g.out.counters != nullptr) != counters) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_create: counters must be given for all segments or none");
+        }
+    }
+    auto pl = std::make_unique<mpmc_flash_plan>();
+    pl->options = o;
+    // segments by component count, then in the given order; a group closes when its
+    // mixture table is full
+    std::vector<int> order(nseg);
+    for (int k = 0; k < nseg; ++k) { order[k] = k; }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return segs[a].mixture->nc < segs[b].mixture->nc; });
+    std::vector<std::vector<int>> members;
+    for (int k : order) {
+        if (segs[k].ncell == 0) { continue; }
+        const int nc = segs[k].mixture->nc;
+        bool fits = !pl->groups.empty() && pl->groups.back().nc == nc;
+        if (fits) {
+            PlanGroup& g = pl->groups.back();
+            const bool known = std::find(g.mixtures.begin(), g.mixtures.end(), segs[k].mixture) != g.mixtures.end();
+            fits = known || static_cast<int>(g.mixtures.size()) < mpmc_flash_multi_max_mixtures(nc);
+            fits = fits && g.n + segs[k].ncell <= (int64_t{1} << 31);
+        }
+        if (!fits) {
+            pl->groups.emplace_back();
+            pl->groups.back().nc = nc;
+            members.emplace_back();
+        }
+        PlanGroup& g = pl->groups.back();
+        if (std::find(g.mixtures.begin(), g.mixtures.end(), segs[k].mixture) == g.mixtures.end()) {
+            g.mixtures.push_back(segs[k].mixture);
+        }
+        g.n += segs[k].ncell;
+        members.back().push_back(k);
+    }
+    if (pl->groups.empty()) {
+        *out = pl.release();
+        return MPMC_OK;
+    }
+    hipError_t e = hipSuccess;
+    size_t ws_need = 0;
+    for (size_t gi = 0; gi < pl->groups.size() && e == hipSuccess; ++gi) {
+        PlanGroup& g = pl->groups[gi];
+        // cells of the group ordered by table entry, segments in the given order within one
+        const int nmix = static_cast<int>(g.mixtures.size());
+        std::vector<std::vector<int>> by_mix(nmix);
+        for (int k : members[gi]) {
+            const int slot = static_cast<int>(std::find(g.mixtures.begin(), g.mixtures.end(), segs[k].mixture) -
+                                              g.mixtures.begin());
+            by_mix[slot].push_back(k);
+        }
+        const int64_t n = g.n;
+        const int nc = g.nc;
+        void* ptr = nullptr;
+        auto take = [&](auto*& dst, size_t count) {
+            using T = std::remove_reference_t<decltype(*dst)>;
+            if (e == hipSuccess) { e = plan_alloc(pl.get(), &ptr, count * sizeof(T)); }
+            dst = static_cast<T*>(e == hipSuccess ? ptr : nullptr);
+        };
+        take(g.p, n);
+        take(g.t, n);
+        take(g.z, static_cast<size_t>(nc) * n);
+        take(g.out.status, n);
+        take(g.out.phase_count, n);
+        take(g.out.beta, 3 * n);
+        take(g.out.zfactor, 3 * n);
+        take(g.out.branch, 3 * n);
+        take(g.out.x, static_cast<size_t>(3 * nc) * n);
+        take(g.out.ln_phi, static_cast<size_t>(3 * nc) * n);
+        if (counters) { take(g.out.counters, 4 * n); }
+        take(g.table, static_cast<size_t>(nmix) * mix_doubles(nc));
+        take(g.mix_of, n);
+        std::vector<SegCopy> gather, scatter;
+        int64_t at = 0;
+        g.offsets.assign(nmix, 0);
+        for (int slot = 0; slot < nmix && e == hipSuccess; ++slot) {
+            g.offsets[slot] = at;
+            const int64_t first = at;
+            for (int k : by_mix[slot]) {
+                const mpmc_flash_segment& sg = segs[k];
+                const int64_t m = sg.ncell;
+                gather.push_back({sg.p, g.p + at, m, m, n, 1, 8});
+                gather.push_back({sg.t, g.t + at, m, m, n, 1, 8});
+                gather.push_back({sg.z, g.z + at, m, m, n, nc, 8});
+                scatter.push_back({g.out.status + at, sg.out.status, m, n, m, 1, 4});
+                scatter.push_back({g.out.phase_count + at, sg.out.phase_count, m, n, m, 1, 4});
+                scatter.push_back({g.out.beta + at, sg.out.beta, m, n, m, 3, 8});
+                scatter.push_back({g.out.zfactor + at, sg.out.zfactor, m, n, m, 3, 8});
+                scatter.push_back({g.out.branch + at, sg.out.branch, m, n, m, 3, 4});
+                scatter.push_back({g.out.x + at, sg.out.x, m, n, m, 3 * nc, 8});
+                scatter.push_back({g.out.ln_phi + at, sg.out.ln_phi, m, n, m, 3 * nc, 8});
+                if (counters) { scatter.push_back({g.out.counters + at, sg.out.counters, m, n, m, 4, 4}); }
+                at += m;
+            }
+            e = hipMemcpy(g.table + slot * mix_doubles(nc), g.mixtures[slot]->dev_constants,
+                          mix_doubles(nc) * sizeof(double), hipMemcpyDeviceToDevice);
+            if (e == hipSuccess && at > first) {
+                hipLaunchKernelGGL(k_fill_mix, dim3(static_cast<unsigned>(std::min<int64_t>((at - first + 255) / 256, 4096))),
+                                   dim3(256), 0, nullptr, g.mix_of + first, at - first, slot);
+                e = hipGetLastError();
+            }
+        }
+        g.ngather = static_cast<int>(gather.size());
+        g.nscatter = static_cast<int>(scatter.size());
+        take(g.gather, gather.size());
+        take(g.scatter, scatter.size());
+        if (e == hipSuccess) { e = hipMemcpy(g.gather, gather.data(), gather.size() * sizeof(SegCopy), hipMemcpyHostToDevice); }
+        if (e == hipSuccess) { e = hipMemcpy(g.scatter, scatter.data(), scatter.size() * sizeof(SegCopy), hipMemcpyHostToDevice); }
+        ws_need = std::max(ws_need, workspace_bytes_for(nc, n));
+    }
+    if (e == hipSuccess) {
+        e = plan_alloc(pl.get(), &pl->workspace, ws_need);
+        pl->workspace_bytes = ws_need;
+    }
+    if (e == hipSuccess) { e = hipDeviceSynchronize(); }
+    if (e != hipSuccess) {
+        mpmc_flash_plan_destroy(pl.release());
+        return e == hipErrorOutOfMemory ? fail(MPMC_E_NO_MEMORY, "flash_plan_create: device allocation failed")
+                                        : hip_fail(e, "flash_plan_create");
+    }
+    // one eager run settles every lazily computed launch parameter, then the sequence is
+    // captured once into a graph
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) { e = plan_record(pl.get(), cs); }
+    if (e == hipSuccess) { e = hipStreamSynchronize(cs); }
+    if (e == hipSuccess) { e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal); }
+    if (e == hipSuccess) {
+        const hipError_t rec = plan_record(pl.get(), cs);
+        e = hipStreamEndCapture(cs, &pl->graph);
+        if (rec != hipSuccess) { e = rec; }
+    }
+    if (e == hipSuccess) { e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0); }
+    if (cs) { (void)hipStreamDestroy(cs); }
+    if (e != hipSuccess) {
+        mpmc_flash_plan_destroy(pl.release());
+        return hip_fail(e, "flash_plan_create capture");
+    }
+    *out = pl.release();
+    return MPMC_OK;
+}
+
+int mpmc_flash_plan_launch(mpmc_flash_plan_t pl, void* stream) {
+    if (!pl) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_launch: null plan"); }
+    if (!pl->exec) { return MPMC_OK; }
+    const hipError_t e = hipGraphLaunch(pl->exec, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_plan_launch");
+}
+
+int mpmc_flash_plan_info(mpmc_flash_plan_t pl, int32_t* groups, size_t* device_bytes) {
+    if (!pl) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_plan_info: null plan"); }
+    if (groups) { *groups = static_cast<int32_t>(pl->groups.size()); }
+    if (device_bytes) { *device_bytes = pl->device_bytes; }
+    return MPMC_OK;
+}
+
+int mpmc_flash_plan_destroy(mpmc_flash_plan_t pl) {
+    if (!pl) { return MPMC_OK; }
+    hipError_t e = hipSuccess;
+    if (pl->exec || !pl->allocations.empty()) { e = hipDeviceSynchronize(); }
+    if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); }
+    if (pl->graph) { (void)hipGraphDestroy(pl->graph); }
+    for (void* ptr : pl->allocations) { (void)hipFree(ptr); }
+    delete pl;
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_plan_destroy");
 }
 
 int mpmc_phase_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* x,

@@ -102,6 +102,10 @@ struct Pipe {
     int32_t* m_br;      // [3][n]
     unsigned long long* g; // [16]
     int32_t* fb_list;   // [n]
+    // Ragged multi-mixture batches: cell c belongs to mixture mix_of[c] of the launch's
+    // table of nmix mixtures (all with NC components); mix_of == nullptr: one mixture.
+    const int32_t* mix_of;
+    int nmix;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -118,10 +122,21 @@ __device__ __forceinline__ int64_t fetch_task(unsigned long long* counter, bool 
     return want ? static_cast<int64_t>(base) + rank : -1;
 }
 
+// The launch's mixture table (nmix consecutive Mix<NC> records) staged in LDS; the
+// launch passes nmix * sizeof(Mix<NC>) bytes of dynamic LDS.
 template <int NC>
-__device__ __forceinline__ void stage_mix(const double* __restrict__ mixc, Mix<NC>& smix) {
-    load_mix<NC>(mixc, smix);
+__device__ __forceinline__ Mix<NC>* stage_mixes(const double* __restrict__ mixc, int nmix) {
+    extern __shared__ double lds_mix[];
+    constexpr int kd = static_cast<int>(sizeof(Mix<NC>) / sizeof(double));
+    const int total = (nmix > 1 ? nmix : 1) * kd;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) { lds_mix[k] = mixc[k]; }
     __syncthreads();
+    return reinterpret_cast<Mix<NC>*>(lds_mix);
+}
+
+template <int NC>
+__device__ __forceinline__ int mix_at(const Pipe<NC>& P, int64_t c) {
+    return P.mix_of ? P.mix_of[c] : 0;
 }
 
 template <int NC>
@@ -155,8 +170,7 @@ static __global__ void k_reset(unsigned long long* g) {
 
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restrict__ mixc) {
-    __shared__ Mix<NC> smix;
-    stage_mix<NC>(mixc, smix);
+    Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int64_t n = P.n;
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -182,7 +196,7 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
         Activity<NC> ref;
         if (!rejected && !lost) {
             ++prop;
-            const int e = stability_provider<NC, kN>(smix, p, t, feed, ref);
+            const int e = stability_provider<NC, kN>(smix[mix_at(P, c)], p, t, feed, ref);
             if (e == REJECTED) { rejected = true; }
             else if (e == ISSUE_NONE) { info |= ref.branch | (ref.smooth ? 0x100 : 0) | 0x200; }
         }
@@ -209,6 +223,7 @@ template <int NC>
 struct TpdLane {
     int64_t slot;   // c (initial) or u (review)
     int64_t cell;
+    int mid;        // the cell's mixture in the launch's table
     int k, mode, bt, it;
     uint32_t evals;
     double p, t, tol;
@@ -224,7 +239,7 @@ struct TpdLane {
 // stage 0: initial feed test; stage 1: common-tangent review of the selected pair.
 // kResume: FP64 continuation of a descent parked by the FP32 phase (mixed precision).
 template <int NC, bool kF32, bool kResume>
-__device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>& smix, int stage, int k, int64_t slot, TpdLane<NC>& L) {
+__device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, int k, int64_t slot, TpdLane<NC>& L) {
     const int64_t n = P.n;
     int64_t c;
     int32_t* st_arr = stage == 0 ? P.tst : P.fst;
@@ -247,6 +262,7 @@ __device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>& smix, int stage, int
     }
     L.slot = slot;
     L.cell = c;
+    L.mid = mix_at(P, c);
     L.k = k;
     L.p = P.p[c];
     L.t = P.t[c];
@@ -303,7 +319,7 @@ __device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>& smix, int stage, int
     L.evals0 = 0;
     L.it = 0;
     L.mode = TM_START;
-    if (kF32) { cell_f32<NC>(smix, L.t, L.cf); }
+    if (kF32) { cell_f32<NC>(smix[L.mid], L.t, L.cf); }
     if (lost) {
         st_arr[k * n + slot] = T_UNREPRESENTABLE;
         (stage == 0 ? P.tev : P.fev)[k * n + slot] = 0;
@@ -384,8 +400,7 @@ __device__ __forceinline__ int tpd_check_f32(TpdLane<NC>& L) {
 // kResume: its FP64 continuation; neither: the FP64 search from the trial start.
 template <int NC, bool kN, bool kF32, bool kResume>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
-    __shared__ Mix<NC> smix;
-    stage_mix<NC>(mixc, smix);
+    Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int tk = stage == 0 ? NC + 2 : NC + 4;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
@@ -438,11 +453,11 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
             if (f32) {
                 float lp[NC];
                 int br = 0;
-                e = stability_provider_f32<NC>(smix, L.cf, L.p, L.t, L.cand, lp, br);
+                e = stability_provider_f32<NC>(smix[L.mid], L.cf, L.p, L.t, L.cand, lp, br);
                 if (e == ISSUE_NONE) { tpd_point_f32<NC>(L.cand, L.z, lp, br, L.ref, nx); }
             } else {
                 Activity<NC> ph;
-                e = stability_provider<NC, kN>(smix, L.p, L.t, L.cand, ph);
+                e = stability_provider<NC, kN>(smix[L.mid], L.p, L.t, L.cand, ph);
                 if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
             }
             if (e == REJECTED || rejected) {
@@ -645,8 +660,7 @@ __device__ __forceinline__ int split_check(SplitLane<NC>& L, double fug_tol) {
 
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc) {
-    __shared__ Mix<NC> smix;
-    stage_mix<NC>(mixc, smix);
+    Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = kAttempts * nu;
     SplitLane<NC> L;
@@ -668,7 +682,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const
         for (int i = 0; i < NC; ++i) { lk[i] = L.mode == TM_START ? L.lk0[i] : L.pt.lk[i] + (L.alpha / L.scale) * L.pt.r[i]; }
         SplitState<NC> nx;
         bool rejected = false;
-        const int e = split_evaluate<NC, kN>(smix, L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
+        const int e = split_evaluate<NC, kN>(smix[mix_at(P, L.cell)], L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
                                          L.mode == TM_START ? 0.5 : L.pt.fr.beta);
         if (rejected) {
             atomicOr(&P.flags[L.cell], CF_REJECTED);
@@ -794,8 +808,7 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
 // ------------------------------------------------------------------ max3 tail
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __restrict__ mixc) {
-    __shared__ Mix<NC> smix;
-    stage_mix<NC>(mixc, smix);
+    Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int64_t n = P.n;
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
     bool exhausted = false;
@@ -821,7 +834,7 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
         Counters cnt{0, 0, 0, 0};
         bool rejected = false;
         Max3Tail<NC> M;
-        max3_tail<NC, kN>(smix, P.p[c], P.t[c], z, x, y, beta, nwit,
+        max3_tail<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], z, x, y, beta, nwit,
                       [&](int i, double* w, int& branch) {
                           int seen = -1, k = 0;
                           for (; k < ntr; ++k) {
@@ -991,8 +1004,7 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
 // Cells whose searches a shared evaluation budget would have cut: the serial path.
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __restrict__ mixc) {
-    __shared__ Mix<NC> smix;
-    stage_mix<NC>(mixc, smix);
+    Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int64_t n = P.n;
     const int64_t nf = static_cast<int64_t>(P.g[G_NFALLBACK]);
     bool exhausted = false;
@@ -1008,12 +1020,12 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
         PhaseView<NC> ph[3];
         if (P.max_phases == 2) {
             VleOut<NC> V;
-            solve_vle<NC, kN>(smix, P.p[c], P.t[c], feed, P.fug_tol, nullptr, 0, V, cnt, rejected);
+            solve_vle<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, nullptr, 0, V, cnt, rejected);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_vle<NC>(P.out, n, c, V, 0xF, cnt); }
         } else {
             Max3Out<NC> M;
-            solve_max3<NC, kN>(smix, P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected);
+            solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_max3<NC>(P.out, n, c, M, cnt); }
         }
@@ -1074,60 +1086,83 @@ size_t pipeline_bytes(int64_t n) {
     return pipe_carve<NC>(P, nullptr, n);
 }
 
-// Persistent-grid size: resident workgroups per CU for `kernel` times the CU count.
+// Persistent-grid size: resident workgroups per CU for `kernel` (with `lds` bytes of
+// dynamic LDS) times the CU count.
 template <typename K>
-inline unsigned persistent_blocks(K kernel) {
+inline unsigned persistent_blocks(K kernel, size_t lds) {
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess || per_cu < 1) { per_cu = 1; }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds) != hipSuccess || per_cu < 1) { per_cu = 1; }
     if (cus < 1) { cus = 1; }
     return static_cast<unsigned>(cus * per_cu);
 }
 
+// Grids of the persistent kernels for one LDS size (the one-mixture size is computed
+// once; a multi-mixture table is sized per call).
+struct PipeGrids {
+    unsigned tpd, tpd32, split, tail, fb;
+};
 template <int NC, bool kN>
-hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, hipStream_t s) {
-    static const unsigned b_tpd = persistent_blocks(k_tpd<NC, kN, false, kN>);
-    static const unsigned b_split = persistent_blocks(k_split<NC, kN>);
+PipeGrids pipe_grids(size_t lds) {
+    PipeGrids g{};
+    g.tpd = persistent_blocks(k_tpd<NC, kN, false, kN>, lds);
+    g.tpd32 = kN ? persistent_blocks(k_tpd<NC, true, true, false>, lds) : 0u;
+    g.split = persistent_blocks(k_split<NC, kN>, lds);
     // The three-phase tail and the serial fallback carry 8-22 KB of private memory per lane
     // and serve few cells; their grids are capped so that one dispatch reserves at most
     // ~100 MB of scratch (8192 and 1024 lanes), which keeps several launch streams of
     // such dispatches within the runtime's scratch limits.
-    static const unsigned b_tail = std::min(persistent_blocks(k_max3_tail<NC, kN>), 128u);
-    static const unsigned b_fb = std::min(persistent_blocks(k_fallback<NC, kN>), 16u);
+    g.tail = std::min(persistent_blocks(k_max3_tail<NC, kN>, lds), 128u);
+    g.fb = std::min(persistent_blocks(k_fallback<NC, kN>, lds), 16u);
+    return g;
+}
+
+template <int NC, bool kN>
+hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, hipStream_t s) {
+    const size_t lds = static_cast<size_t>(P.nmix > 1 ? P.nmix : 1) * sizeof(Mix<NC>);
+    static const PipeGrids one = pipe_grids<NC, kN>(sizeof(Mix<NC>));
+    const PipeGrids b = P.nmix > 1 ? pipe_grids<NC, kN>(lds) : one;
     const int64_t n = P.n;
     const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
-    hipLaunchKernelGGL((k_prep<NC, kN>), dim3(cell_blocks), dim3(64), 0, s, P, mixc);
+    // a multi-mixture table is staged in LDS by every workgroup: fewer, longer-lived
+    // workgroups for the cell-parallel preparation
+    const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
+    hipLaunchKernelGGL((k_prep<NC, kN>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
     if constexpr (kN) {
-        static const unsigned b_tpd32 = persistent_blocks(k_tpd<NC, true, true, false>);
-        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b_tpd32), dim3(64), 0, s, P, mixc, 0);
+        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b.tpd32), dim3(64), lds, s, P, mixc, 0);
     }
-    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b_tpd), dim3(64), 0, s, P, mixc, 0);
+    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0);
     hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_split<NC, kN>), dim3(b_split), dim3(64), 0, s, P, mixc);
+    hipLaunchKernelGGL((k_split<NC, kN>), dim3(b.split), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL(k_split_select<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     if constexpr (kN) {
-        static const unsigned b_tpd32 = persistent_blocks(k_tpd<NC, true, true, false>);
-        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b_tpd32), dim3(64), 0, s, P, mixc, 1);
+        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b.tpd32), dim3(64), lds, s, P, mixc, 1);
     }
-    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b_tpd), dim3(64), 0, s, P, mixc, 1);
+    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1);
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, kN>), dim3(b_tail), dim3(64), 0, s, P, mixc); }
+    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, kN>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
     hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_fallback<NC, kN>), dim3(b_fb), dim3(64), 0, s, P, mixc);
+    hipLaunchKernelGGL((k_fallback<NC, kN>), dim3(b.fb), dim3(64), lds, s, P, mixc);
     return hipGetLastError();
 }
 
 // precision 0: FP64 evaluations with the reference's bisection roots (bit-level
 // parity); 1: mixed precision (FP32 descent phase, Newton roots, FP64 decisions).
+//
+// mixc: nmix consecutive Mix<NC> records; mix_of[n] (device) names each cell's record,
+// nullptr when nmix == 1.
 template <int NC>
-hipError_t launch_pipeline(const double* mixc, void* workspace, size_t workspace_bytes, int64_t n, const double* p,
-                           const double* t, const double* z, int max_phases, double fug_tol, int precision,
-                           const FlashOut& out, hipStream_t s) {
+hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, void* workspace,
+                           size_t workspace_bytes, int64_t n, const double* p, const double* t, const double* z,
+                           int max_phases, double fug_tol, int precision, const FlashOut& out, hipStream_t s) {
     Pipe<NC> P{};
     const size_t need = pipe_carve<NC>(P, static_cast<char*>(workspace), n);
     if (!workspace || need > workspace_bytes) { return hipErrorInvalidValue; }
+    if (nmix < 1 || (nmix > 1 && !mix_of)) { return hipErrorInvalidValue; }
+    P.mix_of = nmix > 1 ? mix_of : nullptr;
+    P.nmix = nmix;
     P.n = n;
     P.max_phases = max_phases;
     P.fug_tol = fug_tol;
@@ -1141,12 +1176,13 @@ hipError_t launch_pipeline(const double* mixc, void* workspace, size_t workspace
 
 #define MPMC_DECLARE_PIPELINE(N)                                                                                  \
     extern template size_t pipeline_bytes<N>(int64_t);                                                            \
-    extern template hipError_t launch_pipeline<N>(const double*, void*, size_t, int64_t, const double*,           \
-                                                  const double*, const double*, int, double, int, const FlashOut&,\
-                                                  hipStream_t);
+    extern template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,     \
+                                                  const double*, const double*, const double*, int, double, int,  \
+                                                  const FlashOut&, hipStream_t);
 #define MPMC_INSTANTIATE_PIPELINE(N)                                                                              \
     template size_t pipeline_bytes<N>(int64_t);                                                                   \
-    template hipError_t launch_pipeline<N>(const double*, void*, size_t, int64_t, const double*, const double*,   \
-                                           const double*, int, double, int, const FlashOut&, hipStream_t);
+    template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,            \
+                                           const double*, const double*, const double*, int, double, int,         \
+                                           const FlashOut&, hipStream_t);
 
 } // namespace mpmc::dev

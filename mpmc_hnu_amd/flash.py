@@ -148,6 +148,108 @@ def workspace_bytes(nc: int, ncell: int) -> int:
     return int(native.library().mpmc_flash_workspace_bytes(nc, ncell))
 
 
+def _options(fugacity_tolerance, max_phases, precision) -> native.FlashOptions:
+    opts = native.FlashOptions()
+    native.library().mpmc_flash_default_options(ctypes.byref(opts))
+    opts.fugacity_tolerance = float(fugacity_tolerance)
+    opts.max_phases = int(max_phases)
+    opts.precision = int(precision)
+    return opts
+
+
+def _outputs_struct(out: dict) -> native.FlashOutputs:
+    return native.FlashOutputs(
+        out["status"].data_ptr(), out["phase_count"].data_ptr(), out["beta"].data_ptr(), out["zfactor"].data_ptr(),
+        out["branch"].data_ptr(), out["x"].data_ptr(), out["ln_phi"].data_ptr(),
+        out["counters"].data_ptr() if out.get("counters") is not None else None)
+
+
+def multi_max_mixtures(nc: int) -> int:
+    return int(native.library().mpmc_flash_multi_max_mixtures(nc))
+
+
+def flash_multi(mixtures, cell_offset, p, t, z, fugacity_tolerance: float = 1e-11, max_phases: int = 3,
+                precision: int = 0, stream=None, out: dict | None = None, counters: bool = True,
+                workspace=None) -> dict:
+    """Ragged multi-mixture batch (mpmc_flash_multi): mixtures[k] owns cells
+    [cell_offset[k], cell_offset[k+1]) of p[ncell], t[ncell], z[nc, ncell]; all mixtures
+    have the same nc. Results equal flash_batch on each mixture's cells."""
+    torch = _torch()
+    lib = native.library()
+    device = torch.device("cuda", torch.cuda.current_device())
+    nc = mixtures[0].nc
+    p = _dev(p, torch.float64, device)
+    t = _dev(t, torch.float64, device)
+    z = _dev(z, torch.float64, device)
+    n = p.numel()
+    off = np.ascontiguousarray(cell_offset, dtype=np.int64)
+    if off.size != len(mixtures) + 1 or off[-1] != n or z.shape != (nc, n):
+        raise ValueError("flash_multi: expected len(mixtures) + 1 offsets ending at ncell and z[nc, ncell]")
+    if out is None:
+        out = allocate_outputs(nc, n, device, counters)
+    need = int(lib.mpmc_flash_multi_workspace_bytes(nc, len(mixtures), n))
+    if workspace is None or workspace.numel() * workspace.element_size() < need:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=device)
+    handles = (ctypes.c_void_p * len(mixtures))(*[m.handle.value if isinstance(m.handle, ctypes.c_void_p) else m.handle
+                                                   for m in mixtures])
+    opts = _options(fugacity_tolerance, max_phases, precision)
+    o = _outputs_struct(out)
+    V = ctypes.c_void_p
+    native.check(lib.mpmc_flash_multi(len(mixtures), handles, off.ctypes.data_as(V), V(p.data_ptr()),
+                                      V(t.data_ptr()), V(z.data_ptr()), ctypes.byref(opts), ctypes.byref(o),
+                                      V(workspace.data_ptr()), workspace.numel() * workspace.element_size(),
+                                      _stream_ptr(stream)), "flash_multi")
+    out["_inputs"] = (p, t, z, workspace)
+    return out
+
+
+class Plan:
+    """A prepared uneven batch (mpmc_flash_plan_*): ``segments`` is a list of
+    (mixture, p, t, z) with device tensors; outputs are allocated per segment. ``launch``
+    replays the captured graph over the segments' current input contents."""
+
+    def __init__(self, segments, fugacity_tolerance: float = 1e-11, max_phases: int = 3, precision: int = 0,
+                 counters: bool = True):
+        torch = _torch()
+        lib = native.library()
+        device = torch.device("cuda", torch.cuda.current_device())
+        self.inputs, self.outputs = [], []
+        segs = (native.FlashSegment * len(segments))()
+        for k, (mix, p, t, z) in enumerate(segments):
+            p = _dev(p, torch.float64, device)
+            t = _dev(t, torch.float64, device)
+            z = _dev(z, torch.float64, device)
+            n = p.numel()
+            if z.shape != (mix.nc, n) or t.numel() != n:
+                raise ValueError("Plan: expected p[n], t[n], z[nc, n] per segment")
+            out = allocate_outputs(mix.nc, n, device, counters)
+            self.inputs.append((mix, p, t, z))
+            self.outputs.append(out)
+            h = mix.handle.value if isinstance(mix.handle, ctypes.c_void_p) else mix.handle
+            segs[k] = native.FlashSegment(h, n, p.data_ptr(), t.data_ptr(), z.data_ptr(), _outputs_struct(out))
+        opts = _options(fugacity_tolerance, max_phases, precision)
+        self.handle = ctypes.c_void_p()
+        native.check(lib.mpmc_flash_plan_create(len(segments), segs, ctypes.byref(opts), ctypes.byref(self.handle)),
+                     "flash_plan_create")
+        g, b = ctypes.c_int32(0), ctypes.c_size_t(0)
+        native.check(lib.mpmc_flash_plan_info(self.handle, ctypes.byref(g), ctypes.byref(b)), "flash_plan_info")
+        self.groups, self.device_bytes = int(g.value), int(b.value)
+
+    def launch(self, stream=None):
+        native.check(native.library().mpmc_flash_plan_launch(self.handle, _stream_ptr(stream)), "flash_plan_launch")
+
+    def close(self):
+        if self.handle:
+            native.library().mpmc_flash_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def phase_batch(mixture: Mixture, p, t, x, stream=None) -> dict:
     """All roots and the ln(phi) of each admissible root at x[nc, ncell]."""
     torch = _torch()

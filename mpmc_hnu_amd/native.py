@@ -24,6 +24,13 @@ EXPORTED = (
     "mpmc_flash_batch",
     "mpmc_flash_batch_ws",
     "mpmc_flash_workspace_bytes",
+    "mpmc_flash_multi_max_mixtures",
+    "mpmc_flash_multi_workspace_bytes",
+    "mpmc_flash_multi",
+    "mpmc_flash_plan_create",
+    "mpmc_flash_plan_launch",
+    "mpmc_flash_plan_info",
+    "mpmc_flash_plan_destroy",
     "mpmc_phase_batch",
     "mpmc_rr_batch",
     "mpmc_closure_default_options",
@@ -68,6 +75,17 @@ class FlashOutputs(ctypes.Structure):
         ("x", ctypes.c_void_p),
         ("ln_phi", ctypes.c_void_p),
         ("counters", ctypes.c_void_p),
+    ]
+
+
+class FlashSegment(ctypes.Structure):
+    _fields_ = [
+        ("mixture", ctypes.c_void_p),
+        ("ncell", ctypes.c_int64),
+        ("p", ctypes.c_void_p),
+        ("t", ctypes.c_void_p),
+        ("z", ctypes.c_void_p),
+        ("out", FlashOutputs),
     ]
 
 
@@ -153,6 +171,16 @@ def library() -> ctypes.CDLL:
                                         ctypes.POINTER(FlashOutputs), P, ctypes.c_size_t, P]
     lib.mpmc_flash_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64]
     lib.mpmc_flash_workspace_bytes.restype = ctypes.c_size_t
+    lib.mpmc_flash_multi_max_mixtures.argtypes = [ctypes.c_int]
+    lib.mpmc_flash_multi_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64]
+    lib.mpmc_flash_multi_workspace_bytes.restype = ctypes.c_size_t
+    lib.mpmc_flash_multi.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.POINTER(FlashOptions),
+                                     ctypes.POINTER(FlashOutputs), P, ctypes.c_size_t, P]
+    lib.mpmc_flash_plan_create.argtypes = [ctypes.c_int, ctypes.POINTER(FlashSegment), ctypes.POINTER(FlashOptions),
+                                           ctypes.POINTER(P)]
+    lib.mpmc_flash_plan_launch.argtypes = [P, P]
+    lib.mpmc_flash_plan_info.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_size_t)]
+    lib.mpmc_flash_plan_destroy.argtypes = [P]
     lib.mpmc_phase_batch.argtypes = [P, ctypes.c_int64, P, P, P, P, P, P, P, P, P]
     lib.mpmc_rr_batch.argtypes = [ctypes.c_int, ctypes.c_int64] + [P] * 9
     lib.mpmc_closure_default_options.argtypes = [ctypes.POINTER(ClosureOptions)]

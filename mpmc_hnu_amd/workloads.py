@@ -244,3 +244,82 @@ def two_phase_state(nc: int, ncell: int, roots, seed: int = 6, dims=None, p_rang
     if todo.size:
         raise RuntimeError(f"two_phase_state: {todo.size} cells without a valid root")
     return np.ascontiguousarray(q), branch
+
+
+# ---------------------------------------------------------------- uneven multi-mixture batch
+def uneven_layout(seed: int = 9, nmix: int = 256, scale: float = 1.0, nc_range=(2, 12),
+                  cells_range=(1.0e3, 2.0e5)) -> list:
+    """Mixtures and cell counts of the uneven batch: nc ~ U{2..12}, each mixture the
+    linspaced family of its nc with its own kij row (seed 1000 + k), cells per mixture
+    log-uniform on [1e3, 2e5] (times ``scale``), about 9.6M cells in all at scale 1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(nmix):
+        nc = int(rng.integers(nc_range[0], nc_range[1] + 1))
+        n = int(math.exp(rng.uniform(math.log(cells_range[0]), math.log(cells_range[1]))) * scale)
+        out.append({"nc": nc, "mixture": mixture_linspaced(nc, seed=1000 + k), "ncell": max(n, 1)})
+    return out
+
+
+def uneven_base_points(layout: list, seed: int = 9, points: int = 32) -> list:
+    """Per mixture, the (T, z) points whose boundary pressures seed the near-boundary
+    cells: T ~ U[320, 420] K, z ~ Dirichlet(2)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    return [(rng.uniform(320.0, 420.0, size=points), _dirichlet(rng, 2.0, m["nc"], points)) for m in layout]
+
+
+def uneven_cells(layout: list, base: list, boundary: list, seed: int = 9, near_fraction: float = 0.3) -> list:
+    """Cells of each mixture: a ``near_fraction`` share sits within +-2% in P of the
+    boundary pressure of one of the mixture's base points (``boundary[k][j]``, NaN where
+    none was found), at that point's T and z; the rest are P ~ U[5, 40] MPa,
+    T ~ U[320, 420] K, z ~ Dirichlet(2). Returns [(p, t, z)] per mixture, cells shuffled."""
+    rng = np.random.Generator(np.random.PCG64(seed + 2))
+    out = []
+    for m, (tb, zb), pb in zip(layout, base, boundary):
+        n, nc = m["ncell"], m["nc"]
+        ok = np.nonzero(np.isfinite(pb))[0]
+        n_near = int(round(near_fraction * n)) if ok.size else 0
+        p = rng.uniform(5.0, 40.0, size=n) * MPA
+        t = rng.uniform(320.0, 420.0, size=n)
+        z = _dirichlet(rng, 2.0, nc, n)
+        if n_near:
+            j = ok[rng.integers(0, ok.size, size=n_near)]
+            p[:n_near] = pb[j] * (1.0 + rng.uniform(-0.02, 0.02, size=n_near))
+            t[:n_near] = tb[j]
+            z[:, :n_near] = zb[:, j]
+        perm = rng.permutation(n)
+        out.append((np.ascontiguousarray(p[perm]), np.ascontiguousarray(t[perm]), np.ascontiguousarray(z[:, perm])))
+    return out
+
+
+def boundary_pressures(phase_count, items: list, lo: float = 0.5 * MPA, hi: float = 60.0 * MPA, grid: int = 24,
+                       steps: int = 12) -> list:
+    """Highest pressure in [lo, hi] at which the phase count changes, per (T, z) point, by a
+    log-spaced scan and bisection on P. ``phase_count(requests)`` flashes a list of
+    (item index, p[m], t[m], z[nc, m]) requests at once and returns the phase counts per
+    request; ``items`` is a list of (T[m], z[nc, m]). NaN where the count never changes."""
+    pgrid = np.exp(np.linspace(np.log(lo), np.log(hi), grid))
+    reqs = []
+    for k, (tb, zb) in enumerate(items):
+        m = tb.size
+        reqs.append((k, np.repeat(pgrid, m), np.tile(tb, grid), np.tile(zb, (1, grid))))
+    counts = phase_count(reqs)
+    lo_p, hi_p, lo_c, found = [], [], [], []
+    for (k, _, _, _), c in zip(reqs, counts):
+        m = items[k][0].size
+        c = c.reshape(grid, m)
+        change = c[1:] != c[:-1]
+        has = change.any(axis=0)
+        last = np.where(has, grid - 2 - np.argmax(change[::-1], axis=0), 0)
+        lo_p.append(pgrid[last])
+        hi_p.append(pgrid[last + 1])
+        lo_c.append(c[last, np.arange(m)])
+        found.append(has)
+    for _ in range(steps):
+        mid = [np.sqrt(a * b) for a, b in zip(lo_p, hi_p)]
+        counts = phase_count([(k, mid[k], items[k][0], items[k][1]) for k in range(len(items))])
+        for k, c in enumerate(counts):
+            same = c == lo_c[k]
+            lo_p[k] = np.where(same, mid[k], lo_p[k])
+            hi_p[k] = np.where(same, hi_p[k], mid[k])
+    return [np.where(f, np.sqrt(a * b), np.nan) for f, a, b in zip(found, lo_p, hi_p)]
