@@ -116,42 +116,6 @@ MPMC_HOT int tpd_point(const double* w, const double* z, const Activity<NC>& tri
 
 __device__ __forceinline__ bool tpd_negative(double value, double guard, double tol) { return value < -tol - guard; }
 
-// Tangent-plane distance from an FP32 branch evaluation (mixed-precision descent only):
-// logarithms in FP32, sums in FP64, and a roundoff guard at FP32 resolution.
-template <int NC>
-__device__ __forceinline__ void tpd_point_f32(const double* w, const double* z, const float* lnphi, int branch,
-                                              const Activity<NC>& ref, TpdPoint<NC>& r) {
-    double sum = 0, value = 0, mag = 1;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) { sum += w[i]; }
-    r.branch = branch;
-    r.smooth = true;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        r.w[i] = w[i];
-        r.res[i] = 0;
-        if (z[i] == 0 || w[i] == 0) { continue; }
-        const float wf = static_cast<float>(w[i]), zf = static_cast<float>(z[i]);
-        const float lw = logf(wf), lz = logf(zf);
-        const float lr = __builtin_fabsf(wf - zf) < 0.5f * zf ? log1pf((wf - zf) / zf) : lw - lz;
-        const float q = lr + (lnphi[i] - static_cast<float>(ref.lnphi[i]));
-        r.res[i] = q;
-        value += w[i] * static_cast<double>(q);
-        mag += w[i] * static_cast<double>(__builtin_fabsf(lw) + __builtin_fabsf(lz) + __builtin_fabsf(lnphi[i]) +
-                                          __builtin_fabsf(static_cast<float>(ref.lnphi[i])));
-    }
-    r.value = value;
-    const double mean = value / sum;
-    r.stat = 0;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        if (z[i] == 0) { continue; }
-        r.res[i] = w[i] == 0 ? -kInf : r.res[i] - mean;
-        r.stat = smax(r.stat, __builtin_fabs(r.res[i]));
-    }
-    r.guard = 64.0 * 1.1920928955078125e-07 * mag;
-}
-
 // Multistart TPD search (detail::test_pt_stability_impl, pt_stability.hpp:383-616). With
 // imposed == nullptr the reference phase is the least-Gibbs feed phase; otherwise the
 // imposed common log-activities define it (test_pt_stability_against, :637-644).

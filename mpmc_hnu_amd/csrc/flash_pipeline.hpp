@@ -32,7 +32,9 @@
 // where a budget would have cut a search to k_fallback, which runs the serial path.
 #pragma once
 
+#include "closure_device.hpp"
 #include "flash_kernels.hpp"
+#include "mixed_device.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -53,13 +55,16 @@ namespace mpmc::dev {
 constexpr int kAttempts = 16;   // PtSplitOptions::max_split_attempts (pt_split.hpp:49)
 constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
-enum CellFlag : int32_t { CF_REJECTED = 1, CF_FALLBACK = 2 };
-enum GlobalSlot : int { G_NU = 0, G_NFALLBACK = 1, G_TASK_INIT = 2, G_TASK_SPLIT = 3, G_TASK_REVIEW = 4,
-                        G_TASK_MAX3 = 5, G_TASK_FALLBACK = 6, G_TASK_INIT64 = 7, G_TASK_REVIEW64 = 8, G_SLOTS = 16 };
+// Outcomes of a reduced-precision trial or split attempt that FP64 must decide.
+constexpr int32_t T_AMBIG = 0x7D;
+constexpr int A_AMBIG = 0x7D;
 
-// Trial slot status of a descent parked by the FP32 phase (mixed precision); the FP64
-// continuation resumes it from the stored point. Iteration count in bits 16..
-constexpr int32_t kParked = 0xFE;
+// CF_FALLBACK: the cell is finished by another path (the serial FP64 path, or in mixed
+// precision the FP64 schedule); CF_RERUN: the cell belongs to that FP64 re-run.
+enum CellFlag : int32_t { CF_REJECTED = 1, CF_FALLBACK = 2, CF_RERUN = 4 };
+enum GlobalSlot : int { G_NU = 0, G_NFALLBACK = 1, G_TASK_INIT = 2, G_TASK_SPLIT = 3, G_TASK_REVIEW = 4,
+                        G_TASK_MAX3 = 5, G_TASK_FALLBACK = 6, G_TASK_INIT32 = 7, G_TASK_REVIEW32 = 8,
+                        G_TASK_SPLIT32 = 9, G_SLOTS = 16 };
 
 // Device workspace carved from one allocation (pipe_carve). n = cells; arrays
 // indexed by the compact unstable index u use capacity n.
@@ -102,10 +107,18 @@ struct Pipe {
     int32_t* m_br;      // [3][n]
     unsigned long long* g; // [16]
     int32_t* fb_list;   // [n]
+    // per-cell constants of the reduced-precision searches (mixed precision only)
+    float* ra32;        // [NC][n] sqrt(a_i(T))
+    float* lz32;        // [NC][n] ln z_i
+    float* d32;         // [NC][n] ln z_i + ln phi_i(reference phase)
     // Ragged multi-mixture batches: cell c belongs to mixture mix_of[c] of the launch's
     // table of nmix mixtures (all with NC components); mix_of == nullptr: one mixture.
     const int32_t* mix_of;
     int nmix;
+    // mixed: the reduced-precision pass of precision 1; rerun: the FP64 schedule restricted
+    // to the cells flagged CF_RERUN
+    int mixed;
+    int rerun;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -174,6 +187,7 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
     const int64_t n = P.n;
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (P.rerun && !(P.flags[c] & CF_RERUN)) { continue; }
         double z[NC];
 #pragma unroll
         for (int i = 0; i < NC; ++i) { z[i] = P.zin[i * n + c]; }
@@ -206,7 +220,20 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
             P.refl[i * n + c] = (info & 0x200) ? ref.lnphi[i] : 0.0;
         }
         P.refinfo[c] = info;
-        P.flags[c] = rejected ? CF_REJECTED : 0;
+        P.flags[c] = (rejected ? CF_REJECTED : 0) | (P.rerun ? CF_RERUN : 0);
+        if (P.mixed) {
+            const Mix<NC>& m = smix[mix_at(P, c)];
+            const double st = __builtin_sqrt(t);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const double br = 1.0 + m.kappa[i] * (1.0 - st / m.sqrt_tc[i]);
+                const double a = (br * m.ac[i]) * br;
+                const double lz = feed[i] > 0 ? log(feed[i]) : 0.0;
+                P.ra32[i * n + c] = static_cast<float>(__builtin_sqrt(a > 0 ? a : 0.0));
+                P.lz32[i * n + c] = static_cast<float>(lz);
+                P.d32[i * n + c] = static_cast<float>(lz + ((info & 0x200) ? ref.lnphi[i] : 0.0));
+            }
+        }
         P.cstat[c] = S_INDETERMINATE;
         P.cell_u[c] = -1;
         P.cnt[0 * n + c] = prop;
@@ -232,21 +259,18 @@ struct TpdLane {
     TpdPoint<NC> pt;
     double dir[NC], lw[NC], cand[NC];
     double slope, scale, alpha;
-    uint32_t evals0;   // evaluations already counted by the FP32 phase
-    CellF32<NC> cf;
 };
 
 // stage 0: initial feed test; stage 1: common-tangent review of the selected pair.
-// kResume: FP64 continuation of a descent parked by the FP32 phase (mixed precision).
-template <int NC, bool kF32, bool kResume>
-__device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, int k, int64_t slot, TpdLane<NC>& L) {
+template <int NC>
+__device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, TpdLane<NC>& L) {
     const int64_t n = P.n;
     int64_t c;
     int32_t* st_arr = stage == 0 ? P.tst : P.fst;
-    if (kResume && (st_arr[k * n + slot] & 0xFF) != kParked) { return false; }
     if (stage == 0) {
         c = slot;
         if (P.flags[c] & CF_REJECTED) { return false; }
+        if (P.rerun && !(P.flags[c] & CF_RERUN)) { return false; }
         const int32_t info = P.refinfo[c];
         const int active = active_count<NC>(info);
         const int ntr = active > 1 ? active + 2 : 1;
@@ -290,16 +314,6 @@ __device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, int
         L.ref.smooth = true;
         if (!fin) { st_arr[k * n + slot] = T_PROPERTY_FAILURE; P.fev[k * n + slot] = 0; return false; }
     }
-    if (kResume) {
-        const double* w = stage == 0 ? P.tw : P.fw;
-#pragma unroll
-        for (int i = 0; i < NC; ++i) { L.cand[i] = w[(k * NC + i) * n + slot]; }
-        L.it = st_arr[k * n + slot] >> 16;
-        L.evals = (stage == 0 ? P.tev : P.fev)[k * n + slot];
-        L.evals0 = L.evals;
-        L.mode = TM_START;
-        return true;
-    }
     if (k < gen) {
         trial_start<NC>(L.z, active, k, L.cand);
     } else {
@@ -316,10 +330,8 @@ __device__ bool tpd_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, int
         lost = lost || (L.z[i] > 0 && L.cand[i] == 0);
     }
     L.evals = 0;
-    L.evals0 = 0;
     L.it = 0;
     L.mode = TM_START;
-    if (kF32) { cell_f32<NC>(smix[L.mid], L.t, L.cf); }
     if (lost) {
         st_arr[k * n + slot] = T_UNREPRESENTABLE;
         (stage == 0 ? P.tev : P.fev)[k * n + slot] = 0;
@@ -339,20 +351,6 @@ __device__ void tpd_finish(const Pipe<NC>& P, int stage, TpdLane<NC>& L, int sta
 #pragma unroll
         for (int i = 0; i < NC; ++i) { w[(L.k * NC + i) * n + L.slot] = L.pt.w[i]; }
     }
-    atomicAdd(&P.cnt[0 * n + L.cell], L.evals - L.evals0);
-    atomicAdd(&P.cnt[1 * n + L.cell], L.evals - L.evals0);
-}
-
-// Park an FP32-phase descent at its current point for the FP64 continuation.
-template <int NC>
-__device__ void tpd_park(const Pipe<NC>& P, int stage, TpdLane<NC>& L, const double* w) {
-    const int64_t n = P.n;
-    const int64_t idx = L.k * n + L.slot;
-    (stage == 0 ? P.tst : P.fst)[idx] = kParked | (L.it << 16);
-    (stage == 0 ? P.tev : P.fev)[idx] = L.evals;
-    double* dst = stage == 0 ? P.tw : P.fw;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) { dst[(L.k * NC + i) * n + L.slot] = w[i]; }
     atomicAdd(&P.cnt[0 * n + L.cell], L.evals);
     atomicAdd(&P.cnt[1 * n + L.cell], L.evals);
 }
@@ -384,28 +382,13 @@ __device__ __forceinline__ int tpd_check(TpdLane<NC>& L) {
     return -1;
 }
 
-// Mixed precision: leave the FP32 phase near any termination test of the reference
-// (TPD clearly negative, stationarity within 1e-4, a nonsmooth branch, a long search or
-// a failed FP32 line search) and re-evaluate the current point in FP64, after which the
-// search continues under the FP64 rules above.
-constexpr int kSwitch64 = -2;
-template <int NC>
-__device__ __forceinline__ int tpd_check_f32(TpdLane<NC>& L) {
-    if (L.pt.value < -1e-3 || L.pt.stat <= 1e-4 || !L.pt.smooth || L.it >= 256) { return kSwitch64; }
-    const int r = tpd_check<NC>(L);
-    return r == -1 ? -1 : kSwitch64;
-}
-
-// kF32: the FP32 descent phase of mixed precision (parks every search for kResume);
-// kResume: its FP64 continuation; neither: the FP64 search from the trial start.
-template <int NC, bool kN, bool kF32, bool kResume>
+template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int tk = stage == 0 ? NC + 2 : NC + 4;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
-    unsigned long long* counter =
-        &P.g[stage == 0 ? (kResume ? G_TASK_INIT64 : G_TASK_INIT) : (kResume ? G_TASK_REVIEW64 : G_TASK_REVIEW)];
+    unsigned long long* counter = &P.g[stage == 0 ? G_TASK_INIT : G_TASK_REVIEW];
     TpdLane<NC> L;
     bool active = false, exhausted = false;
     for (;;) {
@@ -413,9 +396,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         const int64_t task = fetch_task(counter, want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else {
-                active = tpd_begin<NC, kF32, kResume>(P, smix, stage, static_cast<int>(task / nslot), task % nslot, L);
-            }
+            else { active = tpd_begin<NC>(P, stage, static_cast<int>(task / nslot), task % nslot, L); }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
@@ -447,29 +428,14 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         TpdPoint<NC> nx;
         int e = ISSUE_NONE;
         bool rejected = false;
-        constexpr bool f32 = kF32;
         if (do_eval) {
             ++L.evals;
-            if (f32) {
-                float lp[NC];
-                int br = 0;
-                e = stability_provider_f32<NC>(smix[L.mid], L.cf, L.p, L.t, L.cand, lp, br);
-                if (e == ISSUE_NONE) { tpd_point_f32<NC>(L.cand, L.z, lp, br, L.ref, nx); }
-            } else {
-                Activity<NC> ph;
-                e = stability_provider<NC, kN>(smix[L.mid], L.p, L.t, L.cand, ph);
-                if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
-            }
+            Activity<NC> ph;
+            e = stability_provider<NC, kN>(smix[L.mid], L.p, L.t, L.cand, ph);
+            if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
             if (e == REJECTED || rejected) {
                 atomicOr(&P.flags[L.cell], CF_REJECTED);
                 tpd_finish<NC>(P, stage, L, T_PROPERTY_FAILURE);
-                active = false;
-                continue;
-            }
-            if (f32 && e != ISSUE_NONE) {
-                // FP32 could not decide this evaluation: the FP64 continuation redoes the
-                // start point, or resumes from the current accepted point
-                tpd_park<NC>(P, stage, L, L.mode == TM_BT ? L.pt.w : L.cand);
                 active = false;
                 continue;
             }
@@ -477,7 +443,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         int done = -1;
         if (L.mode == TM_START) {
             if (e != ISSUE_NONE) { done = T_PROPERTY_FAILURE; }
-            else { L.pt = nx; done = f32 ? tpd_check_f32<NC>(L) : tpd_check<NC>(L); }
+            else { L.pt = nx; done = tpd_check<NC>(L); }
         } else {
             bool ok = false;
             if (do_eval && e == ISSUE_NONE) {
@@ -493,17 +459,12 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
             if (ok) {
                 L.pt = nx;
                 ++L.it;
-                done = f32 ? tpd_check_f32<NC>(L) : tpd_check<NC>(L);
+                done = tpd_check<NC>(L);
             } else {
                 ++L.bt;
                 L.alpha *= 0.5;
-                if (L.bt >= 32) { done = f32 ? kSwitch64 : T_LINE_SEARCH_FAILED; }
+                if (L.bt >= 32) { done = T_LINE_SEARCH_FAILED; }
             }
-        }
-        if (done == kSwitch64) {
-            tpd_park<NC>(P, stage, L, L.pt.w);
-            active = false;
-            continue;
         }
         if (done >= 0) {
             tpd_finish<NC>(P, stage, L, done);
@@ -519,16 +480,18 @@ __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         if (P.flags[c] & CF_REJECTED) { continue; }
+        if (P.rerun && !(P.flags[c] & CF_RERUN)) { continue; }
         const int32_t info = P.refinfo[c];
         if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; continue; }
         const int active = active_count<NC>(info);
         const int ntr = active > 1 ? active + 2 : 1;
         uint64_t evals = 1;
-        bool neg = false, all_stat = true;
+        bool neg = false, all_stat = true, ambiguous = false;
         int nwit = 0;
         int wit[kWitnesses];
         for (int k = 0; k < ntr; ++k) {
             const int st = P.tst[k * n + c] & 0xFF;
+            ambiguous = ambiguous || st == T_AMBIG;
             evals += P.tev[k * n + c];
             if (st == T_NEGATIVE) {
                 neg = true;
@@ -537,7 +500,7 @@ __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
             }
             all_stat = all_stat && st == T_STATIONARY;
         }
-        if (evals > 100000) { atomicOr(&P.flags[c], CF_FALLBACK); }
+        if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); }
         const int status = neg ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
         P.cstat[c] = status;
         if (status == S_UNSTABLE && !(P.flags[c] & CF_FALLBACK)) {
@@ -791,15 +754,18 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
         const int active = active_count<NC>(P.refinfo[c]);
         const int ntr = (active > 1 ? active + 2 : 1) + 2;
         uint64_t evals = 0;
-        bool neg = false, all_stat = true;
+        bool neg = false, all_stat = true, ambiguous = false;
         for (int k = 0; k < ntr; ++k) {
             const int st = P.fst[k * n + u] & 0xFF;
             evals += P.fev[k * n + u];
             neg = neg || st == T_NEGATIVE;
             all_stat = all_stat && st == T_STATIONARY;
+            ambiguous = ambiguous || st == T_AMBIG;
         }
-        if (evals > 100000) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
+        if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
         const int fin = neg ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
+        // mixed precision: a three-phase candidate is searched by the FP64 schedule
+        if (P.mixed && fin == S_UNSTABLE) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
         P.u_v[u] = fin == S_UNSTABLE ? V_UNSTABLE
                  : (P.u_dgbad[u] ? V_INDETERMINATE : (fin == S_NO_INSTABILITY ? V_TWO : V_INDETERMINATE));
     }
@@ -911,6 +877,7 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int32_t flags = P.flags[c];
+        if (P.rerun && !(flags & CF_RERUN)) { continue; }
         if (flags & CF_FALLBACK) {
             const int64_t slot = static_cast<int64_t>(atomicAdd(&P.g[G_NFALLBACK], 1ull));
             P.fb_list[slot] = static_cast<int32_t>(c);
@@ -1032,6 +999,12 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
     }
 }
 
+} // namespace mpmc::dev
+
+#include "mixed_pipeline.hpp"
+
+namespace mpmc::dev {
+
 // ------------------------------------------------------------------ layout
 template <int NC>
 __host__ size_t pipe_carve(Pipe<NC>& P, char* base, int64_t n) {
@@ -1073,6 +1046,9 @@ __host__ size_t pipe_carve(Pipe<NC>& P, char* base, int64_t n) {
     take(P.m_br, 3 * N);
     take(P.g, G_SLOTS);
     take(P.fb_list, N);
+    take(P.ra32, NC * N);
+    take(P.lz32, NC * N);
+    take(P.d32, NC * N);
     return (off + 255) & ~size_t{255};
 }
 
@@ -1101,55 +1077,84 @@ inline unsigned persistent_blocks(K kernel, size_t lds) {
 // Grids of the persistent kernels for one LDS size (the one-mixture size is computed
 // once; a multi-mixture table is sized per call).
 struct PipeGrids {
-    unsigned tpd, tpd32, split, tail, fb;
+    unsigned tpd, split, tail, fb, tpd32, split32;
 };
-template <int NC, bool kN>
+template <int NC>
 PipeGrids pipe_grids(size_t lds) {
     PipeGrids g{};
-    g.tpd = persistent_blocks(k_tpd<NC, kN, false, kN>, lds);
-    g.tpd32 = kN ? persistent_blocks(k_tpd<NC, true, true, false>, lds) : 0u;
-    g.split = persistent_blocks(k_split<NC, kN>, lds);
+    g.tpd = persistent_blocks(k_tpd<NC, false>, lds);
+    g.split = persistent_blocks(k_split<NC, false>, lds);
     // The three-phase tail and the serial fallback carry 8-22 KB of private memory per lane
     // and serve few cells; their grids are capped so that one dispatch reserves at most
     // ~100 MB of scratch (8192 and 1024 lanes), which keeps several launch streams of
     // such dispatches within the runtime's scratch limits.
-    g.tail = std::min(persistent_blocks(k_max3_tail<NC, kN>, lds), 128u);
-    g.fb = std::min(persistent_blocks(k_fallback<NC, kN>, lds), 16u);
+    g.tail = std::min(persistent_blocks(k_max3_tail<NC, false>, lds), 128u);
+    g.fb = std::min(persistent_blocks(k_fallback<NC, false>, lds), 16u);
+    g.tpd32 = persistent_blocks(k_tpd32<NC>, lds);
+    g.split32 = persistent_blocks(k_split32<NC>, lds);
     return g;
 }
 
-template <int NC, bool kN>
-hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, hipStream_t s) {
-    const size_t lds = static_cast<size_t>(P.nmix > 1 ? P.nmix : 1) * sizeof(Mix<NC>);
-    static const PipeGrids one = pipe_grids<NC, kN>(sizeof(Mix<NC>));
-    const PipeGrids b = P.nmix > 1 ? pipe_grids<NC, kN>(lds) : one;
+// The FP64 schedule over every cell (P.rerun == 0) or over the cells flagged CF_RERUN.
+template <int NC>
+void launch_fp64(const Pipe<NC>& P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds,
+                 hipStream_t s) {
     const int64_t n = P.n;
     const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
     // a multi-mixture table is staged in LDS by every workgroup: fewer, longer-lived
     // workgroups for the cell-parallel preparation
     const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
-    hipLaunchKernelGGL((k_prep<NC, kN>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
-    if constexpr (kN) {
-        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b.tpd32), dim3(64), lds, s, P, mixc, 0);
-    }
-    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0);
+    hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0);
     hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_split<NC, kN>), dim3(b.split), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL((k_split<NC, false>), dim3(b.split), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL(k_split_select<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    if constexpr (kN) {
-        hipLaunchKernelGGL((k_tpd<NC, true, true, false>), dim3(b.tpd32), dim3(64), lds, s, P, mixc, 1);
-    }
-    hipLaunchKernelGGL((k_tpd<NC, kN, false, kN>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1);
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, kN>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
+    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, false>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
     hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_fallback<NC, kN>), dim3(b.fb), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
+}
+
+// Mixed precision: the reduced-precision pass over every cell, then the FP64 schedule over
+// the cells it could not settle.
+template <int NC>
+void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds, hipStream_t s) {
+    const int64_t n = P.n;
+    const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
+    const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
+    P.mixed = 1;
+    P.rerun = 0;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
+    hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 0);
+    hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_split32<NC>, dim3(b.split32), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL(k_polish<NC>, dim3(cell_blocks), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 1);
+    hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_rerun_mark<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    P.mixed = 0;
+    P.rerun = 1;
+    launch_fp64<NC>(P, mixc, max_phases, b, lds, s);
+}
+
+template <int NC>
+hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, int precision,
+                                hipStream_t s) {
+    const size_t lds = static_cast<size_t>(P.nmix > 1 ? P.nmix : 1) * sizeof(Mix<NC>);
+    static const PipeGrids one = pipe_grids<NC>(sizeof(Mix<NC>));
+    const PipeGrids b = P.nmix > 1 ? pipe_grids<NC>(lds) : one;
+    if (precision == 1) { launch_mixed<NC>(P, mixc, max_phases, b, lds, s); }
+    else { launch_fp64<NC>(P, mixc, max_phases, b, lds, s); }
     return hipGetLastError();
 }
 
 // precision 0: FP64 evaluations with the reference's bisection roots (bit-level
-// parity); 1: mixed precision (FP32 descent phase, Newton roots, FP64 decisions).
+// parity); 1: mixed precision (reduced-precision searches, FP64 roots, decisions and
+// polish, FP64 schedule for every cell the searches cannot settle).
 //
 // mixc: nmix consecutive Mix<NC> records; mix_of[n] (device) names each cell's record,
 // nullptr when nmix == 1.
@@ -1170,8 +1175,9 @@ hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, 
     P.t = t;
     P.zin = z;
     P.out = out;
-    return precision == 1 ? launch_pipeline_prec<NC, true>(P, mixc, max_phases, s)
-                          : launch_pipeline_prec<NC, false>(P, mixc, max_phases, s);
+    P.mixed = 0;
+    P.rerun = 0;
+    return launch_pipeline_prec<NC>(P, mixc, max_phases, precision, s);
 }
 
 #define MPMC_DECLARE_PIPELINE(N)                                                                                  \

@@ -397,122 +397,3 @@ MPMC_HOT int side_provider(const Mix<NC>& m, double p, double t, const double* w
 }
 
 } // namespace mpmc::dev
-
-namespace mpmc::dev {
-
-// FP32 evaluation for the mixed-precision TPD descent: the stability provider's
-// least-Gibbs admissible branch with mixing and ln(phi) in FP32, while A, B, the cubic's
-// brackets, the near-multiple test and root selection stay FP64 (Newton roots). Used
-// only away from a search's termination tests; those are always re-done in FP64.
-// Returns ISSUE_NONE, REJECTED, or any other value to request an FP64 evaluation.
-template <int NC>
-struct CellF32 {
-    float a[NC], ra[NC];  // a_i(T) and sqrt(a_i(T)), prepared in FP64 then rounded
-};
-
-template <int NC>
-__device__ __forceinline__ void cell_f32(const Mix<NC>& m, double t, CellF32<NC>& cf) {
-    const double st = __builtin_sqrt(t);
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        const double rr = st / m.sqrt_tc[i];
-        const double br = 1.0 + m.kappa[i] * (1.0 - rr);
-        const double a = (br * m.ac[i]) * br;
-        cf.a[i] = static_cast<float>(a);
-        cf.ra[i] = static_cast<float>(__builtin_sqrt(a));
-    }
-}
-
-__device__ __forceinline__ float log1p_over_x_f32(float u) {
-    if (__builtin_fabsf(u) <= 0.125f) {
-        float v = 1.0f / 11.0f;  // degree 24/3 + 2 = 10
-#pragma unroll
-        for (int k = 9; k >= 0; --k) { v = (k % 2 == 0 ? 1.0f : -1.0f) / static_cast<float>(k + 1) + u * v; }
-        return v;
-    }
-    return log1pf(u) / u;
-}
-
-template <int NC>
-MPMC_HOT int stability_provider_f32(const Mix<NC>& m, const CellF32<NC>& cf, double p, double t, const double* w,
-                                    float* lnphi_out, int& branch) {
-    if (check_fractions<NC>(w) != ISSUE_NONE) { return REJECTED; }
-    float wf[NC], S[NC];
-    float att = 0, cov = 0;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        wf[i] = static_cast<float>(w[i]);
-        att += (wf[i] * cf.a[i]) * wf[i];
-        cov += wf[i] * static_cast<float>(m.b[i]);
-        S[i] = wf[i] * cf.a[i];
-    }
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-#pragma unroll
-        for (int j = i + 1; j < NC; ++j) {
-            const float f = static_cast<float>(m.factor[i * NC + j]);
-            const float aij = (cf.ra[i] * f) * cf.ra[j];
-            att += 2.0f * (wf[i] * wf[j]) * aij;
-            S[i] += wf[j] * aij;
-            S[j] += wf[i] * aij;
-        }
-    }
-    const double rt = m.r_gas * t;
-    const double prt = p / rt;
-    const double A = (static_cast<double>(att) / rt) * prt;
-    const double B = static_cast<double>(cov) * prt;
-    if (!finite(A) || !finite(B) || !(B > 0)) { return ROOT_RANGE; }
-    const RootSet rs = pr76_roots<true>(A, B);
-    if (rs.status != RS_SUCCESS) { return ROOT_RANGE; }
-    const float Bf = static_cast<float>(B), Af = static_cast<float>(A);
-    const float rtf = static_cast<float>(rt), prtf = static_cast<float>(prt);
-    float cand[2][NC];
-    int cb[2];
-    int nc = 0;
-    for (int k = 0; k < rs.count && nc < 2; ++k) {
-        if (rs.r[k].slope_sign <= 0) { continue; }
-        if (!rs.r[k].dvalid) { return ILL_CONDITIONED; }
-        const float fv = static_cast<float>(rs.r[k].x);
-        const float z = static_cast<float>(rs.r[k].z);
-        const float den = fv + (2.0f - 1.41421356f) * Bf;
-        const float u = (2.0f * 1.41421356f) * (Bf / den);
-        const float lf = log1p_over_x_f32(u) / den;
-        float zm1 = z - 1.0f;
-        float lfree = logf(fv);
-        if (__builtin_fabsf(zm1) < 0.25f && Bf < 0.125f) {
-            const float bz = Bf / z;
-            zm1 = Bf / fv - (Af / z) / (1.0f + 2.0f * bz - bz * bz);
-            lfree = log1pf(zm1 - Bf);
-        }
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            const float ratio = static_cast<float>(m.b[i]) / cov;
-            const float coef = ((2.0f * S[i] - att * ratio) / rtf) * prtf;
-            cand[nc][i] = ratio * zm1 - lfree - coef * lf;
-        }
-        cb[nc++] = k;
-    }
-    if (nc == 0) { return NO_ADMISSIBLE; }
-    int best = 0;
-    if (nc == 2) {
-        float d = 0, mag = 1;
-#pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            d += wf[i] * (cand[1][i] - cand[0][i]);
-            mag += wf[i] * (__builtin_fabsf(cand[1][i]) + __builtin_fabsf(cand[0][i]));
-        }
-        if (__builtin_fabsf(d) <= 1e-4f * mag) { return NONFINITE; } // near tie: decide in FP64
-        best = d < 0 ? 1 : 0;
-    }
-    bool fin = true;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-        lnphi_out[i] = cand[best][i];
-        fin = fin && __builtin_isfinite(lnphi_out[i]);
-    }
-    if (!fin) { return NONFINITE; }
-    branch = cb[best];
-    return ISSUE_NONE;
-}
-
-} // namespace mpmc::dev
