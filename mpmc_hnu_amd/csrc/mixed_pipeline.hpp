@@ -94,7 +94,8 @@ __device__ __forceinline__ bool tpd32_point(const Tpd32Lane<NC>& L, const float*
         st = fmaxf(st, __builtin_fabsf(res[i]));
     }
     value = val;
-    guard = 64.0f * kEps32 * mag;
+    // each q_i carries a few FP32 roundings of terms bounded by the magnitude sum
+    guard = 8.0f * kEps32 * mag;
     stat = st;
     return ok && __builtin_isfinite(val) && __builtin_isfinite(st) && __builtin_isfinite(mag);
 }
@@ -335,11 +336,12 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd32(Pipe<NC> P, const
             if (ok_eval) {
                 ok = nval < -(static_cast<float>(L.tol) + 4.0f * ngd + 1e-6f);
                 if (!ok) {
-                    // stability_accept_step (pt_stability.hpp:312-326) at FP32 resolution
+                    // stability_accept_step (pt_stability.hpp:312-326) at FP32 resolution: the
+                    // Armijo decrease is accepted at any step size (the FP64 rule's guard is
+                    // below every step the search takes), residual progress within the guard
                     const float pred = -L.alpha * L.slope;
                     const float gd = L.guard + ngd;
-                    if (pred <= gd) { ok = nval <= L.value + gd && nst < L.stat; }
-                    else { ok = nval <= L.value - 1e-4f * pred; }
+                    ok = nval <= L.value - 1e-4f * pred || (pred <= gd && nval <= L.value + gd && nst < L.stat);
                 }
             }
             if (ok) {
@@ -554,9 +556,11 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split32(Pipe<NC> P, con
         } else if (e == -1) {
             const float pred = -L.alpha * L.slope;
             const float gd = L.guard + ngd;
-            // split_accept_step (pt_split.hpp:232-244) at FP32 resolution
-            if (pred > gd) { accept = ng <= L.gibbs - 1e-4f * pred; }
-            else { accept = ng <= L.gibbs + gd && (nn <= kNorm32 || L.norm - nn > 1e-4f * (L.alpha / L.scale) * L.norm); }
+            // split_accept_step (pt_split.hpp:232-244) at FP32 resolution, the Armijo decrease
+            // accepted at any step size
+            accept = ng <= L.gibbs - 1e-4f * pred ||
+                     (pred <= gd && ng <= L.gibbs + gd &&
+                      (nn <= kNorm32 || L.norm - nn > 1e-4f * (L.alpha / L.scale) * L.norm));
         }
         if (accept) {
 #pragma unroll
