@@ -55,16 +55,21 @@ namespace mpmc::dev {
 constexpr int kAttempts = 16;   // PtSplitOptions::max_split_attempts (pt_split.hpp:49)
 constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
-// Outcomes of a reduced-precision trial or split attempt that FP64 must decide.
+// Outcomes of a reduced-precision trial or split attempt that FP64 must decide: an
+// ambiguous end, or a search handed to the FP64 kernels to be continued from its current
+// point (iteration count in bits 16..).
 constexpr int32_t T_AMBIG = 0x7D;
+constexpr int32_t T_RESUME = 0x7C;
+constexpr int32_t T_REDO = 0x7B;      // review trial to be searched again in FP64 from its start
 constexpr int A_AMBIG = 0x7D;
+constexpr int A_RESUME = 0x7C;
 
-// CF_FALLBACK: the cell is finished by another path (the serial FP64 path, or in mixed
-// precision the FP64 schedule); CF_RERUN: the cell belongs to that FP64 re-run.
-enum CellFlag : int32_t { CF_REJECTED = 1, CF_FALLBACK = 2, CF_RERUN = 4 };
+// CF_FALLBACK: the cell is finished by the serial FP64 path.
+enum CellFlag : int32_t { CF_REJECTED = 1, CF_FALLBACK = 2 };
 enum GlobalSlot : int { G_NU = 0, G_NFALLBACK = 1, G_TASK_INIT = 2, G_TASK_SPLIT = 3, G_TASK_REVIEW = 4,
                         G_TASK_MAX3 = 5, G_TASK_FALLBACK = 6, G_TASK_INIT32 = 7, G_TASK_REVIEW32 = 8,
-                        G_TASK_SPLIT32 = 9, G_SLOTS = 16 };
+                        G_TASK_SPLIT32 = 9, G_TASK_INIT_RESUME = 10, G_TASK_REVIEW_RESUME = 11,
+                        G_TASK_SPLIT_RESUME = 12, G_TASK_REVIEW_REDO = 13, G_SLOTS = 16 };
 
 // Device workspace carved from one allocation (pipe_carve). n = cells; arrays
 // indexed by the compact unstable index u use capacity n.
@@ -115,10 +120,7 @@ struct Pipe {
     // table of nmix mixtures (all with NC components); mix_of == nullptr: one mixture.
     const int32_t* mix_of;
     int nmix;
-    // mixed: the reduced-precision pass of precision 1; rerun: the FP64 schedule restricted
-    // to the cells flagged CF_RERUN
-    int mixed;
-    int rerun;
+    int mixed;          // mixed-precision schedule (precision 1): 1, then 2 after the review redo
 };
 
 // ------------------------------------------------------------------ helpers
@@ -187,7 +189,6 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
     const int64_t n = P.n;
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        if (P.rerun && !(P.flags[c] & CF_RERUN)) { continue; }
         double z[NC];
 #pragma unroll
         for (int i = 0; i < NC; ++i) { z[i] = P.zin[i * n + c]; }
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
             P.refl[i * n + c] = (info & 0x200) ? ref.lnphi[i] : 0.0;
         }
         P.refinfo[c] = info;
-        P.flags[c] = (rejected ? CF_REJECTED : 0) | (P.rerun ? CF_RERUN : 0);
+        P.flags[c] = rejected ? CF_REJECTED : 0;
         if (P.mixed) {
             const Mix<NC>& m = smix[mix_at(P, c)];
             const double st = __builtin_sqrt(t);
@@ -259,18 +260,24 @@ struct TpdLane {
     TpdPoint<NC> pt;
     double dir[NC], lw[NC], cand[NC];
     double slope, scale, alpha;
+    uint32_t evals0;   // evaluations counted before an FP64 continuation
 };
 
 // stage 0: initial feed test; stage 1: common-tangent review of the selected pair.
+// mode 1: FP64 continuation of the reduced-precision descents marked T_RESUME, from their
+// stored point and iteration count; mode 2: the trials marked T_REDO, from their start
+// (mixed precision).
 template <int NC>
-__device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, TpdLane<NC>& L) {
+__device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int mode, TpdLane<NC>& L) {
     const int64_t n = P.n;
     int64_t c;
     int32_t* st_arr = stage == 0 ? P.tst : P.fst;
+    if (mode == 1 && (st_arr[k * n + slot] & 0xFF) != T_RESUME) { return false; }
+    if (mode == 2 && (st_arr[k * n + slot] & 0xFF) != T_REDO) { return false; }
+    const bool resume = mode == 1;
     if (stage == 0) {
         c = slot;
         if (P.flags[c] & CF_REJECTED) { return false; }
-        if (P.rerun && !(P.flags[c] & CF_RERUN)) { return false; }
         const int32_t info = P.refinfo[c];
         const int active = active_count<NC>(info);
         const int ntr = active > 1 ? active + 2 : 1;
@@ -314,6 +321,16 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, Tpd
         L.ref.smooth = true;
         if (!fin) { st_arr[k * n + slot] = T_PROPERTY_FAILURE; P.fev[k * n + slot] = 0; return false; }
     }
+    if (resume) {
+        const double* w = stage == 0 ? P.tw : P.fw;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { L.cand[i] = w[(k * NC + i) * n + slot]; }
+        L.it = st_arr[k * n + slot] >> 16;
+        L.evals = (stage == 0 ? P.tev : P.fev)[k * n + slot];
+        L.evals0 = L.evals;
+        L.mode = TM_START;
+        return true;
+    }
     if (k < gen) {
         trial_start<NC>(L.z, active, k, L.cand);
     } else {
@@ -330,6 +347,7 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, Tpd
         lost = lost || (L.z[i] > 0 && L.cand[i] == 0);
     }
     L.evals = 0;
+    L.evals0 = 0;
     L.it = 0;
     L.mode = TM_START;
     if (lost) {
@@ -351,8 +369,8 @@ __device__ void tpd_finish(const Pipe<NC>& P, int stage, TpdLane<NC>& L, int sta
 #pragma unroll
         for (int i = 0; i < NC; ++i) { w[(L.k * NC + i) * n + L.slot] = L.pt.w[i]; }
     }
-    atomicAdd(&P.cnt[0 * n + L.cell], L.evals);
-    atomicAdd(&P.cnt[1 * n + L.cell], L.evals);
+    atomicAdd(&P.cnt[0 * n + L.cell], L.evals - L.evals0);
+    atomicAdd(&P.cnt[1 * n + L.cell], L.evals - L.evals0);
 }
 
 // Termination tests and the next log-space descent direction (pt_stability.hpp:528-561).
@@ -383,12 +401,15 @@ __device__ __forceinline__ int tpd_check(TpdLane<NC>& L) {
 }
 
 template <int NC, bool kN>
-__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
+__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage,
+                                                              int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int tk = stage == 0 ? NC + 2 : NC + 4;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
-    unsigned long long* counter = &P.g[stage == 0 ? G_TASK_INIT : G_TASK_REVIEW];
+    unsigned long long* counter =
+        &P.g[stage == 0 ? (resume ? G_TASK_INIT_RESUME : G_TASK_INIT)
+                        : (resume == 1 ? G_TASK_REVIEW_RESUME : (resume == 2 ? G_TASK_REVIEW_REDO : G_TASK_REVIEW))];
     TpdLane<NC> L;
     bool active = false, exhausted = false;
     for (;;) {
@@ -396,7 +417,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         const int64_t task = fetch_task(counter, want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else { active = tpd_begin<NC>(P, stage, static_cast<int>(task / nslot), task % nslot, L); }
+            else { active = tpd_begin<NC>(P, stage, static_cast<int>(task / nslot), task % nslot, resume, L); }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
@@ -480,7 +501,6 @@ __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         if (P.flags[c] & CF_REJECTED) { continue; }
-        if (P.rerun && !(P.flags[c] & CF_RERUN)) { continue; }
         const int32_t info = P.refinfo[c];
         if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; continue; }
         const int active = active_count<NC>(info);
@@ -527,10 +547,18 @@ struct SplitLane {
     Counters cnt;
 };
 
+// resume: FP64 continuation of the reduced-precision attempts marked A_RESUME, from their
+// stored logK and iteration count (mixed precision).
 template <int NC>
-__device__ bool split_begin(const Pipe<NC>& P, int j, int64_t u, SplitLane<NC>& L) {
+__device__ bool split_begin(const Pipe<NC>& P, int j, int64_t u, bool resume, SplitLane<NC>& L) {
     const int64_t n = P.n;
-    if (j >= 2 * P.u_nwit[u]) { P.a_st[j * n + u] = -1; P.a_ev[j * n + u] = 0; return false; }
+    if (resume) {
+        if (j >= 2 * P.u_nwit[u] || (P.a_st[j * n + u] & 0xFF) != A_RESUME) { return false; }
+    } else if (j >= 2 * P.u_nwit[u]) {
+        P.a_st[j * n + u] = -1;
+        P.a_ev[j * n + u] = 0;
+        return false;
+    }
     const int64_t c = P.u_cell[u];
     L.u = u;
     L.cell = c;
@@ -539,6 +567,15 @@ __device__ bool split_begin(const Pipe<NC>& P, int j, int64_t u, SplitLane<NC>& 
     L.t = P.t[c];
 #pragma unroll
     for (int i = 0; i < NC; ++i) { L.z[i] = P.feed[i * n + c]; }
+    if (resume) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { L.lk0[i] = P.a_v[((j * 5 + 4) * NC + i) * n + u]; }
+        L.evals = 0;
+        L.it = P.a_st[j * n + u] >> 16;
+        L.mode = TM_START;
+        L.cnt = Counters{0, 0, 0, 0};
+        return true;
+    }
     const int k = P.u_wit[(j / 2) * n + u];
     double w[NC];
 #pragma unroll
@@ -622,7 +659,7 @@ __device__ __forceinline__ int split_check(SplitLane<NC>& L, double fug_tol) {
 }
 
 template <int NC, bool kN>
-__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc) {
+__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc, int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = kAttempts * nu;
@@ -630,10 +667,10 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const
     bool active = false, exhausted = false;
     for (;;) {
         const bool want = !active && !exhausted;
-        const int64_t task = fetch_task(&P.g[G_TASK_SPLIT], want);
+        const int64_t task = fetch_task(&P.g[resume ? G_TASK_SPLIT_RESUME : G_TASK_SPLIT], want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else { active = split_begin<NC>(P, static_cast<int>(task / nu), task % nu, L); }
+            else { active = split_begin<NC>(P, static_cast<int>(task / nu), task % nu, resume != 0, L); }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
@@ -764,8 +801,12 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
         }
         if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
         const int fin = neg ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
-        // mixed precision: a three-phase candidate is searched by the FP64 schedule
-        if (P.mixed && fin == S_UNSTABLE) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
+        if (P.mixed == 1 && fin == S_UNSTABLE) {
+            // mixed precision: a three-phase candidate's review is searched again in FP64 so
+            // that the three-phase tail starts from the reference's witnesses
+            for (int k = 0; k < ntr; ++k) { P.fst[k * n + u] = T_REDO; }
+            continue;
+        }
         P.u_v[u] = fin == S_UNSTABLE ? V_UNSTABLE
                  : (P.u_dgbad[u] ? V_INDETERMINATE : (fin == S_NO_INSTABILITY ? V_TWO : V_INDETERMINATE));
     }
@@ -877,7 +918,6 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int32_t flags = P.flags[c];
-        if (P.rerun && !(flags & CF_RERUN)) { continue; }
         if (flags & CF_FALLBACK) {
             const int64_t slot = static_cast<int64_t>(atomicAdd(&P.g[G_NFALLBACK], 1ull));
             P.fb_list[slot] = static_cast<int32_t>(c);
@@ -1095,7 +1135,7 @@ PipeGrids pipe_grids(size_t lds) {
     return g;
 }
 
-// The FP64 schedule over every cell (P.rerun == 0) or over the cells flagged CF_RERUN.
+// The FP64 schedule.
 template <int NC>
 void launch_fp64(const Pipe<NC>& P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds,
                  hipStream_t s) {
@@ -1106,39 +1146,43 @@ void launch_fp64(const Pipe<NC>& P, const double* mixc, int max_phases, const Pi
     // workgroups for the cell-parallel preparation
     const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
     hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
-    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0, 0);
     hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_split<NC, false>), dim3(b.split), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL((k_split<NC, false>), dim3(b.split), dim3(64), lds, s, P, mixc, 0);
     hipLaunchKernelGGL(k_split_select<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 0);
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, false>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
     hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
 }
 
-// Mixed precision: the reduced-precision pass over every cell, then the FP64 schedule over
-// the cells it could not settle.
+// Mixed precision: reduced-precision searches, FP64 continuation of the searches they
+// leave ambiguous, FP64 polish of the selected split, FP64 three-phase tail.
 template <int NC>
 void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds, hipStream_t s) {
     const int64_t n = P.n;
     const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
     const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
     P.mixed = 1;
-    P.rerun = 0;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
     hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 0);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0, 1);
     hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     hipLaunchKernelGGL(k_split32<NC>, dim3(b.split32), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL(k_split_gate<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_split<NC, false>), dim3(b.split), dim3(64), lds, s, P, mixc, 1);
     hipLaunchKernelGGL(k_polish<NC>, dim3(cell_blocks), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 1);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 1);
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 2);
+    P.mixed = 2;
+    hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, false>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
     hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    hipLaunchKernelGGL(k_rerun_mark<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
-    P.mixed = 0;
-    P.rerun = 1;
-    launch_fp64<NC>(P, mixc, max_phases, b, lds, s);
+    hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
 }
 
 template <int NC>
@@ -1176,7 +1220,6 @@ hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, 
     P.zin = z;
     P.out = out;
     P.mixed = 0;
-    P.rerun = 0;
     return launch_pipeline_prec<NC>(P, mixc, max_phases, precision, s);
 }
 

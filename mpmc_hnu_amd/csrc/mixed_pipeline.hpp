@@ -7,22 +7,25 @@
 //
 //   k_prep            FP64, as in the FP64 schedule (reference phase of the feed)
 //   k_tpd32(initial)  per (trial, cell): FP32 descent, classified with margins
-//   k_init_reduce     shared: witnesses in trial order; an ambiguous trial sends the cell on
+//   k_tpd(resume)     FP64 continuation of every trial the FP32 descent left ambiguous,
+//                     from its end point and iteration count, under the reference's rules
+//   k_init_reduce     shared: witnesses in trial order
 //   k_split32         per (attempt, unstable cell): FP32 successive substitution
-//   k_polish          per unstable cell: the lowest-Gibbs converged attempt (ambiguous
-//                     Gibbs ties send the cell on) polished by FP64 Newton on logK with
-//                     the Dual Jacobian, the reference's acceptance tests in FP64, the
-//                     feed-Gibbs comparison and the review tolerance
+//   k_split_gate      per unstable cell: when no attempt converged in FP32, its ambiguous
+//                     attempts are handed to the FP64 continuation
+//   k_split(resume)   FP64 continuation of those attempts from their logK
+//   k_polish          per unstable cell: converged attempts in Gibbs order, each polished
+//                     by FP64 Newton on logK with the Dual Jacobian until one passes the
+//                     reference's acceptance tests in FP64; then the feed-Gibbs comparison
+//                     and the review tolerance
 //   k_tpd32(review)   per (trial, unstable cell): FP32 descent against the common tangent
-//   k_review_reduce   shared; a cell whose review finds instability (a three-phase
-//                     candidate) is sent on
-//   k_publish         shared
-//   k_rerun_mark      cells sent on are re-flashed by the whole FP64 schedule, restricted
-//                     to them (Pipe::rerun), then published by it.
+//   k_tpd(resume)     FP64 continuation of the ambiguous review trials
+//   k_review_reduce, k_max3_tail (FP64), k_publish, k_fallback: shared with the FP64 schedule
 //
 // A trial or attempt is "ambiguous" whenever its outcome is within reduced-precision
-// reach of one of the reference's thresholds. So every published decision is either
-// one FP32 cannot get wrong by rounding or one the FP64 schedule took.
+// reach of one of the reference's thresholds, or its search is slow enough to meet the
+// reference's iteration limit; its decision is then taken by the FP64 continuation.
+// The published phase set always comes from FP64 values.
 #pragma once
 
 namespace mpmc::dev {
@@ -62,6 +65,7 @@ struct Tpd32Lane {
     float dir[NC], cand[NC], candlw[NC];  // candidate w and ln w (w may underflow, ln w is kept)
     float slope, scale, alpha;
     uint32_t zmask;
+    bool has_pt;        // an accepted point exists (else the candidate is the start point)
 };
 
 // Tangent-plane distance of the candidate in FP32; false when it cannot be formed.
@@ -212,6 +216,7 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
     L.yprev[0] = -1.0;
     L.yprev[1] = -1.0;
     L.branch = 0;
+    L.has_pt = false;
     L.evals = 0;
     L.it = 0;
     L.mode = TM_START;
@@ -223,16 +228,19 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
     return true;
 }
 
+// An ambiguous trial is handed to the FP64 continuation (T_RESUME) from its current point.
 template <int NC>
 __device__ void tpd32_finish(const Pipe<NC>& P, int stage, const Tpd32Lane<NC>& L, int status) {
     const int64_t n = P.n;
     const int64_t idx = L.k * n + L.slot;
-    (stage == 0 ? P.tst : P.fst)[idx] = status | (L.branch << 8);
+    const bool resume = status == T_AMBIG;
+    (stage == 0 ? P.tst : P.fst)[idx] = resume ? (T_RESUME | (L.it << 16)) : (status | (L.branch << 8));
     (stage == 0 ? P.tev : P.fev)[idx] = L.evals;
-    if (status == T_NEGATIVE) {
+    if (status == T_NEGATIVE || resume) {
+        const float* lw = L.has_pt ? L.lw : L.candlw;
         double w[NC];
 #pragma unroll
-        for (int i = 0; i < NC; ++i) { w[i] = (L.zmask >> i) & 1u ? exp(static_cast<double>(L.lw[i])) : 0.0; }
+        for (int i = 0; i < NC; ++i) { w[i] = (L.zmask >> i) & 1u ? exp(static_cast<double>(lw[i])) : 0.0; }
         const double s = ksum<NC>(w);
         double* dst = stage == 0 ? P.tw : P.fw;
 #pragma unroll
@@ -329,6 +337,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd32(Pipe<NC> P, const
 #pragma unroll
                 for (int i = 0; i < NC; ++i) { L.w[i] = L.cand[i]; L.lw[i] = L.candlw[i]; L.res[i] = nres[i]; }
                 L.value = nval; L.guard = ngd; L.stat = nst; L.branch = br;
+                L.has_pt = true;
                 done = tpd32_check<NC>(P, stage, L);
             }
         } else {
@@ -501,12 +510,23 @@ __device__ bool split32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int j, int
     return true;
 }
 
+// Converged attempts are marked kReduced (their logK is in the fifth vector slot);
+// ambiguous ones keep their current logK and iteration count for a possible FP64
+// continuation.
+constexpr int32_t kReduced = 1 << 24;
 template <int NC>
 __device__ void split32_finish(const Pipe<NC>& P, const Split32Lane<NC>& L, int status) {
     const int64_t n = P.n;
     const int64_t idx = L.j * n + L.u;
     P.a_ev[idx] = L.evals;
-    P.a_st[idx] = status;
+    const bool keep = status == A_AMBIG || status == A_INDISTINGUISHABLE;
+    P.a_st[idx] = status == A_CONVERGED ? (status | kReduced) : (keep ? (status | (L.it << 16)) : status);
+    if (keep) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            P.a_v[((L.j * 5 + 4) * NC + i) * n + L.u] = L.mode == TM_START ? L.cand[i] : L.lk[i];
+        }
+    }
     if (status == A_CONVERGED) {
         const double s[6] = {L.gibbs, L.beta, L.zl, L.zv, L.norm, L.guard};
 #pragma unroll
@@ -622,10 +642,79 @@ __device__ __noinline__ bool split_jacobian(const Mix<NC>& m, double p, double t
     return true;
 }
 
-// Per unstable cell: choose the lowest-Gibbs converged FP32 attempt, polish it by FP64
-// Newton on logK, apply the reference's acceptance tests (pt_split.hpp:349-361) and the
-// selection epilogue of k_split_select (pt_split.hpp:459-507). Any outcome the reduced
-// precision cannot settle sends the cell to the FP64 schedule.
+// Per unstable cell: when no attempt converged in reduced precision, its ambiguous and
+// indistinguishable attempts are continued in FP64 from their current logK (an FP32
+// "indistinguishable" end can be a near-critical split the reference accepts).
+template <int NC>
+__global__ __launch_bounds__(64) void k_split_gate(Pipe<NC> P) {
+    const int64_t n = P.n;
+    const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
+    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < nu;
+         u += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int na = 2 * P.u_nwit[u];
+        bool conv = false;
+        for (int j = 0; j < na; ++j) { conv = conv || (P.a_st[j * n + u] & 0xFF) == A_CONVERGED; }
+        if (conv) { continue; }
+        for (int j = 0; j < na; ++j) {
+            const int32_t st = P.a_st[j * n + u];
+            if ((st & 0xFF) == A_AMBIG || (st & 0xFF) == A_INDISTINGUISHABLE) { P.a_st[j * n + u] = A_RESUME | (st & ~0xFF); }
+        }
+    }
+}
+
+// logK of a converged attempt: its reduced-precision iterate, or ln y - ln x of an FP64 one.
+template <int NC>
+__device__ __forceinline__ void attempt_logk(const Pipe<NC>& P, int j, int64_t u, double* lk) {
+    const int64_t n = P.n;
+    const bool reduced = (P.a_st[j * n + u] & kReduced) != 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        lk[i] = reduced ? P.a_v[((j * 5 + 4) * NC + i) * n + u]
+                        : log(P.a_v[((j * 5 + 1) * NC + i) * n + u]) - log(P.a_v[((j * 5 + 0) * NC + i) * n + u]);
+    }
+}
+
+// Newton on logK with step halving, from lk: converged at 1e-2 fug_tol, or within fug_tol
+// once a step no longer reduces the residual by 4x (the FP64 rounding floor).
+template <int NC>
+__device__ __noinline__ bool polish_split(const Mix<NC>& m, double p, double t, const double* z, double* lk,
+                                          double beta0, double fug_tol, SplitState<NC>& s, Counters& cnt,
+                                          bool& rejected) {
+    uint32_t evals = 0;
+    SplitState<NC> trial;
+    if (split_evaluate<NC, false>(m, p, t, z, lk, evals, 0xFFFFFFFFu, s, cnt, rejected, beta0) != -1 || rejected) {
+        return false;
+    }
+    for (int it = 0; it < 12; ++it) {
+        if (s.norm <= 1e-2 * fug_tol) { return true; }
+        double Am[NC * NC], dl[NC];
+        int piv[NC];
+        double nrm;
+        if (!split_jacobian<NC>(m, p, t, z, s, Am) || !lu_factor<NC>(Am, piv, nrm)) { return false; }
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { dl[i] = -s.r[i]; }
+        if (!lu_solve<NC>(Am, piv, dl)) { return false; }
+        double step = 1.0;
+        bool took = false;
+        for (int bt = 0; bt < 5 && !took; ++bt, step *= 0.5) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) { lk[i] = s.lk[i] + step * dl[i]; }
+            const int e = split_evaluate<NC, false>(m, p, t, z, lk, evals, 0xFFFFFFFFu, trial, cnt, rejected, s.fr.beta);
+            if (rejected) { return false; }
+            took = e == -1 && trial.norm < s.norm;
+        }
+        if (!took) { return s.norm <= fug_tol; }
+        const bool stalled = trial.norm > 0.25 * s.norm;
+        s = trial;
+        if (stalled && s.norm <= fug_tol) { return true; }
+    }
+    return false;
+}
+
+// Per unstable cell: the converged attempts in Gibbs order (at most three tried), each
+// polished by FP64 Newton on logK until one passes the reference's acceptance tests
+// (pt_split.hpp:349-361) in FP64; then the selection epilogue of k_split_select
+// (pt_split.hpp:459-507). No converged attempt: the reference's indeterminate outcome.
 template <int NC>
 __global__ __launch_bounds__(64) void k_polish(Pipe<NC> P, const double* __restrict__ mixc) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
@@ -639,109 +728,68 @@ __global__ __launch_bounds__(64) void k_polish(Pipe<NC> P, const double* __restr
         P.u_ftol[u] = __builtin_nan("");
         P.u_dgbad[u] = 0;
         if (P.flags[c] & CF_REJECTED) { continue; }
-        auto send_on = [&]() { atomicOr(&P.flags[c], CF_FALLBACK); };
         const int na = 2 * P.u_nwit[u];
-        int best = -1;
-        double best_g = 0;
-        bool ambiguous = false;
-        for (int j = 0; j < na; ++j) {
-            const int st = P.a_st[j * n + u] & 0xFF;
-            if (st == A_AMBIG) { ambiguous = true; }
-            if (st == A_CONVERGED) {
-                const double g = P.a_s[(j * 6 + 0) * n + u];
-                if (best < 0 || g < best_g) { best = j; best_g = g; }
-            }
-        }
-        // Ambiguous attempts are left out of the selection when another attempt converged
-        // (attempts of one cell that converge reach the same split in all but rare cells).
-        if (best < 0) { send_on(); continue; }
-        ambiguous = false;
-        // a distinct converged solution within rounding reach of the lowest Gibbs energy
-        const double tie = 1e-5 * (1.0 + __builtin_fabs(best_g)) + 4.0 * P.a_s[(best * 6 + 5) * n + u];
-        for (int j = 0; j < na && !ambiguous; ++j) {
-            if (j == best || (P.a_st[j * n + u] & 0xFF) != A_CONVERGED) { continue; }
-            if (P.a_s[(j * 6 + 0) * n + u] - best_g > tie) { continue; }
-            double gap = 0;
-            for (int i = 0; i < NC; ++i) {
-                gap = smax(gap, __builtin_fabs(P.a_v[((j * 5 + 4) * NC + i) * n + u] -
-                                               P.a_v[((best * 5 + 4) * NC + i) * n + u]));
-            }
-            ambiguous = gap > 1e-2;
-        }
-        if (ambiguous) { send_on(); continue; }
-        double z[NC], lk[NC];
+        double z[NC];
         bool support = true;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             z[i] = P.feed[i * n + c];
-            lk[i] = P.a_v[((best * 5 + 4) * NC + i) * n + u];
             support = support && z[i] > 0;
         }
-        if (!support) { send_on(); continue; }
+        bool any = false;
+        for (int j = 0; j < na; ++j) { any = any || (P.a_st[j * n + u] & 0xFF) == A_CONVERGED; }
+        if (!any) { continue; }
+        if (!support) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
         const Mix<NC>& m = smix[mix_at(P, c)];
         const double pc = P.p[c], tc = P.t[c];
         Counters cnt{0, 0, 0, 0};
-        uint32_t evals = 0;
-        bool rejected = false, ok = false;
-        // Newton on logK with step halving: converged at 1e-2 fug_tol, or within fug_tol once
-        // a step no longer reduces the residual by 4x (FP64 rounding floor).
-        SplitState<NC> s, trial;
-        double guess = P.a_s[(best * 6 + 1) * n + u];
-        ok = split_evaluate<NC, false>(m, pc, tc, z, lk, evals, 0xFFFFFFFFu, s, cnt, rejected, guess) == -1 &&
-             !rejected;
-        bool done = !ok;
-        ok = false;
-        for (int it = 0; it < 12 && !done; ++it) {
-            if (s.norm <= 1e-2 * P.fug_tol) { ok = true; break; }
-            double Am[NC * NC], dl[NC];
-            int piv[NC];
-            double nrm;
-            if (!split_jacobian<NC>(m, pc, tc, z, s, Am) || !lu_factor<NC>(Am, piv, nrm)) { break; }
-#pragma unroll
-            for (int i = 0; i < NC; ++i) { dl[i] = -s.r[i]; }
-            if (!lu_solve<NC>(Am, piv, dl)) { break; }
-            double step = 1.0;
-            bool took = false;
-            for (int bt = 0; bt < 5 && !took; ++bt, step *= 0.5) {
-#pragma unroll
-                for (int i = 0; i < NC; ++i) { lk[i] = s.lk[i] + step * dl[i]; }
-                const int e = split_evaluate<NC, false>(m, pc, tc, z, lk, evals, 0xFFFFFFFFu, trial, cnt, rejected,
-                                                        s.fr.beta);
-                if (rejected) { break; }
-                took = e == -1 && trial.norm < s.norm;
+        bool rejected = false;
+        SplitState<NC> s;
+        int sel = -1;
+        uint32_t tried = 0;
+        for (int attempt = 0; attempt < 3 && sel < 0; ++attempt) {
+            int best = -1;
+            double best_g = 0;
+            for (int j = 0; j < na; ++j) {
+                if ((P.a_st[j * n + u] & 0xFF) != A_CONVERGED || ((tried >> j) & 1u)) { continue; }
+                const double g = P.a_s[(j * 6 + 0) * n + u];
+                if (best < 0 || g < best_g) { best = j; best_g = g; }
             }
-            if (rejected) { break; }
-            if (!took) { ok = s.norm <= P.fug_tol; break; }
-            const bool stalled = trial.norm > 0.25 * s.norm;
-            s = trial;
-            if (stalled && s.norm <= P.fug_tol) { ok = true; break; }
+            if (best < 0) { break; }
+            tried |= 1u << best;
+            double lk[NC];
+            attempt_logk<NC>(P, best, u, lk);
+            if (!polish_split<NC>(m, pc, tc, z, lk, P.a_s[(best * 6 + 1) * n + u], P.fug_tol, s, cnt, rejected)) {
+                if (rejected) { break; }
+                continue;
+            }
+            double contrast = 0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) { contrast = smax(contrast, __builtin_fabs(s.lk[i])); }
+            if (!(s.norm <= P.fug_tol) || !(s.fr.mass_abs <= 1e-12 && s.fr.mass_rel <= 1e-10) || s.fr.beta <= 1e-10 ||
+                1 - s.fr.beta <= 1e-10 || contrast <= 1e-7 || s.zv - s.zl <= 1e-8 * smax(s.zv, s.zl)) {
+                continue;
+            }
+            sel = best;
         }
         atomicAdd(&P.cnt[0 * n + c], cnt.prop);
         atomicAdd(&P.cnt[2 * n + c], cnt.rr);
         if (rejected) { atomicOr(&P.flags[c], CF_REJECTED); continue; }
-        if (!ok) { send_on(); continue; }
-        // split_check's acceptance tests in FP64 (pt_split.hpp:349-361)
-        double contrast = 0;
-#pragma unroll
-        for (int i = 0; i < NC; ++i) { contrast = smax(contrast, __builtin_fabs(s.lk[i])); }
-        if (!(s.fr.mass_abs <= 1e-12 && s.fr.mass_rel <= 1e-10) || s.fr.beta <= 1e-10 || 1 - s.fr.beta <= 1e-10 ||
-            contrast <= 1e-7 || s.zv - s.zl <= 1e-8 * smax(s.zv, s.zl)) {
-            send_on();
-            continue;
-        }
-        P.a_st[best * n + u] = A_CONVERGED | (s.liq.branch << 8) | (s.vap.branch << 16);
+        // no candidate holds in FP64: the reference's outcome without a converged attempt
+        if (sel < 0) { continue; }
+        P.a_st[sel * n + u] = A_CONVERGED | (s.liq.branch << 8) | (s.vap.branch << 16);
         const double sv[6] = {s.gibbs, s.fr.beta, s.zl, s.zv, s.norm, s.guard};
 #pragma unroll
-        for (int q = 0; q < 6; ++q) { P.a_s[(best * 6 + q) * n + u] = sv[q]; }
+        for (int q = 0; q < 6; ++q) { P.a_s[(sel * 6 + q) * n + u] = sv[q]; }
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            P.a_v[((best * 5 + 0) * NC + i) * n + u] = s.fr.x[i];
-            P.a_v[((best * 5 + 1) * NC + i) * n + u] = s.fr.y[i];
-            P.a_v[((best * 5 + 2) * NC + i) * n + u] = s.liq.lnphi[i];
-            P.a_v[((best * 5 + 3) * NC + i) * n + u] = s.vap.lnphi[i];
-            P.a_v[((best * 5 + 4) * NC + i) * n + u] = s.common[i];
+            P.a_v[((sel * 5 + 0) * NC + i) * n + u] = s.fr.x[i];
+            P.a_v[((sel * 5 + 1) * NC + i) * n + u] = s.fr.y[i];
+            P.a_v[((sel * 5 + 2) * NC + i) * n + u] = s.liq.lnphi[i];
+            P.a_v[((sel * 5 + 3) * NC + i) * n + u] = s.vap.lnphi[i];
+            P.a_v[((sel * 5 + 4) * NC + i) * n + u] = s.common[i];
         }
-        P.u_sel[u] = best;
+        P.u_sel[u] = sel;
         double feed_g = 0, corr = 0, mag = 1;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
@@ -753,23 +801,10 @@ __global__ __launch_bounds__(64) void k_polish(Pipe<NC> P, const double* __restr
         const double dg = s.gibbs - feed_g;
         const double guard = s.guard + 256 * kEps * mag;
         const double ftol = 1e-10 + 0.5 * s.norm;
-        if (!finite(dg) || !finite(guard) || !finite(ftol)) { send_on(); continue; }
-        // a feed-Gibbs comparison within rounding reach of its guard is left to FP64
-        if (__builtin_fabs(dg - guard) <= 1e-9 * mag) { send_on(); continue; }
+        if (!finite(dg) || !finite(guard) || !finite(ftol)) { continue; }
         P.u_v[u] = -1;
         P.u_ftol[u] = ftol;
         P.u_dgbad[u] = dg > guard;
-    }
-}
-
-// After the mixed publication: the cells sent on are re-flashed by the FP64 schedule.
-template <int NC>
-__global__ __launch_bounds__(64) void k_rerun_mark(Pipe<NC> P) {
-    const int64_t n = P.n;
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t f = P.flags[c];
-        P.flags[c] = (f & CF_FALLBACK) ? CF_RERUN : (f & ~CF_RERUN);
     }
 }
 

@@ -53,6 +53,7 @@ def compare(a: dict, b: dict, rel: float = 1e-9, abs_lnphi: float = 1e-9, ignore
         "count_mismatch": int(count_bad.sum()),
         "branch_mismatch": int(branch_bad.sum()),
         "value_mismatch": int((value_bad & ~decision_bad).sum()),
+        "decision_cells": int(decision_bad.sum()),
         "max_beta_rel": float(beta_err[~decision_bad].max(initial=0.0)),
         "max_x_rel": float(x_err[~decision_bad].max(initial=0.0)),
         "max_lnphi_err": float(lp_err[~decision_bad].max(initial=0.0) * rel),
