@@ -87,10 +87,14 @@ typedef struct mpmc_flash_options {
      * 3: Pr76PtFlashBackend::solve (max3, pr76_pt_flash_backend.hpp:190-237) */
     int32_t max_phases;
     /* 0: every evaluation in FP64 with the reference's bisection roots.
-     * 1: mixed precision. TPD descents start with mixing, ln(phi) and the TPD in FP32 and
-     *    switch to FP64 before any termination test; cubic brackets, the near-multiple test
-     *    and root selection stay FP64 (roots by safeguarded Newton inside the same
-     *    brackets); Rachford-Rice, splits, Gibbs comparisons and reviews are FP64.
+     * 1: mixed precision. TPD descents and split attempts run with mixing, ln(phi), the
+     *    TPD and successive substitution in FP32, Rachford-Rice in FP32 away from the ends
+     *    of (0, 1); cubic brackets, roots and root selection stay FP64. A search whose
+     *    outcome is within FP32 rounding of a threshold (or slow enough to meet the
+     *    iteration limit) is continued in FP64 under the reference's rules; the selected
+     *    split is polished by FP64 Newton and every published value is FP64. Decisions
+     *    can differ from precision 0 where the reference's own search fails (line-search
+     *    failure, iteration limit); results within 1e-8 relative otherwise.
      *    Staged schedule only. */
     int32_t precision;
     /* 0: staged batch (one lane per stability trial / split attempt, persistent kernels)
