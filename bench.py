@@ -111,15 +111,28 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
         e1[k].record(stream)
     torch.cuda.synchronize()
     ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
+    # the same evaluation captured once into a HIP graph and replayed (one launch per Newton step)
+    plan = A.NewtonPlan(m, g, md, qd, bd, pd, dt)
+    for _ in range(warmup):
+        plan.launch(stream)
+    torch.cuda.synchronize()
+    for k in range(steps):
+        e0[k].record(stream)
+        plan.launch(stream)
+        e1[k].record(stream)
+    torch.cuda.synchronize()
+    plan_ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
+    plan.close()
     ncell, nface, Q = grid["ncell"], grid["owner"].size, 2 * nc + 1
     nnz = ncell * Q * Q + 2 * nface * (nc + 1) * Q
     algo_bytes = 8.0 * (ncell * (Q + nc + 3) + nface * 2 + nnz + ncell * Q)
     gbs = algo_bytes / (ms * 1e-3) / 1e9
     print(f"[bench] assembly: {ms:.3f} ms per Newton evaluation, {gbs:.0f} GB/s algorithmic "
-          f"({ncell} cells, {nface} faces)", file=sys.stderr, flush=True)
+          f"({ncell} cells, {nface} faces); graph replay {plan_ms:.3f} ms", file=sys.stderr, flush=True)
     g.close()
     m.close()
     out = {"grid": list(dims), "cells": ncell, "faces": int(nface), "unknowns_per_cell": Q, "ms": ms,
+           "graph_replay_ms": plan_ms,
            "algorithmic_bytes": algo_bytes, "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS,
            "frac": gbs / PEAK_HBM_GBS,
            "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables"}
@@ -174,8 +187,12 @@ def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
     near = int(sum(round(0.3 * m["ncell"]) if np.isfinite(b).any() else 0 for m, b in zip(layout, pb)))
     print(f"[bench] uneven: {len(layout)} mixtures, {ncell} cells ({near} near a boundary), "
           f"built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    plan = flash.Plan(segments, counters=False)
     stream = torch.cuda.current_stream()
+    # launch-group autotuner: the mixtures per launch group chosen by timing graph replays
+    plan, tune = flash.autotune_plan(segments, candidates=(0, 16, 8), replays=1, stream=stream, counters=False)
+    print(f"[bench] uneven: autotuned mixtures per group {plan.group_mixtures} "
+          f"(ms per replay by candidate: {', '.join(f'{k}: {v:.0f}' for k, v in tune.items())})",
+          file=sys.stderr, flush=True)
     for _ in range(warmup):
         plan.launch(stream)
     torch.cuda.synchronize()
@@ -216,6 +233,7 @@ def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
           file=sys.stderr, flush=True)
     return {"mixtures": len(layout), "cells": ncell, "near_boundary_cells": near, "launch_groups": groups,
             "plan_ms": plan_ms, "cells_per_s": ncell / (plan_ms * 1e-3), "one_mixture_at_a_time_ms": alone_ms,
+            "group_mixtures": plan.group_mixtures, "autotune_ms_by_group_mixtures": tune,
             "speedup_vs_one_at_a_time": alone_ms / plan_ms, "identical_records": bool(identical),
             "phase_counts": counts, "precision": "fp64",
             "what": "max-3-phase flash of every cell; ragged launches grouped by nc, one HIP graph"}

@@ -100,7 +100,12 @@ typedef struct mpmc_flash_options {
     /* 0: staged batch (one lane per stability trial / split attempt, persistent kernels)
      * 1: serial (one lane per cell runs the whole search) */
     int32_t schedule;
-    int32_t reserved[3];
+    /* flash plans: the most mixtures in one launch group (0: as many as the group's LDS
+     * mixture table holds, mpmc_flash_multi_max_mixtures). Smaller groups mean more
+     * launches but smaller tables and more resident workgroups per CU; the autotuner in
+     * mpmc_hnu_amd/flash.py (autotune_plan) picks it by timing. */
+    int32_t group_mixtures;
+    int32_t reserved[2];
 } mpmc_flash_options;
 
 typedef struct mpmc_flash_outputs {
@@ -333,6 +338,24 @@ int mpmc_assembly_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_a
 int mpmc_inventory_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
                       const double* q, const int32_t* branch, int32_t* cell_status, double* inventory,
                       int32_t* domain_error, void* stream);
+
+/* Newton-evaluation plan: one mpmc_assembly_2p evaluation on these exact device buffers
+ * (the per-cell selected-phase property closure, then the TPFA rows), captured once into a
+ * HIP graph. mpmc_assembly_plan_launch replays it on the caller's stream from the current
+ * contents of q, branch and prev (dt fixed at creation), with no allocation and one launch
+ * call; the outputs are those of mpmc_assembly_2p. The plan owns its record workspace and
+ * error flag, so plans on one grid are independent; replays of one plan must not overlap.
+ * Replaces the per-iteration evaluate_complete_assembly call of snes_function and
+ * snes_jacobian (two_phase_snes_assembly.hpp:1698,1753). */
+typedef struct mpmc_assembly_plan* mpmc_assembly_plan_t;
+int mpmc_assembly_plan_create(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
+                              const double* q, const int32_t* branch, const double* prev, double dt,
+                              const mpmc_assembly_outputs* out, mpmc_assembly_plan_t* plan);
+int mpmc_assembly_plan_launch(mpmc_assembly_plan_t plan, void* stream);
+/* Synchronises the stream and reports whether the last replay raised a domain error. */
+int mpmc_assembly_plan_domain_error(mpmc_assembly_plan_t plan, void* stream, int32_t* domain_error);
+/* Waits for the device, then frees the plan. */
+int mpmc_assembly_plan_destroy(mpmc_assembly_plan_t plan);
 
 /* ---------------------------------------------------------------- serving
  * Continuous-batching flash server: the PT service request loop (reference

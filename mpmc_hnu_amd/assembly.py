@@ -128,6 +128,50 @@ def assemble(mixture: Mixture, grid: Grid, model, q, branch, prev, dt: float, ou
     return out
 
 
+class NewtonPlan:
+    """One Newton evaluation (mpmc_assembly_2p) on fixed device buffers, captured once into
+    a HIP graph (mpmc_assembly_plan_*). Write the next iterate into ``q`` (and ``branch``,
+    ``prev``) in place, then ``launch``; the outputs land in ``out``."""
+
+    def __init__(self, mixture: Mixture, grid: Grid, model, q, branch, prev, dt: float, out: dict | None = None):
+        torch = _torch()
+        lib = native.library()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.q = _dev(q, torch.float64, dev)
+        self.branch = _dev(branch, torch.int32, dev)
+        self.prev = _dev(prev, torch.float64, dev)
+        self.out = out if out is not None else allocate(mixture.nc, grid, dev)
+        o = native.AssemblyOutputs(self.out["cell_status"].data_ptr(), self.out["residual"].data_ptr(),
+                                   self.out["diag"].data_ptr(), self.out["offdiag"].data_ptr())
+        h = ctypes.c_void_p()
+        native.check(lib.mpmc_assembly_plan_create(mixture.handle, grid.h, ctypes.byref(model),
+                                                   V(self.q.data_ptr()), V(self.branch.data_ptr()),
+                                                   V(self.prev.data_ptr()), float(dt), ctypes.byref(o),
+                                                   ctypes.byref(h)), "assembly_plan_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        native.check(native.library().mpmc_assembly_plan_launch(self.h, _stream_ptr(stream)), "assembly_plan_launch")
+        return self.out
+
+    def domain_error(self, stream=None) -> int:
+        flag = ctypes.c_int32(0)
+        native.check(native.library().mpmc_assembly_plan_domain_error(self.h, _stream_ptr(stream), ctypes.byref(flag)),
+                     "assembly_plan_domain_error")
+        return int(flag.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.library().mpmc_assembly_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def inventory(mixture: Mixture, grid: Grid, model, q, branch, stream=None):
     torch = _torch()
     lib = native.library()

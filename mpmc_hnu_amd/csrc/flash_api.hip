@@ -175,6 +175,7 @@ static int flash_options(const mpmc_flash_options* options, mpmc_flash_options& 
     if (o.schedule == 1 && o.precision != 0) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: the serial schedule runs FP64 only");
     }
+    if (o.group_mixtures < 0) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: group_mixtures must be >= 0"); }
     return MPMC_OK;
 }
 
@@ -494,7 +495,9 @@ int mpmc_flash_plan_create(int nseg, const mpmc_flash_segment* segs, const mpmc_
         if (fits) {
             PlanGroup& g = pl->groups.back();
             const bool known = std::find(g.mixtures.begin(), g.mixtures.end(), segs[k].mixture) != g.mixtures.end();
-            fits = known || static_cast<int>(g.mixtures.size()) < mpmc_flash_multi_max_mixtures(nc);
+            const int cap = o.group_mixtures > 0 ? std::min(o.group_mixtures, mpmc_flash_multi_max_mixtures(nc))
+                                                 : mpmc_flash_multi_max_mixtures(nc);
+            fits = known || static_cast<int>(g.mixtures.size()) < cap;
             fits = fits && g.n + segs[k].ncell <= (int64_t{1} << 31);
         }
         if (!fits) {
@@ -888,9 +891,10 @@ void mpmc_assembly_default_model(int nc, mpmc_assembly_model* md) {
     for (int i = 0; i < nc && i < MPMC_MAX_COMPONENTS; ++i) { md->molar_mass[i] = 0.016 + 0.014 * i; }
 }
 
-static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
-                         const int32_t* branch, const double* prev, double dt, const mpmc_assembly_outputs* out,
-                         int32_t* domain_error, bool inventory, void* stream) {
+// Argument contract of one Newton evaluation; on success `need` is the record workspace size.
+static int assembly_check(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
+                          const int32_t* branch, const double* prev, double dt, const mpmc_assembly_outputs* out,
+                          bool inventory, size_t& need) {
     if (!m || !g || !md || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: null argument"); }
     const int nc = m->nc;
     if (nc < 2) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly: two-phase assembly needs nc >= 2"); }
@@ -917,14 +921,46 @@ static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assemb
             return fail(MPMC_E_INVALID_ARGUMENT, "assembly: offdiag must be device memory");
         }
     }
-    const hipStream_t s = static_cast<hipStream_t>(stream);
-    size_t need = 0;
+    need = 0;
     switch (nc) {
 #define MPMC_CASE(N) case N: need = mpmc::dev::asm_record_bytes<N>(g->ncell); break;
         MPMC_FOR_EACH_NC2(MPMC_CASE)
 #undef MPMC_CASE
     default: return fail(MPMC_E_INVALID_ARGUMENT, "assembly: unsupported nc");
     }
+    return MPMC_OK;
+}
+
+// The kernels of one Newton evaluation with the given record workspace and error flag.
+static hipError_t assembly_launch(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md,
+                                  const double* q, const int32_t* branch, const double* prev, double dt,
+                                  const mpmc_assembly_outputs* out, void* rec, unsigned long long* flag,
+                                  bool inventory, hipStream_t s) {
+    mpmc::dev::AsmModelDev dm{};
+    dm.mu_a0 = md->mu_a0; dm.mu_a1 = md->mu_a1; dm.cp = md->cp; dm.t0 = md->t0; dm.cv_rock = md->cv_rock;
+    for (int i = 0; i < MPMC_MAX_COMPONENTS; ++i) { dm.molar_mass[i] = md->molar_mass[i]; }
+    const mpmc::dev::AsmGrid G{g->ncell, g->nface,    g->bulk,     g->poro,      g->owner,     g->neighbour,
+                               g->trans, g->gdot,     g->gth,      g->adj_off,   g->adj_face,  g->slot_face,
+                               g->slot_other, g->max_degree, g->block_stride};
+    const mpmc::dev::AsmIo io{q, branch, prev, dt, out->cell_status, out->residual, out->diag, out->offdiag,
+                              static_cast<double*>(rec), flag};
+    hipError_t e = hipErrorInvalidValue;
+    switch (m->nc) {
+#define MPMC_CASE(N) case N: e = mpmc::dev::launch_assembly<N>(m->dev_constants, dm, G, io, inventory, s); break;
+        MPMC_FOR_EACH_NC2(MPMC_CASE)
+#undef MPMC_CASE
+    default: break;
+    }
+    return e;
+}
+
+static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
+                         const int32_t* branch, const double* prev, double dt, const mpmc_assembly_outputs* out,
+                         int32_t* domain_error, bool inventory, void* stream) {
+    size_t need = 0;
+    const int rc = assembly_check(m, g, md, q, branch, prev, dt, out, inventory, need);
+    if (rc != MPMC_OK) { return rc; }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
     hipError_t e = hipSuccess;
     if (g->rec_bytes < need) {
         if (g->rec) {
@@ -938,21 +974,7 @@ static int assembly_impl(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assemb
         if (e != hipSuccess) { g->rec = nullptr; return fail(MPMC_E_NO_MEMORY, "assembly: workspace allocation failed"); }
         g->rec_bytes = need;
     }
-    mpmc::dev::AsmModelDev dm{};
-    dm.mu_a0 = md->mu_a0; dm.mu_a1 = md->mu_a1; dm.cp = md->cp; dm.t0 = md->t0; dm.cv_rock = md->cv_rock;
-    for (int i = 0; i < MPMC_MAX_COMPONENTS; ++i) { dm.molar_mass[i] = md->molar_mass[i]; }
-    const mpmc::dev::AsmGrid G{g->ncell, g->nface,    g->bulk,     g->poro,      g->owner,     g->neighbour,
-                               g->trans, g->gdot,     g->gth,      g->adj_off,   g->adj_face,  g->slot_face,
-                               g->slot_other, g->max_degree, g->block_stride};
-    const mpmc::dev::AsmIo io{q, branch, prev, dt, out->cell_status, out->residual, out->diag, out->offdiag,
-                              static_cast<double*>(g->rec), g->flag};
-    e = hipErrorInvalidValue;
-    switch (nc) {
-#define MPMC_CASE(N) case N: e = mpmc::dev::launch_assembly<N>(m->dev_constants, dm, G, io, inventory, s); break;
-        MPMC_FOR_EACH_NC2(MPMC_CASE)
-#undef MPMC_CASE
-    default: break;
-    }
+    e = assembly_launch(m, g, md, q, branch, prev, dt, out, g->rec, g->flag, inventory, s);
     if (e != hipSuccess) { return hip_fail(e, "assembly launch"); }
     if (domain_error) {
         unsigned long long flag = 0;
@@ -975,6 +997,78 @@ int mpmc_inventory_2p(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_
                       void* stream) {
     const mpmc_assembly_outputs out{cell_status, inventory, nullptr, nullptr};
     return assembly_impl(m, g, md, q, branch, nullptr, 1.0, &out, domain_error, true, stream);
+}
+
+// ---------------------------------------------------------------- Newton-evaluation plan
+struct mpmc_assembly_plan {
+    void* rec = nullptr;                 // the plan's own record workspace
+    unsigned long long* flag = nullptr;  // the plan's domain-error flag
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int mpmc_assembly_plan_destroy(mpmc_assembly_plan_t pl) {
+    if (!pl) { return MPMC_OK; }
+    const hipError_t e = hipDeviceSynchronize();
+    if (pl->exec) { (void)hipGraphExecDestroy(pl->exec); }
+    if (pl->graph) { (void)hipGraphDestroy(pl->graph); }
+    if (pl->rec) { (void)hipFree(pl->rec); }
+    if (pl->flag) { (void)hipFree(pl->flag); }
+    delete pl;
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "assembly_plan_destroy");
+}
+
+int mpmc_assembly_plan_create(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_model* md, const double* q,
+                              const int32_t* branch, const double* prev, double dt,
+                              const mpmc_assembly_outputs* out, mpmc_assembly_plan_t* plan) {
+    if (!plan) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly_plan_create: null handle"); }
+    *plan = nullptr;
+    size_t need = 0;
+    const int rc = assembly_check(m, g, md, q, branch, prev, dt, out, false, need);
+    if (rc != MPMC_OK) { return rc; }
+    std::unique_ptr<mpmc_assembly_plan> pl(new mpmc_assembly_plan());
+    hipError_t e = hipMalloc(&pl->rec, need);
+    if (e == hipSuccess) { e = hipMalloc(&pl->flag, sizeof(unsigned long long)); }
+    if (e != hipSuccess) {
+        mpmc_assembly_plan_destroy(pl.release());
+        return fail(MPMC_E_NO_MEMORY, "assembly_plan_create: device allocation failed");
+    }
+    // one eager evaluation, then the same launch sequence captured once
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) { e = assembly_launch(m, g, md, q, branch, prev, dt, out, pl->rec, pl->flag, false, cs); }
+    if (e == hipSuccess) { e = hipStreamSynchronize(cs); }
+    if (e == hipSuccess) { e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal); }
+    if (e == hipSuccess) {
+        const hipError_t rec = assembly_launch(m, g, md, q, branch, prev, dt, out, pl->rec, pl->flag, false, cs);
+        e = hipStreamEndCapture(cs, &pl->graph);
+        if (rec != hipSuccess) { e = rec; }
+    }
+    if (e == hipSuccess) { e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0); }
+    if (cs) { (void)hipStreamDestroy(cs); }
+    if (e != hipSuccess) {
+        mpmc_assembly_plan_destroy(pl.release());
+        return hip_fail(e, "assembly_plan_create capture");
+    }
+    *plan = pl.release();
+    return MPMC_OK;
+}
+
+int mpmc_assembly_plan_launch(mpmc_assembly_plan_t pl, void* stream) {
+    if (!pl || !pl->exec) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly_plan_launch: null plan"); }
+    const hipError_t e = hipGraphLaunch(pl->exec, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "assembly_plan_launch");
+}
+
+int mpmc_assembly_plan_domain_error(mpmc_assembly_plan_t pl, void* stream, int32_t* domain_error) {
+    if (!pl || !domain_error) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly_plan_domain_error: null argument"); }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long flag = 0;
+    hipError_t e = hipMemcpyAsync(&flag, pl->flag, sizeof(flag), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) { e = hipStreamSynchronize(s); }
+    if (e != hipSuccess) { return hip_fail(e, "assembly_plan_domain_error"); }
+    *domain_error = flag ? 1 : 0;
+    return MPMC_OK;
 }
 
 } // extern "C"

@@ -209,7 +209,7 @@ class Plan:
     replays the captured graph over the segments' current input contents."""
 
     def __init__(self, segments, fugacity_tolerance: float = 1e-11, max_phases: int = 3, precision: int = 0,
-                 counters: bool = True):
+                 counters: bool = True, group_mixtures: int = 0):
         torch = _torch()
         lib = native.library()
         device = torch.device("cuda", torch.cuda.current_device())
@@ -228,6 +228,8 @@ class Plan:
             h = mix.handle.value if isinstance(mix.handle, ctypes.c_void_p) else mix.handle
             segs[k] = native.FlashSegment(h, n, p.data_ptr(), t.data_ptr(), z.data_ptr(), _outputs_struct(out))
         opts = _options(fugacity_tolerance, max_phases, precision)
+        opts.group_mixtures = int(group_mixtures)
+        self.group_mixtures = int(group_mixtures)
         self.handle = ctypes.c_void_p()
         native.check(lib.mpmc_flash_plan_create(len(segments), segs, ctypes.byref(opts), ctypes.byref(self.handle)),
                      "flash_plan_create")
@@ -248,6 +250,36 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def autotune_plan(segments, candidates=(0, 16, 8), replays: int = 1, stream=None, **plan_kwargs):
+    """Launch-group autotuner for an uneven batch: builds the plan once per candidate
+    ``group_mixtures`` (0: the LDS-table maximum per group), times ``replays`` graph replays
+    of each with HIP events on ``stream``, and keeps the fastest. Small groups trade more
+    launches for smaller LDS mixture tables (more resident workgroups per CU) and shorter
+    per-launch tails. Every candidate gives the same records (the grouping does not change
+    any cell's computation). Returns (plan, {candidate: ms per replay})."""
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    timings, best, best_ms = {}, None, float("inf")
+    for cand in candidates:
+        plan = Plan(segments, group_mixtures=cand, **plan_kwargs)
+        plan.launch(s)  # first replay outside the timing
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(replays):
+            plan.launch(s)
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / replays
+        timings[int(cand)] = ms
+        if ms < best_ms:
+            if best is not None:
+                best.close()
+            best, best_ms = plan, ms
+        else:
+            plan.close()
+    return best, timings
 
 
 def phase_batch(mixture: Mixture, p, t, x, stream=None) -> dict:

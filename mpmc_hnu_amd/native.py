@@ -42,6 +42,10 @@ EXPORTED = (
     "mpmc_assembly_default_model",
     "mpmc_assembly_2p",
     "mpmc_inventory_2p",
+    "mpmc_assembly_plan_create",
+    "mpmc_assembly_plan_launch",
+    "mpmc_assembly_plan_domain_error",
+    "mpmc_assembly_plan_destroy",
     "mpmc_server_default_options",
     "mpmc_server_create",
     "mpmc_server_destroy",
@@ -61,7 +65,8 @@ class FlashOptions(ctypes.Structure):
         ("max_phases", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("group_mixtures", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -197,6 +202,11 @@ def library() -> ctypes.CDLL:
                                      ctypes.POINTER(AssemblyOutputs), ctypes.POINTER(ctypes.c_int32), P]
     lib.mpmc_inventory_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, P,
                                       ctypes.POINTER(ctypes.c_int32), P]
+    lib.mpmc_assembly_plan_create.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, ctypes.c_double,
+                                              ctypes.POINTER(AssemblyOutputs), ctypes.POINTER(P)]
+    lib.mpmc_assembly_plan_launch.argtypes = [P, P]
+    lib.mpmc_assembly_plan_domain_error.argtypes = [P, P, ctypes.POINTER(ctypes.c_int32)]
+    lib.mpmc_assembly_plan_destroy.argtypes = [P]
     lib.mpmc_server_default_options.argtypes = [ctypes.POINTER(ServerOptions)]
     lib.mpmc_server_default_options.restype = None
     lib.mpmc_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]

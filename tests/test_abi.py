@@ -38,7 +38,7 @@ def test_options_struct_layout_matches_header():
     lib.mpmc_flash_default_options(ctypes.byref(o))
     assert o.fugacity_tolerance == 1e-11 and o.max_phases == 3 and o.precision == 0
     assert ctypes.sizeof(native.FlashOptions) == 8 + 4 + 4 + 4 + 12
-    assert o.schedule == 0
+    assert o.schedule == 0 and o.group_mixtures == 0
     assert lib.mpmc_flash_workspace_bytes(6, 1000) > 1000 * 8 * 6
     assert lib.mpmc_flash_workspace_bytes(13, 1000) == 0
     assert ctypes.sizeof(native.FlashOutputs) == 8 * 8

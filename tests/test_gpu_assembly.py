@@ -98,3 +98,49 @@ def test_gpu_assembly_matches_oracle(gpu, nc, dims, low_p):
     # inventories (the previous-time input)
     ok, worst = _close(g["inventory"].T, ob.inventory(mix, model, q, br, grid["poro"]).T)
     assert ok, worst
+
+
+@pytest.mark.gpu
+def test_newton_plan_replays_match_eager_assembly(gpu):
+    """A captured Newton evaluation (mpmc_assembly_plan_*) replayed over iterates written in
+    place gives, on every replay, the records the eager mpmc_assembly_2p gives on the same
+    iterate; eager work and synchronisations run between the replays."""
+    import torch
+
+    from mpmc_hnu_amd import assembly as A, flash
+
+    nc, dims = 6, (6, 5, 4)
+    mix = W.mixture_linspaced(nc)
+    grid = W.tpfa_grid(*dims, seed=11)
+    roots = ob.phase_roots(mix)
+    states = [W.two_phase_state(nc, grid["ncell"], roots, seed=70 + k, dims=dims) for k in range(3)]
+    m = flash.Mixture.from_dict(mix)
+    g = A.Grid(grid)
+    md = A.default_model(nc)
+    try:
+        q0, br0 = states[0]
+        inv, _, _ = A.inventory(m, g, md, q0, br0)
+        prev = (inv * 0.99).contiguous()
+        dev = torch.device("cuda", 0)
+        qbuf = torch.as_tensor(q0, device=dev).clone()
+        bbuf = torch.as_tensor(br0, device=dev).clone()
+        plan = A.NewtonPlan(m, g, md, qbuf, bbuf, prev, 3600.0)
+        try:
+            for rep in range(6):
+                q, br = states[rep % len(states)]
+                qbuf.copy_(torch.as_tensor(q, device=dev))
+                bbuf.copy_(torch.as_tensor(br, device=dev))
+                out = plan.launch()
+                torch.cuda.synchronize()
+                got = {k: out[k].cpu().numpy().copy() for k in ("cell_status", "residual", "diag", "offdiag")}
+                assert plan.domain_error() == 0
+                ref = A.assemble(m, g, md, q, br, prev, 3600.0)
+                assert ref["domain_error"] == 0
+                torch.cuda.synchronize()
+                for k in ("cell_status", "residual", "diag", "offdiag"):
+                    assert np.array_equal(got[k], ref[k].cpu().numpy()), (rep, k)
+        finally:
+            plan.close()
+    finally:
+        g.close()
+        m.close()

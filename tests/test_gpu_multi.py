@@ -124,6 +124,46 @@ def test_plan_groups_replays_and_updates(gpu):
 
 
 @pytest.mark.gpu
+def test_autotuned_group_size_keeps_every_record(gpu):
+    """The launch-group autotuner: plans with 1, 2 and the maximum mixtures per launch group
+    split the same segments into different launch groups and give identical records; the
+    autotuner returns one of them with a timing per candidate."""
+    import torch
+
+    from mpmc_hnu_amd import flash
+
+    mixes = [flash.Mixture.from_dict(W.mixture_linspaced(6, seed=400 + k)) for k in range(5)]
+    try:
+        segments = []
+        for j in range(10):
+            p, t, z = W.cells_uniform(6, 200 + 37 * j, seed=500 + j)
+            segments.append((mixes[j % 5], torch.as_tensor(p, device="cuda"), torch.as_tensor(t, device="cuda"),
+                             torch.as_tensor(z, device="cuda")))
+        print("[case] autotuned plan over 10 segments of 5 mixtures", flush=True)
+        plans = {g: flash.Plan(segments, group_mixtures=g) for g in (0, 2, 1)}
+        try:
+            # segments in the given order, a group closing when its table is full
+            assert plans[0].groups == 1 and plans[2].groups == 5 and plans[1].groups == 10
+            torch.cuda.synchronize()
+            for k in range(len(segments)):
+                ref = _host(plans[0].outputs[k])
+                for g in (2, 1):
+                    _same(_host(plans[g].outputs[k]), ref, ("group_mixtures", g, k))
+        finally:
+            for pl in plans.values():
+                pl.close()
+        best, tune = flash.autotune_plan(segments, candidates=(0, 2, 1), replays=1)
+        try:
+            assert set(tune) == {0, 2, 1} and best.group_mixtures in tune
+            assert all(v > 0 for v in tune.values())
+        finally:
+            best.close()
+    finally:
+        for m in mixes:
+            m.close()
+
+
+@pytest.mark.gpu
 def test_uneven_batch_plan_equals_one_mixture_at_a_time(gpu):
     """The uneven-batch bench leg at 1% of its cells: 256 mixtures of nc 2..12, 30% of the
     cells near a phase boundary; the plan's records equal the per-mixture flashes."""
