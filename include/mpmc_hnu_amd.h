@@ -144,6 +144,20 @@ int mpmc_flash_batch_ws(mpmc_mixture_t mixture, int64_t ncell, const double* p, 
                         const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* Warm-started batch flash: up to 2 compositions per cell, hints[(h*nc + i)*ncell + c]
+ * (device memory), appended in order to the initial stability test's starts after the
+ * generated ones, as Pr76PtFlashBackendOptions::initial_starts does for the
+ * initial_stability_starts of PtSolveHints (pt_solve_hints.hpp:26-43,
+ * pr76_executable_model.hpp:134-140; pr76_three_phase.hpp:328-341). A hint whose component 0
+ * is NaN is absent for that cell. A present hint must be a composition (sum 1 within 64 eps)
+ * with exactly the feed's support, else the cell is rejected (MPMC_FLAG_REJECTED), as the
+ * reference rejects the request (pt_stability.hpp:264-271). workspace NULL: the mixture's own.
+ * Staged schedule only. */
+int mpmc_flash_batch_hinted(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                            const double* z, int nhint, const double* hints,
+                            const mpmc_flash_options* options, const mpmc_flash_outputs* out,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
 /*
  * Ragged multi-mixture batch: the cells of several mixtures with the same component count
  * in one staged launch sequence. Mixture k owns cells [cell_offset[k], cell_offset[k+1])

@@ -661,9 +661,13 @@ struct VleOut {
 
 // solve_pt_vle (pt_split.hpp:414-525) with the PR76 evaluator as both providers.
 template <int NC, bool kN = false>
+// starts: extra starts of the initial stability test; final_starts (nfinal >= 0): extra
+// starts of the common-tangent review (pt_split.hpp:414-512 initial_starts / final_starts),
+// nfinal < 0: the initial starts again.
 __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
                                        const double* const* starts, int nstarts, VleOut<NC>& V, Counters& cnt,
-                                       bool& rejected) {
+                                       bool& rejected, const double* const* final_starts = nullptr,
+                                       int nfinal = -1) {
     constexpr uint32_t kMaxSplitEval = 20000;
     constexpr int kMaxAttempts = 16;
     V.status = V_INDETERMINATE;
@@ -722,7 +726,11 @@ __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, con
     if (!finite(dg) || !finite(guard) || !finite(ftol)) { return; }
     const double* fs[6];
     int nf = 0;
-    for (int e = 0; e < nstarts; ++e) { fs[nf++] = starts[e]; }
+    if (nfinal < 0) {
+        for (int e = 0; e < nstarts; ++e) { fs[nf++] = starts[e]; }
+    } else {
+        for (int e = 0; e < nfinal; ++e) { fs[nf++] = final_starts[e]; }
+    }
     fs[nf++] = c.fr.x;
     fs[nf++] = c.fr.y;
     stability<NC, kN>(m, p, t, z, c.common, fs, nf, ftol, V.fin, cnt, rejected);
@@ -1105,12 +1113,15 @@ struct Max3Out {
 // solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), without caller continuation hints,
 // serial in one lane.
 template <int NC, bool kN = false>
+// initial: extra starts of the initial stability test (solve_pr76_pt_max3 initial_starts,
+// pr76_three_phase.hpp:328-341; the warm-start hints of PtSolveHints).
 __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
-                                        Max3Out<NC>& M, Counters& cnt, bool& rejected) {
+                                        Max3Out<NC>& M, Counters& cnt, bool& rejected,
+                                        const double* const* initial = nullptr, int ninitial = 0) {
     M.tail.use_neighbor = false;
     (void)check_composition<NC>(feed, rejected);
     if (rejected) { return; }
-    solve_vle<NC, kN>(m, p, t, feed, fug_tol, nullptr, 0, M.base, cnt, rejected);
+    solve_vle<NC, kN>(m, p, t, feed, fug_tol, initial, ninitial, M.base, cnt, rejected, nullptr, 0);
     if (rejected) { return; }
     switch (M.base.status) {
     case V_SINGLE: M.status = M_SINGLE; return;

@@ -53,6 +53,7 @@ namespace mpmc::dev {
 #endif
 
 constexpr int kAttempts = 16;   // PtSplitOptions::max_split_attempts (pt_split.hpp:49)
+constexpr int kMaxHints = 2;    // warm-start compositions per cell (extra initial stability starts)
 constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
 // Outcomes of a reduced-precision trial or split attempt that FP64 must decide: an
@@ -84,14 +85,14 @@ struct Pipe {
     FlashOut out;
     double* feed;       // [NC][n]
     double* refl;       // [NC][n]
-    int32_t* refinfo;   // [n] branch | smooth << 8 | ok << 9 | active << 16
+    int32_t* refinfo;   // [n] branch | smooth << 8 | ok << 9 | active << 16 | hints << 24
     int32_t* flags;     // [n]
     int32_t* cstat;     // [n] initial stability status
     int32_t* cell_u;    // [n]
     uint32_t* cnt;      // [4][n]
-    int32_t* tst;       // [NC+2][n] status | branch << 8 (-1: no such trial)
-    uint32_t* tev;      // [NC+2][n]
-    double* tw;         // [NC+2][NC][n] final point of negative trials
+    int32_t* tst;       // [NC+2+kMaxHints][n] status | branch << 8 (-1: no such trial)
+    uint32_t* tev;      // [NC+2+kMaxHints][n]
+    double* tw;         // [NC+2+kMaxHints][NC][n] final point of negative trials
     int32_t* u_cell;    // [n]
     int32_t* u_nwit;    // [n]
     int32_t* u_wit;     // [8][n]
@@ -121,6 +122,10 @@ struct Pipe {
     const int32_t* mix_of;
     int nmix;
     int mixed;          // mixed-precision schedule (precision 1): 1, then 2 after the review redo
+    // warm-start hints: nhint compositions per cell, hints[(h*NC + i)*n + c], appended to the
+    // initial stability test's starts in order (NaN in component 0: no hint h for cell c)
+    const double* hints;
+    int nhint;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -156,6 +161,23 @@ __device__ __forceinline__ int mix_at(const Pipe<NC>& P, int64_t c) {
 
 template <int NC>
 __device__ __forceinline__ int active_count(int32_t info) { return (info >> 16) & 0xFF; }
+
+template <int NC>
+__device__ __forceinline__ int hint_count(int32_t info) { return (info >> 24) & 0xF; }
+
+// The m-th warm-start hint present for cell c (hints absent for a cell are skipped).
+template <int NC>
+__device__ __forceinline__ void hint_start(const Pipe<NC>& P, int64_t c, int m, double* w) {
+    int seen = -1;
+    for (int h = 0; h < P.nhint; ++h) {
+        if (__builtin_isnan(P.hints[(h * NC + 0) * P.n + c])) { continue; }
+        if (++seen == m) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) { w[i] = P.hints[(h * NC + i) * P.n + c]; }
+            return;
+        }
+    }
+}
 
 // Start composition of trial k (pt_stability.hpp:442-466): feed, uniform over the active
 // support, 0.1 feed + 0.9 e_i per active i; then the extra starts.
@@ -208,6 +230,20 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
         }
         uint32_t prop = 0;
         int32_t info = active << 16;
+        // warm-start hints: validated before any provider call, as the reference validates
+        // every supplied start (pt_stability.hpp:264-271)
+        int nh = 0;
+        for (int h = 0; h < P.nhint; ++h) {
+            if (__builtin_isnan(P.hints[(h * NC + 0) * n + c])) { continue; }
+            double w[NC];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) { w[i] = P.hints[(h * NC + i) * n + c]; }
+            (void)check_composition<NC>(w, rejected);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) { if ((z[i] > 0) != (w[i] > 0)) { rejected = true; } }
+            ++nh;
+        }
+        info |= nh << 24;
         Activity<NC> ref;
         if (!rejected && !lost) {
             ++prop;
@@ -280,7 +316,7 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int
         if (P.flags[c] & CF_REJECTED) { return false; }
         const int32_t info = P.refinfo[c];
         const int active = active_count<NC>(info);
-        const int ntr = active > 1 ? active + 2 : 1;
+        const int ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info);
         if (!(info & 0x200) || k >= ntr) { st_arr[k * n + slot] = -1; return false; }
     } else {
         const int sel = P.u_sel[slot];
@@ -333,6 +369,8 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int
     }
     if (k < gen) {
         trial_start<NC>(L.z, active, k, L.cand);
+    } else if (stage == 0) {
+        hint_start<NC>(P, c, k - gen, L.cand);
     } else {
         const int sel = P.u_sel[slot];
         const int q = k - gen; // 0: liquid x, 1: vapor y of the selected pair
@@ -404,7 +442,7 @@ template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage,
                                                               int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
-    const int tk = stage == 0 ? NC + 2 : NC + 4;
+    const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
     unsigned long long* counter =
@@ -504,7 +542,7 @@ __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
         const int32_t info = P.refinfo[c];
         if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; continue; }
         const int active = active_count<NC>(info);
-        const int ntr = active > 1 ? active + 2 : 1;
+        const int ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info);
         uint64_t evals = 1;
         bool neg = false, all_stat = true, ambiguous = false;
         int nwit = 0;
@@ -1025,14 +1063,24 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
         Counters cnt{0, 0, 0, 0};
         bool rejected = false;
         PhaseView<NC> ph[3];
+        double hw[kMaxHints][NC];
+        const double* hs[kMaxHints];
+        int nh = 0;
+        for (int h = 0; h < P.nhint; ++h) {
+            if (__builtin_isnan(P.hints[(h * NC + 0) * n + c])) { continue; }
+            for (int q = 0; q < NC; ++q) { hw[nh][q] = P.hints[(h * NC + q) * n + c]; }
+            hs[nh] = hw[nh];
+            ++nh;
+        }
         if (P.max_phases == 2) {
             VleOut<NC> V;
-            solve_vle<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, nullptr, 0, V, cnt, rejected);
+            solve_vle<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, hs, nh, V, cnt, rejected, nullptr,
+                              0);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_vle<NC>(P.out, n, c, V, 0xF, cnt); }
         } else {
             Max3Out<NC> M;
-            solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected);
+            solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected, hs, nh);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_max3<NC>(P.out, n, c, M, cnt); }
         }
@@ -1063,9 +1111,9 @@ __host__ size_t pipe_carve(Pipe<NC>& P, char* base, int64_t n) {
     take(P.cstat, N);
     take(P.cell_u, N);
     take(P.cnt, 4 * N);
-    take(P.tst, (NC + 2) * N);
-    take(P.tev, (NC + 2) * N);
-    take(P.tw, (NC + 2) * NC * N);
+    take(P.tst, (NC + 2 + kMaxHints) * N);
+    take(P.tev, (NC + 2 + kMaxHints) * N);
+    take(P.tw, (NC + 2 + kMaxHints) * NC * N);
     take(P.u_cell, N);
     take(P.u_nwit, N);
     take(P.u_wit, kWitnesses * N);
@@ -1205,8 +1253,12 @@ hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_p
 template <int NC>
 hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, void* workspace,
                            size_t workspace_bytes, int64_t n, const double* p, const double* t, const double* z,
-                           int max_phases, double fug_tol, int precision, const FlashOut& out, hipStream_t s) {
+                           int max_phases, double fug_tol, int precision, const FlashOut& out, hipStream_t s,
+                           int nhint, const double* hints) {
     Pipe<NC> P{};
+    if (nhint < 0 || nhint > kMaxHints || (nhint > 0 && !hints)) { return hipErrorInvalidValue; }
+    P.nhint = nhint;
+    P.hints = hints;
     const size_t need = pipe_carve<NC>(P, static_cast<char*>(workspace), n);
     if (!workspace || need > workspace_bytes) { return hipErrorInvalidValue; }
     if (nmix < 1 || (nmix > 1 && !mix_of)) { return hipErrorInvalidValue; }
@@ -1227,11 +1279,11 @@ hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, 
     extern template size_t pipeline_bytes<N>(int64_t);                                                            \
     extern template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,     \
                                                   const double*, const double*, const double*, int, double, int,  \
-                                                  const FlashOut&, hipStream_t);
+                                                  const FlashOut&, hipStream_t, int, const double*);
 #define MPMC_INSTANTIATE_PIPELINE(N)                                                                              \
     template size_t pipeline_bytes<N>(int64_t);                                                                   \
     template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,            \
                                            const double*, const double*, const double*, int, double, int,         \
-                                           const FlashOut&, hipStream_t);
+                                           const FlashOut&, hipStream_t, int, const double*);
 
 } // namespace mpmc::dev

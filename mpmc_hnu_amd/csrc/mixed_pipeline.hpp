@@ -159,7 +159,7 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
         c = slot;
         if (P.flags[c] & CF_REJECTED) { return false; }
         const int active = active_count<NC>(info0);
-        ntr = active > 1 ? active + 2 : 1;
+        ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info0);
         if (!(info0 & 0x200) || k >= ntr) { st_arr[k * n + slot] = -1; return false; }
     } else {
         const int sel = P.u_sel[slot];
@@ -200,6 +200,8 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
     double w0[NC];
     if (k < gen) {
         trial_start<NC>(z, active, k, w0);
+    } else if (stage == 0) {
+        hint_start<NC>(P, c, k - gen, w0);
     } else {
         const int sel = P.u_sel[slot];
 #pragma unroll
@@ -279,7 +281,7 @@ __device__ __forceinline__ int tpd32_check(const Pipe<NC>& P, int stage, Tpd32La
 template <int NC>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd32(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
-    const int tk = stage == 0 ? NC + 2 : NC + 4;
+    const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
     unsigned long long* counter = &P.g[stage == 0 ? G_TASK_INIT32 : G_TASK_REVIEW32];

@@ -102,8 +102,12 @@ def allocate_outputs(nc: int, ncell: int, device, counters: bool = True) -> dict
 
 def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, max_phases: int = 3,
                 precision: int = 0, stream=None, out: dict | None = None, counters: bool = True,
-                schedule: int = 0, workspace=None) -> dict:
+                schedule: int = 0, workspace=None, hints=None) -> dict:
     """Flash every cell (p[ncell] Pa, t[ncell] K, z[nc, ncell]); returns device tensors.
+
+    ``hints`` ([nhint, nc, ncell], nhint <= 2; NaN in component 0 marks an absent hint)
+    are warm-start compositions appended to each cell's initial stability starts
+    (mpmc_flash_batch_hinted).
 
     schedule 0 runs the staged batch (one lane per stability trial / split attempt),
     schedule 1 one lane per cell. ``workspace`` (a uint8 device tensor of at least
@@ -132,6 +136,18 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
         out["branch"].data_ptr(), out["x"].data_ptr(), out["ln_phi"].data_ptr(),
         out["counters"].data_ptr() if out.get("counters") is not None else None)
     V = ctypes.c_void_p
+    if hints is not None:
+        hints = _dev(hints, torch.float64, device)
+        if hints.dim() != 3 or hints.shape[1:] != (mixture.nc, n) or hints.shape[0] > 2:
+            raise ValueError("flash_batch: expected hints[nhint <= 2, nc, ncell]")
+        ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
+            else (None, 0)
+        rc = lib.mpmc_flash_batch_hinted(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
+                                         int(hints.shape[0]), V(hints.data_ptr()), ctypes.byref(opts), ctypes.byref(o),
+                                         ws, wsb, _stream_ptr(stream))
+        native.check(rc, "flash_batch_hinted")
+        out["_inputs"] = (p, t, z, hints)
+        return out
     if workspace is None:
         rc = lib.mpmc_flash_batch(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
                                   ctypes.byref(opts), ctypes.byref(o), _stream_ptr(stream))

@@ -323,3 +323,18 @@ def boundary_pressures(phase_count, items: list, lo: float = 0.5 * MPA, hi: floa
             lo_p[k] = np.where(same, mid[k], lo_p[k])
             hi_p[k] = np.where(same, hi_p[k], mid[k])
     return [np.where(f, np.sqrt(a * b), np.nan) for f, a, b in zip(found, lo_p, hi_p)]
+
+
+def warm_start_hints(prev: dict, z: np.ndarray, nhint: int = 2) -> np.ndarray:
+    """Warm-start hints [nhint, nc, ncell] from a previous flash result of the same cells
+    (host dict of SoA arrays): hint h is the composition of published phase h. A hint is
+    absent (NaN) where that phase was not published, or where its composition does not
+    have exactly the feed's support."""
+    nc, n = z.shape
+    hints = np.full((nhint, nc, n), np.nan)
+    for h in range(nhint):
+        have = prev["phase_count"] > h
+        x = prev["x"][h]
+        ok = have & np.all(np.isfinite(x), axis=0) & np.all((x > 0) == (z > 0), axis=0)
+        hints[h][:, ok] = x[:, ok]
+    return hints

@@ -852,8 +852,12 @@ struct VleResult {
 };
 
 // solve_pt_vle, pt_split.hpp:414-525 (with Pr76VleEvaluator as both providers).
+// starts: extra starts of the initial stability test; final_starts (nfinal >= 0): extra
+// starts of the common-tangent review (pt_split.hpp:414-512 initial_starts / final_starts),
+// nfinal < 0: the initial starts again.
 void solve_vle(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
-               const double* const* starts, int nstarts, VleResult& V, Counters& cnt) {
+               const double* const* starts, int nstarts, VleResult& V, Counters& cnt,
+               const double* const* final_starts = nullptr, int nfinal = -1) {
     const int n = m.n;
     if (!std::isfinite(p) || p <= 0 || !std::isfinite(t) || t <= 0) { throw Rejected{}; }
     (void)check_composition(feed, n);
@@ -910,7 +914,11 @@ void solve_vle(const Mixture& m, double p, double t, const double* feed, const S
     if (!std::isfinite(dg) || !std::isfinite(guard) || !std::isfinite(fo.tpd_tol)) { return; }
     const double* fs[6];
     int nf = 0;
-    for (int e = 0; e < nstarts; ++e) { fs[nf++] = starts[e]; }
+    if (nfinal < 0) {
+        for (int e = 0; e < nstarts; ++e) { fs[nf++] = starts[e]; }
+    } else {
+        for (int e = 0; e < nfinal; ++e) { fs[nf++] = final_starts[e]; }
+    }
     fs[nf++] = c.fr.x;
     fs[nf++] = c.fr.y;
     stability(m, p, t, z, c.common, fs, nf, fo, V.fin, cnt);
@@ -1201,12 +1209,14 @@ bool log_ratio(int n, const double* num, const double* den, const double* z, dou
 }
 
 // solve_pr76_pt_max3, pr76_three_phase.hpp:328-561 (no caller continuation hints).
+// initial: extra starts of the initial stability test (solve_pr76_pt_max3 initial_starts,
+// pr76_three_phase.hpp:328-341).
 void solve_max3(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
-                Max3Result& M, Counters& cnt) {
+                Max3Result& M, Counters& cnt, const double* const* initial = nullptr, int ninitial = 0) {
     const int n = m.n;
     (void)check_composition(feed, n);
     M.use_neighbor = false;
-    solve_vle(m, p, t, feed, so, nullptr, 0, M.base, cnt);
+    solve_vle(m, p, t, feed, so, initial, ninitial, M.base, cnt, nullptr, 0);
     switch (M.base.status) {
     case V_SINGLE: M.status = M_SINGLE; return;
     case V_TWO: M.status = M_TWO; return;
@@ -2270,11 +2280,32 @@ void parallel_cells(int64_t ncell, int threads, Body body) {
 
 using namespace cpuref;
 
+extern "C" int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
+                                         const double* kij, int64_t ncell, const double* p, const double* t,
+                                         const double* z, int nhint, const double* hints,
+                                         double fugacity_tolerance, int max_phases, int threads, int32_t* status,
+                                         int32_t* phase_count, double* beta, double* zfactor, int32_t* branch,
+                                         double* x, double* ln_phi, uint32_t* counters);
+
 extern "C" int cpuref_flash_batch(int nc, const double* tc, const double* pc, const double* omega,
                                   const double* kij, int64_t ncell, const double* p, const double* t,
                                   const double* z, double fugacity_tolerance, int max_phases, int threads,
                                   int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
                                   int32_t* branch, double* x, double* ln_phi, uint32_t* counters) {
+    return cpuref_flash_batch_hinted(nc, tc, pc, omega, kij, ncell, p, t, z, 0, nullptr, fugacity_tolerance,
+                                     max_phases, threads, status, phase_count, beta, zfactor, branch, x, ln_phi,
+                                     counters);
+}
+
+// Warm-start hints: hints[(h*nc + i)*ncell + c], h < nhint, appended to the initial stability
+// starts (Pr76PtFlashBackendOptions::initial_starts); NaN in component 0: no hint h for cell c.
+extern "C" int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
+                                         const double* kij, int64_t ncell, const double* p, const double* t,
+                                         const double* z, int nhint, const double* hints,
+                                         double fugacity_tolerance, int max_phases, int threads, int32_t* status,
+                                         int32_t* phase_count, double* beta, double* zfactor, int32_t* branch,
+                                         double* x, double* ln_phi, uint32_t* counters) {
+    if (nhint < 0 || nhint > 2 || (nhint > 0 && !hints)) { return 1; }
     if (nc < 1 || nc > NCMAX || ncell < 0 || (max_phases != 2 && max_phases != 3)) { return 1; }
     const Mixture m = make_mixture(nc, tc, pc, omega, kij);
     const Out o{status, phase_count, beta, zfactor, branch, x, ln_phi, counters};
@@ -2283,14 +2314,23 @@ extern "C" int cpuref_flash_batch(int nc, const double* tc, const double* pc, co
     parallel_cells(ncell, threads, [&](int64_t c) {
         double feed[NCMAX];
         for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
+        double hw[2][NCMAX];
+        const double* hs[2];
+        int nh = 0;
+        for (int h = 0; h < nhint; ++h) {
+            if (std::isnan(hints[(h * nc + 0) * ncell + c])) { continue; }
+            for (int i = 0; i < nc; ++i) { hw[nh][i] = hints[(h * nc + i) * ncell + c]; }
+            hs[nh] = hw[nh];
+            ++nh;
+        }
         Counters cnt;
         auto M = std::make_unique<Max3Result>();
         try {
             if (max_phases == 2) {
-                solve_vle(m, p[c], t[c], feed, so, nullptr, 0, M->base, cnt);
+                solve_vle(m, p[c], t[c], feed, so, hs, nh, M->base, cnt, nullptr, 0);
                 publish_vle(o, nc, ncell, c, M->base, 0xF, cnt);
             } else {
-                solve_max3(m, p[c], t[c], feed, so, *M, cnt);
+                solve_max3(m, p[c], t[c], feed, so, *M, cnt, hs, nh);
                 publish_max3(o, nc, ncell, c, *M, cnt);
             }
         } catch (const Rejected&) {

@@ -19,6 +19,8 @@ _P = ctypes.c_void_p
 _FLASH_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_double, ctypes.c_int,
                ctypes.c_int] + [_P] * 8
 _PHASE_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P] + [_P] * 5
+_FLASH_HINT_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P, ctypes.c_double,
+                    ctypes.c_int, ctypes.c_int] + [_P] * 8
 
 
 def _ptr(a: np.ndarray):
@@ -30,6 +32,7 @@ def _load(path: Path, prefix: str):
         return None
     lib = ctypes.CDLL(str(path))
     getattr(lib, prefix + "_flash_batch").argtypes = _FLASH_ARGS
+    getattr(lib, prefix + "_flash_batch_hinted").argtypes = _FLASH_HINT_ARGS
     getattr(lib, prefix + "_phase_batch").argtypes = _PHASE_ARGS
     return lib
 
@@ -54,8 +57,10 @@ def _mix_arrays(mix):
             np.ascontiguousarray(mix["kij"], dtype=np.float64))
 
 
-def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which="cpuref"):
-    """Run the checker flash on host arrays; returns a dict of SoA arrays."""
+def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which="cpuref", hints=None):
+    """Run the checker flash on host arrays; returns a dict of SoA arrays. ``hints``
+    ([nhint <= 2, nc, ncell], NaN in component 0 = absent) are warm-start compositions
+    appended to the initial stability starts."""
     L = lib(which)
     if L is None:
         raise RuntimeError(f"checker library '{which}' is not built")
@@ -74,10 +79,20 @@ def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which
         "ln_phi": np.zeros((3, nc, n)),
         "counters": np.zeros((4, n), np.uint32),
     }
-    fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch")
-    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), float(fugacity_tolerance),
-            int(max_phases), int(threads), _ptr(out["status"]), _ptr(out["phase_count"]), _ptr(out["beta"]),
-            _ptr(out["zfactor"]), _ptr(out["branch"]), _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]))
+    if hints is not None:
+        hints = np.ascontiguousarray(hints, dtype=np.float64)
+        assert hints.ndim == 3 and hints.shape[1:] == (nc, n) and hints.shape[0] <= 2
+        fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_hinted")
+        rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), int(hints.shape[0]),
+                _ptr(hints), float(fugacity_tolerance), int(max_phases), int(threads), _ptr(out["status"]),
+                _ptr(out["phase_count"]), _ptr(out["beta"]), _ptr(out["zfactor"]), _ptr(out["branch"]),
+                _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]))
+    else:
+        fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch")
+        rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z),
+                float(fugacity_tolerance), int(max_phases), int(threads), _ptr(out["status"]),
+                _ptr(out["phase_count"]), _ptr(out["beta"]), _ptr(out["zfactor"]), _ptr(out["branch"]),
+                _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]))
     if rc != 0:
         raise RuntimeError(f"{which}_flash_batch failed rc={rc}")
     return out

@@ -183,3 +183,38 @@ def test_mixed_precision_three_phase_sweep(gpu):
     print("mixed sour", rep, flush=True)
     assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
     assert np.sum((a["status"] >> 4 & 0xF) == 2) > 100
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_warm_start_hints(gpu, precision):
+    """mpmc_flash_batch_hinted: the phase compositions of a flash at 3% lower pressure as
+    warm-start hints. FP64 matches the oracle's hinted flash at the usual bar (hints add
+    stability trials, counted identically); mixed precision at the mixed bar."""
+    from mpmc_hnu_amd import flash
+
+    nc = 6
+    mix = W.mixture_linspaced(nc)
+    p, t, z = W.cells_uniform(nc, 2048, seed=77)
+    prev = ob.flash(mix, p * 0.97, t, z, 1e-11, 3)
+    hints = W.warm_start_hints(prev, z)
+    print(f"[case] hinted flash precision={precision}, {int(np.isfinite(hints[0, 0]).sum())} hinted cells", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        g = flash.to_host(flash.flash_batch(m, p, t, z, precision=precision, hints=hints))
+    finally:
+        m.close()
+    o = ob.flash(mix, p, t, z, 1e-11, 3, hints=hints)
+    rel = 1e-9 if precision == 0 else 1e-8
+    rep, bad = parity.compare(g, o, rel=rel, abs_lnphi=rel)
+    print("hinted", rep, flush=True)
+    assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
+    assert rep["value_mismatch"] <= MAX_LOOSE * p.size, rep
+    res = parity.fugacity_residual(g)
+    acc = (g["status"] & 0xF) == 0
+    assert np.all(res[acc] <= 1e-10), float(res[acc].max())
+    if precision == 0:
+        same = ~bad
+        for k in range(2):
+            a, b = g["counters"][k, same].sum(dtype=np.float64), o["counters"][k, same].sum(dtype=np.float64)
+            assert abs(a - b) <= 0.01 * b, (k, a, b)

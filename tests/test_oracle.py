@@ -192,3 +192,30 @@ def test_planted_mistake_is_caught():
     wrong = ob.flash(bad_mix, p, t, z, 1e-11, 3)
     rep, bad = parity.compare(wrong, good)
     assert bad.any() and rep["value_mismatch"] + rep["status_mismatch"] > 0
+
+
+@pytest.mark.skipif(not ob.have_ref(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("nc", [3, 6])
+def test_warm_start_hints_match_reference(nc):
+    """Warm starts (PtSolveHints::initial_stability_starts through the backend's
+    initial_starts): the phase compositions of a flash at 3% lower pressure, handed to the
+    flash of the same feeds. The oracle and the compiled reference agree bit for bit, the
+    hints add stability trials, and a hint with the wrong support rejects the cell."""
+    mix = W.mixture_linspaced(nc, seed=nc + 60)
+    p, t, z = W.cells_uniform(nc, 384, seed=nc + 70)
+    prev = ob.flash(mix, p * 0.97, t, z, 1e-11, 3, which="ref")
+    hints = W.warm_start_hints(prev, z)
+    assert np.isfinite(hints[1, 0]).sum() > 10
+    bad_cell = int(np.nonzero(np.isfinite(hints[0, 0]))[0][0])
+    hints[0, :, bad_cell] = 0.0
+    hints[0, 0, bad_cell] = 1.0  # support differs from the feed's
+    o = ob.flash(mix, p, t, z, 1e-11, 3, hints=hints)
+    r = ob.flash(mix, p, t, z, 1e-11, 3, which="ref", hints=hints)
+    rep, bad = parity.compare(o, r, rel=0.0, abs_lnphi=0.0)
+    assert not bad.any(), rep
+    assert np.array_equal(o["counters"][:2], r["counters"][:2])
+    assert o["status"][bad_cell] & 0x2000 and r["status"][bad_cell] & 0x2000
+    plain = ob.flash(mix, p, t, z, 1e-11, 3)
+    hinted = np.isfinite(hints[0, 0])
+    hinted[bad_cell] = False
+    assert np.all(o["counters"][1][hinted] > plain["counters"][1][hinted])
