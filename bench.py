@@ -273,7 +273,7 @@ def serve_leg(seconds: float) -> dict:
     mixtures = {nc: W.mixture_linspaced(nc) for nc in range(2, 11)}
     out = {"arrival_rate_per_s": 200.0, "arrival_seconds": seconds, "jobs": len(jobs),
            "cells": int(sum(j["p"].size for j in jobs)), "small_job_cells": 65536,
-           "warm_starts": "not modelled (no continuation hints in this build)"}
+           "warm_start_jobs": int(sum("warm_from" in j for j in jobs))}
     for prec, name in ((0, "fp64"), (1, "mixed")):
         srv = S.Server(slots=2, precision=prec)
         t0 = time.perf_counter()
@@ -282,6 +282,7 @@ def serve_leg(seconds: float) -> dict:
         srv.close()
         n = np.array([r[0] for r in res])
         lat = np.array([r[1] for r in res]) * 1e3
+        warm = np.array([r[2] for r in res])
         small = n <= 65536
         out[name] = {
             "p50_ms_small": float(np.percentile(lat[small], 50)),
@@ -290,6 +291,7 @@ def serve_leg(seconds: float) -> dict:
             "p99_ms_all": float(np.percentile(lat, 99)),
             "cells_per_s": float(n.sum() / makespan),
             "makespan_s": makespan,
+            "warm_started_jobs": int(warm.sum()),
             "slots": srv.slots,
         }
         print(f"[bench] serving {name}: p50/p99 small {out[name]['p50_ms_small']:.2f}/{out[name]['p99_ms_small']:.2f} ms, "

@@ -418,6 +418,13 @@ int mpmc_server_submit(mpmc_server_t server, int nc, const double* tc_k, const d
                        const double* omega, const double* kij, int64_t ncell, const double* p,
                        const double* t, const double* z, const mpmc_host_outputs* out,
                        int64_t* job_id);
+/* mpmc_server_submit with warm-start hints (host, [nhint][nc][ncell], nhint <= 2, read until
+ * the job is done): the per-request PtSolveHints initial_stability_starts, as in
+ * mpmc_flash_batch_hinted. */
+int mpmc_server_submit_hinted(mpmc_server_t server, int nc, const double* tc_k, const double* pc_pa,
+                              const double* omega, const double* kij, int64_t ncell, const double* p,
+                              const double* t, const double* z, int nhint, const double* hints,
+                              const mpmc_host_outputs* out, int64_t* job_id);
 /* Block until the job is done; latency_s = completion - submission (seconds). */
 int mpmc_server_wait(mpmc_server_t server, int64_t job_id, double* latency_s);
 int mpmc_server_poll(mpmc_server_t server, int64_t job_id, int32_t* done, double* latency_s);

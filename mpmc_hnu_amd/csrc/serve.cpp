@@ -37,6 +37,7 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -49,6 +50,8 @@ struct Job {
     mpmc_mixture_t mixture;
     int64_t n;
     const double *p, *t, *z;
+    int nhint = 0;                 // warm-start hints [nhint][nc][n] (host), nhint <= 2
+    const double* hints = nullptr;
     mpmc_host_outputs out;
     Clock::time_point submitted;
     int64_t next_cell = 0;   // first cell not yet issued
@@ -75,13 +78,14 @@ struct Slot {
 
 // Chunk buffer layout for `len` cells of an nc-component mixture (doubles then ints).
 struct ChunkLayout {
-    size_t p, t, z, beta, zf, x, lp, status, count, branch, total;
+    size_t p, t, z, hint, beta, zf, x, lp, status, count, branch, total;
     ChunkLayout(int nc, int64_t len) {
         size_t o = 0;
         auto take = [&](size_t b) { const size_t at = o; o += (b + 255) / 256 * 256; return at; };
         p = take(8 * len);
         t = take(8 * len);
         z = take(8 * nc * len);
+        hint = take(8 * 2 * nc * len);   // warm-start hints [2][nc][len], NaN rows where absent
         beta = take(8 * 3 * len);
         zf = take(8 * 3 * len);
         x = take(8 * 3 * nc * len);
@@ -137,6 +141,9 @@ struct mpmc_server {
         char* h = static_cast<char*>(s.host);
         char* d = static_cast<char*>(s.dev);
         int64_t at = 0;
+        bool hinted = false;
+        for (const auto& g : s.segs) { hinted = hinted || g.job->nhint > 0; }
+        const double nan = std::numeric_limits<double>::quiet_NaN();
         for (const auto& g : s.segs) {
             const auto& j = g.job;
             std::memcpy(h + L.p + 8 * at, j->p + g.c0, 8 * g.len);
@@ -144,9 +151,22 @@ struct mpmc_server {
             for (int i = 0; i < nc; ++i) {
                 std::memcpy(h + L.z + 8 * (i * len + at), j->z + i * j->n + g.c0, 8 * g.len);
             }
+            if (hinted) {
+                for (int k = 0; k < 2; ++k) {
+                    if (k < j->nhint) {
+                        for (int i = 0; i < nc; ++i) {
+                            std::memcpy(h + L.hint + 8 * ((k * nc + i) * len + at), j->hints + (k * nc + i) * j->n + g.c0,
+                                        8 * g.len);
+                        }
+                    } else {
+                        double* row = reinterpret_cast<double*>(h + L.hint) + (k * nc + 0) * len + at;
+                        for (int64_t c = 0; c < g.len; ++c) { row[c] = nan; }
+                    }
+                }
+            }
             at += g.len;
         }
-        const size_t in_bytes = L.beta; // p, t, z are the leading regions
+        const size_t in_bytes = hinted ? L.beta : L.hint; // p, t, z (and hints) are the leading regions
         hipError_t e = hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) { fail_msg = std::string("copy in: ") + hipGetErrorString(e); return false; }
         mpmc_flash_options fo;
@@ -158,9 +178,15 @@ struct mpmc_server {
                                      reinterpret_cast<double*>(d + L.beta), reinterpret_cast<double*>(d + L.zf),
                                      reinterpret_cast<int32_t*>(d + L.branch), reinterpret_cast<double*>(d + L.x),
                                      reinterpret_cast<double*>(d + L.lp), nullptr};
-        if (mpmc_flash_batch_ws(mixture, len, reinterpret_cast<double*>(d + L.p), reinterpret_cast<double*>(d + L.t),
-                                reinterpret_cast<double*>(d + L.z), &fo, &out, workspace, workspace_bytes,
-                                stream) != MPMC_OK) {
+        const int rc =
+            hinted ? mpmc_flash_batch_hinted(mixture, len, reinterpret_cast<double*>(d + L.p),
+                                             reinterpret_cast<double*>(d + L.t), reinterpret_cast<double*>(d + L.z), 2,
+                                             reinterpret_cast<double*>(d + L.hint), &fo, &out, workspace,
+                                             workspace_bytes, stream)
+                   : mpmc_flash_batch_ws(mixture, len, reinterpret_cast<double*>(d + L.p),
+                                         reinterpret_cast<double*>(d + L.t), reinterpret_cast<double*>(d + L.z), &fo,
+                                         &out, workspace, workspace_bytes, stream);
+        if (rc != MPMC_OK) {
             fail_msg = std::string("flash: ") + mpmc_last_error();
             return false;
         }
@@ -370,7 +396,16 @@ int mpmc_server_slots(mpmc_server_t s) { return s ? static_cast<int>(s->slots.si
 int mpmc_server_submit(mpmc_server_t s, int nc, const double* tc, const double* pc, const double* omega,
                        const double* kij, int64_t ncell, const double* p, const double* t, const double* z,
                        const mpmc_host_outputs* out, int64_t* job_id) {
+    return mpmc_server_submit_hinted(s, nc, tc, pc, omega, kij, ncell, p, t, z, 0, nullptr, out, job_id);
+}
+
+int mpmc_server_submit_hinted(mpmc_server_t s, int nc, const double* tc, const double* pc, const double* omega,
+                              const double* kij, int64_t ncell, const double* p, const double* t, const double* z,
+                              int nhint, const double* hints, const mpmc_host_outputs* out, int64_t* job_id) {
     if (!s || !out || !job_id || !p || !t || !z) { return server_fail(MPMC_E_INVALID_ARGUMENT, "server_submit: null argument"); }
+    if (nhint < 0 || nhint > 2 || (nhint > 0 && !hints)) {
+        return server_fail(MPMC_E_INVALID_ARGUMENT, "server_submit: nhint must be 0..2 with host hints");
+    }
     if (nc < 1 || nc > MPMC_MAX_COMPONENTS || ncell < 1 || ncell > (int64_t{1} << 31)) {
         return server_fail(MPMC_E_INVALID_ARGUMENT, "server_submit: nc or ncell out of range");
     }
@@ -408,6 +443,8 @@ int mpmc_server_submit(mpmc_server_t s, int nc, const double* tc, const double* 
     j->p = p;
     j->t = t;
     j->z = z;
+    j->nhint = nhint;
+    j->hints = hints;
     j->out = *out;
     j->small = ncell <= s->opt.small_job_cells;
     {

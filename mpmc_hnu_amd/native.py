@@ -53,6 +53,7 @@ EXPORTED = (
     "mpmc_server_slots",
     "mpmc_server_device_bytes",
     "mpmc_server_submit",
+    "mpmc_server_submit_hinted",
     "mpmc_server_wait",
     "mpmc_server_poll",
     "mpmc_server_last_error",
@@ -219,6 +220,8 @@ def library() -> ctypes.CDLL:
     lib.mpmc_server_device_bytes.restype = ctypes.c_size_t
     lib.mpmc_server_submit.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int64, P, P, P,
                                        ctypes.POINTER(HostOutputs), ctypes.POINTER(ctypes.c_int64)]
+    lib.mpmc_server_submit_hinted.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_int64, P, P, P, ctypes.c_int, P,
+                                              ctypes.POINTER(HostOutputs), ctypes.POINTER(ctypes.c_int64)]
     lib.mpmc_server_wait.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
     lib.mpmc_server_poll.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]
     lib.mpmc_server_last_error.restype = ctypes.c_char_p

@@ -4,6 +4,7 @@ large ones."""
 import numpy as np
 import pytest
 
+import oracle_bindings as ob
 from mpmc_hnu_amd import workloads as W
 
 KEYS = ("status", "phase_count", "beta", "zfactor", "branch", "x", "ln_phi")
@@ -65,3 +66,37 @@ def test_small_jobs_overtake_a_large_one(gpu):
     worst_small = max(s.latency_s for s in small)
     print(f"  big {big.latency_s * 1e3:.1f} ms, worst small {worst_small * 1e3:.1f} ms", flush=True)
     assert worst_small < 0.5 * big.latency_s
+
+
+@pytest.mark.gpu
+def test_warm_started_jobs_alone_and_in_the_mix(gpu):
+    """Jobs with warm-start hints (the phase compositions of the same cells at a shifted
+    pressure) and jobs without share launches; every job's records equal the hinted (or
+    plain) flash of that job alone."""
+    import torch
+
+    from mpmc_hnu_amd import flash, serving as S
+
+    mixtures = {nc: W.mixture_linspaced(nc) for nc in (4, 6)}
+    jobs = []
+    for k in range(12):
+        nc = (4, 6)[k % 2]
+        p, t, z = W.cells_uniform(nc, 700 + 311 * k, seed=600 + k)
+        hints = W.warm_start_hints(ob.flash(mixtures[nc], p * 0.98, t, z, 1e-11, 3), z) if k % 3 else None
+        jobs.append((nc, p, t, z, hints))
+    print(f"[case] server mix of {len(jobs)} jobs, {sum(j[4] is not None for j in jobs)} warm-started", flush=True)
+    srv = S.Server(slots=2, chunk_cells=8192, small_job_cells=4096)
+    try:
+        handles = [srv.submit(mixtures[nc], p, t, z, hints=h) for nc, p, t, z, h in jobs]
+        results = [h.wait() for h in handles]
+    finally:
+        srv.close()
+    for (nc, p, t, z, hints), r in zip(jobs, results):
+        m = flash.Mixture.from_dict(mixtures[nc])
+        try:
+            a = flash.to_host(flash.flash_batch(m, p, t, z, hints=hints, counters=False))
+        finally:
+            m.close()
+        torch.cuda.synchronize()
+        for k in KEYS:
+            assert np.array_equal(r[k], a[k], equal_nan=True), (nc, p.size, k, hints is not None)
