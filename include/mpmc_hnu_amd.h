@@ -320,6 +320,8 @@ int mpmc_tpfa_grid_destroy(mpmc_tpfa_grid_t grid);
 /* The grid's cell -> face CSR: adj_off[ncell+1], adj_face[2 nface] (device memory); a
  * cell's faces are in face order. */
 int mpmc_tpfa_grid_adjacency(mpmc_tpfa_grid_t grid, const int32_t** adj_off, const int32_t** adj_face);
+/* Number of cells of the grid. */
+int mpmc_tpfa_grid_cells(mpmc_tpfa_grid_t grid, int64_t* ncell);
 /* The largest number of faces of one cell: the slot count of the off-diagonal blocks. The
  * reference's DMPlex adjacency (two_phase_snes_assembly.hpp:1719-1784 preallocation) has no
  * such call; it sizes the ELL layout here. */
@@ -370,6 +372,69 @@ int mpmc_assembly_plan_launch(mpmc_assembly_plan_t plan, void* stream);
 int mpmc_assembly_plan_domain_error(mpmc_assembly_plan_t plan, void* stream, int32_t* domain_error);
 /* Waits for the device, then frees the plan. */
 int mpmc_assembly_plan_destroy(mpmc_assembly_plan_t plan);
+
+/* ---------------------------------------------------------------- phase-transition scan
+ * The post-SNES PT-flash phase-transition scanner
+ * (flow_discretization_petsc/post_snes_pt_flash_phase_transition_scanner.hpp): one flash
+ * per cell at its current state, compared with the cell's current phase count.
+ *
+ * Per-cell code (scan_post_snes_pt_flash_source_cell_3d, :203-301):
+ *   MPMC_SCAN_SAME           accepted phase set of the source's size: no proposal
+ *   MPMC_SCAN_PROPOSAL       accepted set of another size: a transition proposal, with
+ *                            target_count phases (the flash record holds their beta, x, Z,
+ *                            ln phi) and target phase molar densities p / (Z R T)
+ *   MPMC_SCAN_INDETERMINATE  no accepted phase set, a target density that is not finite and
+ *                            positive, or (two-phase state scan) a cell whose state or
+ *                            inventory cannot define a PT feed
+ *   MPMC_SCAN_ERROR          a malformed source or an unsupported source phase count
+ *                            (invalid_argument / PETSC_ERR_ARG_INCOMP), or a feed the flash
+ *                            rejects
+ * The reference walks the cells in order and stops at the first indeterminate cell (scan
+ * status indeterminate, proposals dropped) or the first error (the error returned). The
+ * summary reports that outcome: first_stop = index of the first such cell (-1: none),
+ * status = MPMC_SCAN_STATUS_INDETERMINATE when that cell is indeterminate, and the call
+ * returns MPMC_E_INVALID_ARGUMENT when it is an error. Every cell's code is written either
+ * way. The transition evidence report and the trigger of the reference's candidate
+ * (phase_set_transition_flash_adapter.hpp:132-220) are not produced; the caller builds the
+ * candidate from the record. The call synchronises the stream to fill the summary. */
+enum mpmc_scan_code { MPMC_SCAN_SAME = 0, MPMC_SCAN_PROPOSAL = 1, MPMC_SCAN_INDETERMINATE = 2,
+                      MPMC_SCAN_ERROR = 3 };
+/* PostSnesPhaseTransitionScanStatus3D (post_snes_phase_transition_controller.hpp:78-81) */
+enum mpmc_scan_status { MPMC_SCAN_COMPLETE = 0, MPMC_SCAN_STATUS_INDETERMINATE = 1 };
+
+typedef struct mpmc_scan_outputs {
+    int32_t* code;           /* [ncell] mpmc_scan_code */
+    int32_t* target_count;   /* [ncell] accepted phase count, 0 without an accepted set */
+    double* target_density;  /* [3][ncell] mol/m3 of the target phases of a proposal, else NaN */
+} mpmc_scan_outputs;
+
+typedef struct mpmc_scan_summary {
+    int32_t status;          /* mpmc_scan_status */
+    int64_t first_stop;      /* first indeterminate or error cell, -1 when none */
+    int64_t proposals;       /* cells with MPMC_SCAN_PROPOSAL */
+} mpmc_scan_summary;
+
+/* Scan of caller-given PT sources (PostSnesPtFlashSourceCellSnapshot3D, :30-37): p, t,
+ * z[nc][ncell] and source_phase_count[ncell] (1..3) on the device; the flash runs with
+ * `options` and writes its records to `out`. Source validation is
+ * validate_source_snapshot (:76-115). Replaces the per-cell
+ * scan_post_snes_pt_flash_source_cell_3d call. */
+int mpmc_transition_scan(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t, const double* z,
+                         const int32_t* source_phase_count, const mpmc_flash_options* options,
+                         const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan, mpmc_scan_summary* summary,
+                         void* stream);
+
+/* Scan of a converged two-phase state q[Q][ncell] (the assembly's natural variables) on
+ * `grid`: each cell's source is its current pore-volume inventory (mpmc_inventory_2p, into
+ * inventory[nc+1][ncell]) normalised to z_i = N_i / sum N (into feed[nc][ncell]) at the
+ * cell's pressure q[0] and temperature q[1], with source phase count 2
+ * (make_source_snapshot, :160-200). Replaces scan_post_snes_pt_flash_phase_transitions_3d
+ * (:303-423) for the owned cells of a two-phase system; the caller checks that SNES
+ * converged. */
+int mpmc_transition_scan_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
+                            const double* q, const int32_t* branch, double* inventory, double* feed,
+                            const mpmc_flash_options* options, const mpmc_flash_outputs* out,
+                            const mpmc_scan_outputs* scan, mpmc_scan_summary* summary, void* stream);
 
 /* ---------------------------------------------------------------- serving
  * Continuous-batching flash server: the PT service request loop (reference

@@ -50,6 +50,10 @@ bool is_device_ptr(const void* ptr) {
 
 } // namespace
 
+namespace mpmc::api {
+int record_error(int code, const std::string& what) { return fail(code, what); }
+} // namespace mpmc::api
+
 // Library-owned device residency of one prepared mixture.
 struct mpmc_mixture {
     int nc;
@@ -889,6 +893,12 @@ int mpmc_tpfa_grid_adjacency(mpmc_tpfa_grid_t g, const int32_t** adj_off, const 
     if (!g || !adj_off || !adj_face) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_adjacency: null argument"); }
     *adj_off = g->adj_off;
     *adj_face = g->adj_face;
+    return MPMC_OK;
+}
+
+int mpmc_tpfa_grid_cells(mpmc_tpfa_grid_t g, int64_t* ncell) {
+    if (!g || !ncell) { return fail(MPMC_E_INVALID_ARGUMENT, "tpfa_grid_cells: null argument"); }
+    *ncell = g->ncell;
     return MPMC_OK;
 }
 

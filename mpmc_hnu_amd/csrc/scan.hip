@@ -1,0 +1,263 @@
+// Post-SNES phase-transition scan: the simulator-side batch caller of the flash
+// (modules/flow/discretization/petsc/include/mpmc/flow_discretization_petsc/
+//  post_snes_pt_flash_phase_transition_scanner.hpp).
+//
+// The reference walks the owned cells one by one (scan_post_snes_pt_flash_phase_transitions_3d,
+// :303-423): it rebuilds each cell's PT feed from its current component inventory
+// (make_source_snapshot, :160-200), solves one flash per cell through the backend and
+// compares the accepted phase count with the cell's current one
+// (scan_post_snes_pt_flash_source_cell_3d, :203-301). Here the whole batch is three
+// cell-parallel steps on one stream: source (feed and validation), the staged flash of
+// mpmc_flash_batch, decision. The reference stops at the first cell that is indeterminate
+// or raises an error; the batch computes every cell and reduces the lowest such cell
+// index, so the caller sees the same outcome (the summary) plus every cell's code.
+#include "../../include/mpmc_hnu_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace mpmc::api {
+// The library's error record (mpmc_last_error), flash_api.hip.
+int record_error(int code, const std::string& what);
+} // namespace mpmc::api
+
+namespace {
+
+// Universal gas constant of the PR76 provider (the mixture's r_gas).
+constexpr double kGasConstant = static_cast<double>(8.31446261815324L);
+constexpr double kEps = 2.220446049250313e-16;
+
+__device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
+
+// Sequential compensated sum (hi + lo) standing in for the reference's long double
+// accumulation.
+__device__ __forceinline__ void dd_add(double v, double& hi, double& lo) {
+    const double s = hi + v;
+    const double bb = s - hi;
+    lo += (hi - (s - bb)) + (v - bb);
+    hi = s;
+}
+
+// validate_source_snapshot (:76-115): positive pressure and temperature, a phase count in
+// [1, 3], strictly positive finite fractions whose sum is within 4096 eps * nc of one.
+__device__ bool source_valid(int nc, double p, double t, int32_t count, const double* z, int64_t n, int64_t c) {
+    if (count < 1 || count > MPMC_MAX_PHASES || !finite(p) || !(p > 0) || !finite(t) || !(t > 0)) { return false; }
+    double hi = 0, lo = 0;
+    for (int i = 0; i < nc; ++i) {
+        const double v = z[i * n + c];
+        if (!finite(v) || !(v > 0)) { return false; }
+        dd_add(v, hi, lo);
+    }
+    const double dev = (hi - 1.0) + lo;
+    return finite(hi) && __builtin_fabs(dev) <= 4096.0 * kEps * static_cast<double>(nc);
+}
+
+// Source of a caller-given batch: a malformed source, or a phase count the backend does
+// not support (max_phases), raises invalid_argument / returns PETSC_ERR_ARG_INCOMP in the
+// reference's per-cell scan (:217-227), code 3.
+__global__ void k_scan_source(int nc, int64_t n, const double* __restrict__ p, const double* __restrict__ t,
+                              const double* __restrict__ z, const int32_t* __restrict__ count, int max_phases,
+                              int32_t* __restrict__ code) {
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const bool ok = source_valid(nc, p[c], t[c], count[c], z, n, c) && count[c] <= max_phases;
+        code[c] = ok ? MPMC_SCAN_SAME : MPMC_SCAN_ERROR;
+    }
+}
+
+// make_source_snapshot (:160-200) of a two-phase cell: z_i = N_i / sum N from the current
+// pore-volume inventory, at the cell's pressure q[0] and temperature q[1]. The reference
+// divides in long double; here the quotient by the compensated total, corrected once by
+// its fma residual. A cell whose state evaluation failed (code already 1 from the
+// inventory pass), whose inventory is not finite and positive, or whose snapshot fails
+// validation is indeterminate (the reference catches these and reports indeterminate),
+// code 2; its feed is NaN.
+__global__ void k_scan_source_2p(int nc, int64_t n, const double* __restrict__ q, const double* __restrict__ inv,
+                                 double* __restrict__ feed, int32_t* __restrict__ code) {
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        bool ok = code[c] == 0;
+        double hi = 0, lo = 0;
+        for (int i = 0; i < nc; ++i) {
+            const double v = inv[i * n + c];
+            ok = ok && finite(v) && v > 0;
+            dd_add(v, hi, lo);
+        }
+        ok = ok && finite(hi) && hi > 0;
+        for (int i = 0; i < nc; ++i) {
+            double zi = __builtin_nan("");
+            if (ok) {
+                const double v = inv[i * n + c];
+                const double q0 = v / hi;
+                const double r = __builtin_fma(-q0, hi, v) - q0 * lo;
+                zi = q0 + r / hi;
+            }
+            feed[i * n + c] = zi;
+        }
+        ok = ok && source_valid(nc, q[c], q[n + c], 2, feed, n, c);
+        code[c] = ok ? MPMC_SCAN_SAME : MPMC_SCAN_INDETERMINATE;
+    }
+}
+
+// Decision (:236-292) on the flash record. A cell already marked (2, 3) keeps its code.
+// The flash rejecting the input is the reference's invalid_argument (code 3); no accepted
+// phase set is indeterminate; an accepted set of the source's size is no transition;
+// otherwise a proposal whose target phase molar densities are p / (Z R T) of the
+// accepted phases, the density resolver of the reference's scanner test
+// (post_snes_pt_flash_phase_transition_scanner_test.cpp:26-57): a density that is not
+// finite and positive leaves the cell indeterminate.
+// stop[0] / stop[1]: lowest index of an indeterminate / error cell; stop[2]: proposals.
+__global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const double* __restrict__ t,
+                              const int32_t* __restrict__ count, int32_t count_all,
+                              const int32_t* __restrict__ status, const int32_t* __restrict__ phase_count,
+                              const double* __restrict__ zfactor, int32_t* __restrict__ code,
+                              int32_t* __restrict__ target_count, double* __restrict__ density,
+                              unsigned long long* __restrict__ stop) {
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t src = count ? count[c] : count_all;
+        int32_t k = code[c];
+        int32_t target = 0;
+        double rho[MPMC_MAX_PHASES];
+        for (int ph = 0; ph < MPMC_MAX_PHASES; ++ph) { rho[ph] = __builtin_nan(""); }
+        if (k == MPMC_SCAN_SAME) {
+            const int32_t st = status[c];
+            if (st & MPMC_FLAG_REJECTED) {
+                k = MPMC_SCAN_ERROR;
+            } else if (MPMC_STATUS_SET(st) != MPMC_SET_ACCEPTED) {
+                k = MPMC_SCAN_INDETERMINATE;
+            } else {
+                target = phase_count[c];
+                if (target != src) {
+                    k = MPMC_SCAN_PROPOSAL;
+                    for (int ph = 0; ph < MPMC_MAX_PHASES; ++ph) {
+                        if (ph >= target) { continue; }
+                        const double zf = zfactor[ph * n + c];
+                        const double den = zf * kGasConstant * t[c];
+                        const double r = p[c] / den;
+                        if (!finite(zf) || !(zf > 0) || !finite(r) || !(r > 0)) { k = MPMC_SCAN_INDETERMINATE; }
+                        rho[ph] = r;
+                    }
+                }
+            }
+        }
+        code[c] = k;
+        target_count[c] = target;
+        for (int ph = 0; ph < MPMC_MAX_PHASES; ++ph) {
+            density[ph * n + c] = k == MPMC_SCAN_PROPOSAL ? rho[ph] : __builtin_nan("");
+        }
+        if (k == MPMC_SCAN_INDETERMINATE) { atomicMin(&stop[0], static_cast<unsigned long long>(c)); }
+        if (k == MPMC_SCAN_ERROR) { atomicMin(&stop[1], static_cast<unsigned long long>(c)); }
+        if (k == MPMC_SCAN_PROPOSAL) { atomicAdd(&stop[2], 1ull); }
+    }
+}
+
+__global__ void k_scan_init(unsigned long long* stop) {
+    if (threadIdx.x < 2) { stop[threadIdx.x] = ~0ull; }
+    if (threadIdx.x == 2) { stop[2] = 0; }
+}
+
+int scan_fail(int code, const std::string& what) { return mpmc::api::record_error(code, what); }
+
+unsigned grid_for(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return static_cast<unsigned>(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+// Flash, decision and the summary of the reference's stop-at-first rule.
+int scan_finish(mpmc_mixture_t mixture, int64_t n, const double* p, const double* t, const double* z,
+                const int32_t* count, int32_t count_all, const mpmc_flash_options* options,
+                const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan, mpmc_scan_summary* summary,
+                hipStream_t s) {
+    int rc = mpmc_flash_batch(mixture, n, p, t, z, options, out, s);
+    if (rc != MPMC_OK) { return rc; }
+    unsigned long long* stop = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&stop), 3 * sizeof(unsigned long long), s);
+    if (e != hipSuccess) { return scan_fail(MPMC_E_NO_MEMORY, "transition_scan: device allocation failed"); }
+    unsigned long long host[3] = {~0ull, ~0ull, 0};
+    hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, s, stop);
+    hipLaunchKernelGGL(k_scan_decide, dim3(grid_for(n)), dim3(256), 0, s, n, p, t, count, count_all, out->status,
+                       out->phase_count, out->zfactor, scan->code, scan->target_count, scan->target_density, stop);
+    e = hipGetLastError();
+    if (e == hipSuccess) { e = hipMemcpyAsync(host, stop, sizeof(host), hipMemcpyDeviceToHost, s); }
+    const hipError_t ef = hipFreeAsync(stop, s);
+    if (e == hipSuccess) { e = ef; }
+    if (e == hipSuccess) { e = hipStreamSynchronize(s); }
+    if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan: ") + hipGetErrorString(e)); }
+    // the reference returns at the first indeterminate cell (status indeterminate) or at the
+    // first cell that raises an error (the error code, status untouched)
+    const unsigned long long first = host[0] < host[1] ? host[0] : host[1];
+    summary->first_stop = first == ~0ull ? -1 : static_cast<int64_t>(first);
+    summary->proposals = static_cast<int64_t>(host[2]);
+    summary->status = host[0] < host[1] ? MPMC_SCAN_STATUS_INDETERMINATE : MPMC_SCAN_COMPLETE;
+    if (host[1] < host[0]) {
+        return scan_fail(MPMC_E_INVALID_ARGUMENT,
+                         "transition_scan: the first stopping cell has a malformed source or a rejected feed");
+    }
+    return MPMC_OK;
+}
+
+int scan_check(mpmc_mixture_t mixture, int64_t n, const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan,
+               const mpmc_scan_summary* summary) {
+    if (!mixture || !out || !scan || !summary || n < 0) {
+        return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: null argument or negative cell count");
+    }
+    if (mpmc_mixture_components(mixture) < 2) {
+        return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: a PT source needs at least two components");
+    }
+    if (n > 0 && (!scan->code || !scan->target_count || !scan->target_density || !out->status ||
+                  !out->phase_count || !out->zfactor)) {
+        return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: null output buffer");
+    }
+    return MPMC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int mpmc_transition_scan(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t, const double* z,
+                         const int32_t* source_phase_count, const mpmc_flash_options* options,
+                         const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan, mpmc_scan_summary* summary,
+                         void* stream) {
+    int rc = scan_check(mixture, ncell, out, scan, summary);
+    if (rc != MPMC_OK) { return rc; }
+    *summary = mpmc_scan_summary{MPMC_SCAN_COMPLETE, -1, 0};
+    if (ncell == 0) { return MPMC_OK; }
+    if (!p || !t || !z || !source_phase_count) { return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: null input"); }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const int max_phases = options ? options->max_phases : MPMC_MAX_PHASES;
+    hipLaunchKernelGGL(k_scan_source, dim3(grid_for(ncell)), dim3(256), 0, s, mpmc_mixture_components(mixture), ncell,
+                       p, t, z, source_phase_count, max_phases, scan->code);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan: ") + hipGetErrorString(e)); }
+    return scan_finish(mixture, ncell, p, t, z, source_phase_count, 0, options, out, scan, summary, s);
+}
+
+int mpmc_transition_scan_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
+                            const double* q, const int32_t* branch, double* inventory, double* feed,
+                            const mpmc_flash_options* options, const mpmc_flash_outputs* out,
+                            const mpmc_scan_outputs* scan, mpmc_scan_summary* summary, void* stream) {
+    int64_t ncell = 0;
+    int rc = mpmc_tpfa_grid_cells(grid, &ncell);
+    if (rc != MPMC_OK) { return rc; }
+    rc = scan_check(mixture, ncell, out, scan, summary);
+    if (rc != MPMC_OK) { return rc; }
+    *summary = mpmc_scan_summary{MPMC_SCAN_COMPLETE, -1, 0};
+    if (ncell == 0) { return MPMC_OK; }
+    if (!q || !inventory || !feed) { return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan_2p: null buffer"); }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    // state evaluation per cell: the inventory pass writes 0 or 1 (domain error) per cell
+    rc = mpmc_inventory_2p(mixture, grid, model, q, branch, scan->code, inventory, nullptr, stream);
+    if (rc != MPMC_OK) { return rc; }
+    const int nc = mpmc_mixture_components(mixture);
+    hipLaunchKernelGGL(k_scan_source_2p, dim3(grid_for(ncell)), dim3(256), 0, s, nc, ncell, q, inventory, feed,
+                       scan->code);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan_2p: ") + hipGetErrorString(e)); }
+    return scan_finish(mixture, ncell, q, q + ncell, feed, nullptr, 2, options, out, scan, summary, s);
+}
+
+} // extern "C"

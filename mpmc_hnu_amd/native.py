@@ -39,6 +39,7 @@ EXPORTED = (
     "mpmc_tpfa_grid_create",
     "mpmc_tpfa_grid_destroy",
     "mpmc_tpfa_grid_adjacency",
+    "mpmc_tpfa_grid_cells",
     "mpmc_tpfa_grid_max_degree",
     "mpmc_assembly_default_model",
     "mpmc_assembly_2p",
@@ -47,6 +48,8 @@ EXPORTED = (
     "mpmc_assembly_plan_launch",
     "mpmc_assembly_plan_domain_error",
     "mpmc_assembly_plan_destroy",
+    "mpmc_transition_scan",
+    "mpmc_transition_scan_2p",
     "mpmc_server_default_options",
     "mpmc_server_create",
     "mpmc_server_destroy",
@@ -59,6 +62,14 @@ EXPORTED = (
     "mpmc_server_last_error",
     "mpmc_last_error",
 )
+
+
+class ScanOutputs(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_void_p), ("target_count", ctypes.c_void_p), ("target_density", ctypes.c_void_p)]
+
+
+class ScanSummary(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("first_stop", ctypes.c_int64), ("proposals", ctypes.c_int64)]
 
 
 class FlashOptions(ctypes.Structure):
@@ -200,6 +211,7 @@ def library() -> ctypes.CDLL:
     lib.mpmc_tpfa_grid_destroy.argtypes = [P]
     lib.mpmc_tpfa_grid_adjacency.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
     lib.mpmc_tpfa_grid_max_degree.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
+    lib.mpmc_tpfa_grid_cells.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
     lib.mpmc_assembly_default_model.argtypes = [ctypes.c_int, ctypes.POINTER(AssemblyModel)]
     lib.mpmc_assembly_default_model.restype = None
     lib.mpmc_assembly_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, ctypes.c_double,
@@ -211,6 +223,12 @@ def library() -> ctypes.CDLL:
     lib.mpmc_assembly_plan_launch.argtypes = [P, P]
     lib.mpmc_assembly_plan_domain_error.argtypes = [P, P, ctypes.POINTER(ctypes.c_int32)]
     lib.mpmc_assembly_plan_destroy.argtypes = [P]
+    lib.mpmc_transition_scan.argtypes = [P, ctypes.c_int64, P, P, P, P, ctypes.POINTER(FlashOptions),
+                                         ctypes.POINTER(FlashOutputs), ctypes.POINTER(ScanOutputs),
+                                         ctypes.POINTER(ScanSummary), P]
+    lib.mpmc_transition_scan_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, P,
+                                            ctypes.POINTER(FlashOptions), ctypes.POINTER(FlashOutputs),
+                                            ctypes.POINTER(ScanOutputs), ctypes.POINTER(ScanSummary), P]
     lib.mpmc_server_default_options.argtypes = [ctypes.POINTER(ServerOptions)]
     lib.mpmc_server_default_options.restype = None
     lib.mpmc_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]

@@ -1,0 +1,130 @@
+"""GPU parity of the post-SNES phase-transition scan (mpmc_transition_scan,
+mpmc_transition_scan_2p) with the oracle's restatement of the reference's scanner
+(oracle/transition_scan.py).
+
+Bar: the scan rules applied to the device's own flash record give bit-identical codes,
+target counts and target densities (the decision kernel is integer logic plus one IEEE
+quotient); the flash record itself meets the flash's parity bar against the checker
+flash; the two-phase feeds are within one ulp of the long double normalisation of the
+device inventory, and the inventory within the assembly's 1e-8 of the oracle's."""
+import numpy as np
+import pytest
+
+import oracle_bindings as ob
+import parity
+from mpmc_hnu_amd import workloads as W
+from oracle import transition_scan as S
+
+MAX_FLIP = 2e-3
+MAX_LOOSE = 1e-3
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+def _check_rules(g, p, t, cnt, code0):
+    from mpmc_hnu_amd import flash
+
+    rec = flash.to_host(g["flash"])
+    code, target, dens = S.decide(cnt, p, t, rec["status"], rec["phase_count"], rec["zfactor"], code0)
+    assert np.array_equal(_host(g["code"]), code)
+    assert np.array_equal(_host(g["target_count"]), target)
+    gd = _host(g["target_density"])
+    assert np.array_equal(np.isnan(gd), np.isnan(dens))
+    fin = ~np.isnan(dens)
+    assert np.array_equal(gd[fin], dens[fin])
+    st, first, err, props = S.summary(code)
+    assert (g["status"], g["first_stop"], g["rc"] == 1, g["proposals"]) == (st, first, err, props)
+    return rec, code
+
+
+def _check_record(rec, o, n):
+    """The flash's own bar (test_gpu_flash.py): decisions within MAX_FLIP, values within 1e-9
+    relative except a MAX_LOOSE share, and every value within 1e-7."""
+    rep, _ = parity.compare(rec, o)
+    flips = rep["status_mismatch"] + rep["count_mismatch"] + rep["branch_mismatch"]
+    print("  flash record vs checker:", rep, flush=True)
+    assert flips <= max(2, MAX_FLIP * n), rep
+    assert rep["value_mismatch"] <= MAX_LOOSE * n, rep
+    loose, _ = parity.compare(rec, o, rel=1e-7, abs_lnphi=1e-7)
+    assert loose["value_mismatch"] == 0, loose
+
+
+@pytest.mark.gpu
+def test_scan_of_given_sources_matches_oracle(gpu):
+    from mpmc_hnu_amd import flash, scan
+
+    mix, p, t, z = W.sour_gas_sweep(1024)
+    rng = np.random.Generator(np.random.PCG64(21))
+    cnt = rng.integers(1, 4, p.size).astype(np.int32)
+    z, p = z.copy(), p.copy()
+    # malformed sources (the reference's invalid_argument): a negative fraction, zero
+    # pressure, phase counts 0 and 4, a composition whose sum is off one
+    z[0, 300] = -z[0, 300]
+    p[317] = 0.0
+    cnt[340], cnt[341] = 0, 4
+    z[1, 360] *= 1.01
+    print("[case] scan of 1024 sour-gas sources, nc=6", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        g = scan.transition_scan(m, p, t, z, cnt)
+    finally:
+        m.close()
+    code0 = np.where(S.validate_source(p, t, cnt, z), S.SAME, S.ERROR).astype(np.int32)
+    assert np.sum(code0 == S.ERROR) == 5
+    rec, code = _check_rules(g, p, t, cnt, code0)
+    assert np.sum(code == S.PROPOSAL) > 100 and np.sum(code == S.SAME) > 100
+    # proposals in both directions, including 2 -> 3 (the reference's regression case)
+    assert np.any((code == S.PROPOSAL) & (cnt == 2) & (_host(g["target_count"]) == 3))
+    assert np.any((code == S.PROPOSAL) & (cnt == 3) & (_host(g["target_count"]) < 3))
+    _check_record(rec, ob.flash(mix, p, t, z, 1e-11, 3), p.size)
+
+
+@pytest.mark.gpu
+def test_scan_without_stops_is_complete(gpu):
+    from mpmc_hnu_amd import flash, scan
+
+    mix = W.mixture_linspaced(6)
+    p, t, z = W.cells_uniform(6, 512, seed=33)
+    cnt = np.full(p.size, 2, np.int32)
+    print("[case] scan of 512 uniform sources, nc=6, two-phase backend", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        g = scan.transition_scan(m, p, t, z, cnt, max_phases=2)
+    finally:
+        m.close()
+    rec, code = _check_rules(g, p, t, cnt, np.zeros(p.size, np.int32))
+    o = ob.flash(mix, p, t, z, 1e-11, 2)
+    _check_record(rec, o, p.size)
+    if not np.any((code == S.INDETERMINATE) | (code == S.ERROR)):
+        assert g["status"] == S.COMPLETE and g["first_stop"] == -1 and g["rc"] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nc,dims", [(3, (4, 4, 3)), (6, (5, 4, 3))])
+def test_scan_of_a_two_phase_state_matches_oracle(gpu, nc, dims):
+    from mpmc_hnu_amd import assembly as A, flash, scan
+
+    mix = W.mixture_linspaced(nc)
+    grid = W.tpfa_grid(*dims, seed=nc)
+    q, br = W.two_phase_state(nc, grid["ncell"], ob.phase_roots(mix), seed=20 + nc, dims=dims)
+    model = A.default_model(nc)
+    print(f"[case] two-phase state scan nc={nc} grid={dims}", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    gr = A.Grid(grid)
+    try:
+        g = scan.transition_scan_2p(m, gr, model, q, br)
+    finally:
+        gr.close()
+        m.close()
+    inv, feed = _host(g["inventory"]), _host(g["feed"])
+    oinv = ob.inventory(mix, A.model_array(model, nc), q, br, grid["poro"])
+    assert np.all(np.abs(inv - oinv) <= 1e-8 * np.abs(oinv))
+    ref_feed, ok = S.source_feed(inv)
+    assert np.all(ok)
+    assert np.all(np.abs(feed - ref_feed) <= np.spacing(ref_feed))
+    p, t = q[0], q[1]
+    code0 = np.where(S.validate_source(p, t, np.full(p.size, 2), feed), S.SAME, S.INDETERMINATE).astype(np.int32)
+    rec, code = _check_rules(g, p, t, 2, code0)
+    _check_record(rec, ob.flash(mix, p, t, feed, 1e-11, 3), p.size)
