@@ -123,6 +123,23 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     torch.cuda.synchronize()
     plan_ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
     plan.close()
+    # the post-SNES phase-transition scan of the same state: inventory -> feed, flash, decision
+    from mpmc_hnu_amd import scan as S
+
+    scan = {}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec)
+        times = []
+        for _ in range(max(1, min(steps, 3))):
+            t0 = time.perf_counter()
+            res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec)
+            times.append((time.perf_counter() - t0) * 1e3)
+        codes = np.bincount(res["code"].cpu().numpy(), minlength=4)
+        scan[name] = {"ms": float(np.mean(times)), "cells_per_s": grid["ncell"] / (np.mean(times) * 1e-3),
+                      "same": int(codes[0]), "proposals": int(codes[1]), "indeterminate": int(codes[2]),
+                      "errors": int(codes[3]), "status": res["status"], "first_stop": res["first_stop"]}
+        print(f"[bench] transition scan ({name}): {scan[name]['ms']:.1f} ms, {codes[1]} proposals, "
+              f"{codes[2]} indeterminate", file=sys.stderr, flush=True)
     ncell, nface, Q = grid["ncell"], grid["owner"].size, 2 * nc + 1
     nnz = ncell * Q * Q + 2 * nface * (nc + 1) * Q
     algo_bytes = 8.0 * (ncell * (Q + nc + 3) + nface * 2 + nnz + ncell * Q)
@@ -135,7 +152,9 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
            "graph_replay_ms": plan_ms,
            "algorithmic_bytes": algo_bytes, "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS,
            "frac": gbs / PEAK_HBM_GBS,
-           "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables"}
+           "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables",
+           "transition_scan": dict(scan, what="post-SNES phase-transition scan of the same state (inventory, feed, "
+                                               "flash, per-cell decision; host clock, includes the summary sync)")}
     if tuple(dims) == (100, 100, 50):
         out["roofline"] = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": gbs / PEAK_HBM_GBS, "traffic": ASSEMBLY_TRAFFIC_BYTES,
