@@ -56,3 +56,17 @@ def test_invalid_arguments_are_rejected_without_device_work():
     assert b"nc" in lib.mpmc_last_error()
     rc = lib.mpmc_flash_batch(None, 0, None, None, None, None, None, None)
     assert rc == 1
+    # the phase-transition scan: null handles and outputs are argument errors, and the
+    # summary is left untouched
+    summary = native.ScanSummary(7, 5, 3)
+    scan = native.ScanOutputs(None, None, None)
+    rc = lib.mpmc_transition_scan(None, 4, None, None, None, None, None, None, ctypes.byref(scan),
+                                  ctypes.byref(summary), None)
+    assert rc == 1 and b"transition_scan" in lib.mpmc_last_error()
+    assert (summary.status, summary.first_stop, summary.proposals) == (7, 5, 3)
+    rc = lib.mpmc_transition_scan_2p(None, None, None, None, None, None, None, None, None, ctypes.byref(scan),
+                                     ctypes.byref(summary), None)
+    assert rc == 1
+    n = ctypes.c_int64(-1)
+    assert lib.mpmc_tpfa_grid_cells(None, ctypes.byref(n)) == 1 and n.value == -1
+    assert ctypes.sizeof(native.ScanSummary) == 24 and ctypes.sizeof(native.ScanOutputs) == 24

@@ -128,3 +128,21 @@ def test_scan_of_a_two_phase_state_matches_oracle(gpu, nc, dims):
     code0 = np.where(S.validate_source(p, t, np.full(p.size, 2), feed), S.SAME, S.INDETERMINATE).astype(np.int32)
     rec, code = _check_rules(g, p, t, 2, code0)
     _check_record(rec, ob.flash(mix, p, t, feed, 1e-11, 3), p.size)
+
+
+@pytest.mark.gpu
+def test_mixed_precision_scan_applies_the_same_rules(gpu):
+    """precision 1: the scan's decisions are the rules applied to the mixed flash record
+    (whose own agreement with FP64 is tested with the flash)."""
+    from mpmc_hnu_amd import flash, scan
+
+    mix, p, t, z = W.sour_gas_sweep(1024, seed=12)
+    cnt = np.full(p.size, 2, np.int32)
+    print("[case] mixed-precision scan of 1024 sour-gas sources", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        g = scan.transition_scan(m, p, t, z, cnt, precision=1)
+    finally:
+        m.close()
+    _, code = _check_rules(g, p, t, cnt, np.zeros(p.size, np.int32))
+    assert np.any(code == S.PROPOSAL)
