@@ -244,17 +244,33 @@ def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
         for a, b in zip(v, c):
             counts[int(a)] = counts.get(int(a), 0) + int(b)
     groups = plan.groups
+    # the same plan in mixed precision (same launch groups)
+    mplan = flash.Plan(segments, precision=1, group_mixtures=plan.group_mixtures, counters=False)
+    for _ in range(warmup):
+        mplan.launch(stream)
+    torch.cuda.synchronize()
+    for k in range(steps):
+        e0[k].record(stream)
+        mplan.launch(stream)
+        e1[k].record(stream)
+    torch.cuda.synchronize()
+    mixed_ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
+    mixed_count_diff = int(sum(int(torch.sum(o1["phase_count"] != o2["phase_count"]).item())
+                               for o1, o2 in zip(plan.outputs, mplan.outputs)))
+    mplan.close()
     plan.close()
     for m in mixes:
         m.close()
     print(f"[bench] uneven: plan {plan_ms:.1f} ms ({ncell / (plan_ms * 1e-3) / 1e6:.2f} M cells/s, {groups} launch "
-          f"groups); one mixture at a time {alone_ms:.1f} ms; identical records: {identical}",
-          file=sys.stderr, flush=True)
+          f"groups); one mixture at a time {alone_ms:.1f} ms; identical records: {identical}; mixed-precision plan "
+          f"{mixed_ms:.1f} ms ({mixed_count_diff} phase-count differences)", file=sys.stderr, flush=True)
     return {"mixtures": len(layout), "cells": ncell, "near_boundary_cells": near, "launch_groups": groups,
             "plan_ms": plan_ms, "cells_per_s": ncell / (plan_ms * 1e-3), "one_mixture_at_a_time_ms": alone_ms,
             "group_mixtures": plan.group_mixtures, "autotune_ms_by_group_mixtures": tune,
             "speedup_vs_one_at_a_time": alone_ms / plan_ms, "identical_records": bool(identical),
             "phase_counts": counts, "precision": "fp64",
+            "mixed_plan_ms": mixed_ms, "mixed_cells_per_s": ncell / (mixed_ms * 1e-3),
+            "mixed_phase_count_differences": mixed_count_diff,
             "what": "max-3-phase flash of every cell; ragged launches grouped by nc, one HIP graph"}
 
 

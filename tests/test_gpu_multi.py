@@ -52,6 +52,32 @@ def test_multi_equals_each_mixture_alone(gpu, nc, sizes):
 
 
 @pytest.mark.gpu
+def test_mixed_precision_multi_equals_each_mixture_alone(gpu):
+    """The mixed-precision schedule on a ragged batch: the reduced-precision per-cell
+    constants and every search read the cell's own mixture, so records are bit-identical
+    to mixed-precision mpmc_flash_batch on each mixture's cells."""
+    from mpmc_hnu_amd import flash
+
+    nc, sizes = 6, [500, 1, 1200, 64]
+    mixes = [flash.Mixture.from_dict(W.mixture_linspaced(nc, seed=30 + k)) for k in range(len(sizes))]
+    try:
+        cells = [W.cells_uniform(nc, n, seed=70 + k) for k, n in enumerate(sizes)]
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        p = np.concatenate([c[0] for c in cells])
+        t = np.concatenate([c[1] for c in cells])
+        z = np.ascontiguousarray(np.concatenate([c[2] for c in cells], axis=1))
+        print(f"[case] mixed-precision flash_multi nc={nc} sizes={sizes}", flush=True)
+        got = _host(flash.flash_multi(mixes, off, p, t, z, precision=1))
+        for k in range(len(sizes)):
+            a = _host(flash.flash_batch(mixes[k], *cells[k], max_phases=3, precision=1))
+            sl = slice(off[k], off[k + 1])
+            _same({key: got[key][..., sl] for key in KEYS}, a, ("mixed", k))
+    finally:
+        for m in mixes:
+            m.close()
+
+
+@pytest.mark.gpu
 def test_multi_many_mixtures_and_limits(gpu):
     from mpmc_hnu_amd import flash, native
 
