@@ -158,6 +158,17 @@ int mpmc_flash_batch_hinted(mpmc_mixture_t mixture, int64_t ncell, const double*
                             const mpmc_flash_options* options, const mpmc_flash_outputs* out,
                             void* workspace, size_t workspace_bytes, void* stream);
 
+/* mpmc_flash_batch_hinted plus final-review warm starts: up to 2 compositions per cell,
+ * final_hints[(h*nc + i)*ncell + c] (device memory), placed before the selected pair's two
+ * compositions in the common-tangent review, as Pr76PtFlashBackendOptions::final_starts
+ * does for the final_two_phase_stability_starts of PtSolveHints (pt_solve_hints.hpp:34-36,
+ * pr76_executable_model.hpp:139; pt_split.hpp:508-510). The same NaN, composition and
+ * support rules as the initial hints (pt_split.hpp:440-446). Staged schedule only. */
+int mpmc_flash_batch_warm(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                          const double* z, int nhint, const double* hints, int nfinal, const double* final_hints,
+                          const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /*
  * Ragged multi-mixture batch: the cells of several mixtures with the same component count
  * in one staged launch sequence. Mixture k owns cells [cell_offset[k], cell_offset[k+1])

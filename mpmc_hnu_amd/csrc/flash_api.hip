@@ -198,7 +198,7 @@ static int flash_buffers(const double* p, const double* t, const double* z, cons
 static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                             const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                             size_t workspace_bytes, bool own_workspace, void* stream, int nhint = 0,
-                            const double* hints = nullptr) {
+                            const double* hints = nullptr, int nfhint = 0, const double* fhints = nullptr) {
     if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: null mixture or outputs"); }
     if (ncell < 0 || ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
     mpmc_flash_options o;
@@ -209,7 +209,7 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
                                  out->ln_phi, out->counters};
     const hipStream_t s = static_cast<hipStream_t>(stream);
     hipError_t e = hipErrorInvalidValue;
-    if (nhint > 0 && o.schedule == 1) {
+    if ((nhint > 0 || nfhint > 0) && o.schedule == 1) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: hints run on the staged schedule only");
     }
     if (o.schedule == 1) {
@@ -256,7 +256,7 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
     case N:                                                                                                      \
         e = mpmc::dev::launch_pipeline<N>(m->dev_constants, 1, nullptr, workspace, workspace_bytes, ncell, p, t,  \
                                           z, o.max_phases, o.fugacity_tolerance, o.precision, fo, s, nhint,      \
-                                          hints);                                                                \
+                                          hints, nfhint, fhints);                                                \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE
@@ -274,6 +274,20 @@ int mpmc_flash_batch_ws(mpmc_mixture_t m, int64_t ncell, const double* p, const 
                         const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                         size_t workspace_bytes, void* stream) {
     return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, false, stream);
+}
+
+int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                          int nhint, const double* hints, int nfinal, const double* final_hints,
+                          const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
+                          size_t workspace_bytes, void* stream) {
+    if (nhint < 0 || nhint > 2 || nfinal < 0 || nfinal > 2) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: nhint and nfinal must be 0..2");
+    }
+    if ((nhint > 0 && ncell > 0 && !is_device_ptr(hints)) || (nfinal > 0 && ncell > 0 && !is_device_ptr(final_hints))) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: hints must be device memory");
+    }
+    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, workspace == nullptr,
+                            stream, nhint, hints, nfinal, final_hints);
 }
 
 int mpmc_flash_batch_hinted(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
@@ -372,7 +386,7 @@ int mpmc_flash_multi(int nmix, const mpmc_mixture_t* mixtures, const int64_t* ce
     case N:                                                                                                      \
         e = mpmc::dev::launch_pipeline<N>(table, nmix, nmix > 1 ? mix_of : nullptr, workspace, pipe_bytes, ncell, \
                                           p, t, z, o.max_phases, o.fugacity_tolerance, o.precision, fo, s, 0,   \
-                                          nullptr);                                                             \
+                                          nullptr, 0, nullptr);                                                 \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE
@@ -465,7 +479,7 @@ hipError_t plan_record(mpmc_flash_plan* pl, hipStream_t s) {
         e = mpmc::dev::launch_pipeline<N>(g.table, nmix, nmix > 1 ? g.mix_of : nullptr, pl->workspace,            \
                                           pl->workspace_bytes, g.n, g.p, g.t, g.z, pl->options.max_phases,        \
                                           pl->options.fugacity_tolerance, pl->options.precision, fo, s, 0,        \
-                                          nullptr);                                                               \
+                                          nullptr, 0, nullptr);                                                   \
         break;
             MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE

@@ -1113,15 +1113,17 @@ struct Max3Out {
 // solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), without caller continuation hints,
 // serial in one lane.
 template <int NC, bool kN = false>
-// initial: extra starts of the initial stability test (solve_pr76_pt_max3 initial_starts,
+// initial / final: extra starts of the initial stability test and of the base split's
+// common-tangent review (solve_pr76_pt_max3 initial_starts / final_starts,
 // pr76_three_phase.hpp:328-341; the warm-start hints of PtSolveHints).
 __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
                                         Max3Out<NC>& M, Counters& cnt, bool& rejected,
-                                        const double* const* initial = nullptr, int ninitial = 0) {
+                                        const double* const* initial = nullptr, int ninitial = 0,
+                                        const double* const* final_starts = nullptr, int nfinal = 0) {
     M.tail.use_neighbor = false;
     (void)check_composition<NC>(feed, rejected);
     if (rejected) { return; }
-    solve_vle<NC, kN>(m, p, t, feed, fug_tol, initial, ninitial, M.base, cnt, rejected, nullptr, 0);
+    solve_vle<NC, kN>(m, p, t, feed, fug_tol, initial, ninitial, M.base, cnt, rejected, final_starts, nfinal);
     if (rejected) { return; }
     switch (M.base.status) {
     case V_SINGLE: M.status = M_SINGLE; return;

@@ -85,7 +85,7 @@ struct Pipe {
     FlashOut out;
     double* feed;       // [NC][n]
     double* refl;       // [NC][n]
-    int32_t* refinfo;   // [n] branch | smooth << 8 | ok << 9 | active << 16 | hints << 24
+    int32_t* refinfo;   // [n] branch | smooth << 8 | ok << 9 | active << 16 | hints << 24 | final hints << 26
     int32_t* flags;     // [n]
     int32_t* cstat;     // [n] initial stability status
     int32_t* cell_u;    // [n]
@@ -104,9 +104,9 @@ struct Pipe {
     int32_t* u_v;       // [n] VLE status (-1 pending)
     double* u_ftol;     // [n]
     int32_t* u_dgbad;   // [n]
-    int32_t* fst;       // [NC+4][n]
-    uint32_t* fev;      // [NC+4][n]
-    double* fw;         // [NC+4][NC][n]
+    int32_t* fst;       // [NC+4+kMaxHints][n]
+    uint32_t* fev;      // [NC+4+kMaxHints][n]
+    double* fw;         // [NC+4+kMaxHints][NC][n]
     int32_t* m_st;      // [n] max3 status | neighbour << 8
     double* m_s;        // [6][n] beta[3], z[3]
     double* m_v;        // [6][NC][n] x[3], lnphi[3]
@@ -126,6 +126,10 @@ struct Pipe {
     // initial stability test's starts in order (NaN in component 0: no hint h for cell c)
     const double* hints;
     int nhint;
+    // final-review hints, the same layout: extra starts of the common-tangent review, placed
+    // before the selected pair's two compositions (pt_split.hpp:508-510)
+    const double* fhints;
+    int nfhint;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -163,20 +167,37 @@ template <int NC>
 __device__ __forceinline__ int active_count(int32_t info) { return (info >> 16) & 0xFF; }
 
 template <int NC>
-__device__ __forceinline__ int hint_count(int32_t info) { return (info >> 24) & 0xF; }
+__device__ __forceinline__ int hint_count(int32_t info) { return (info >> 24) & 0x3; }
 
-// The m-th warm-start hint present for cell c (hints absent for a cell are skipped).
 template <int NC>
-__device__ __forceinline__ void hint_start(const Pipe<NC>& P, int64_t c, int m, double* w) {
+__device__ __forceinline__ int final_hint_count(int32_t info) { return (info >> 26) & 0x3; }
+
+// The m-th hint present for cell c in an [nh][NC][n] hint array (hints absent for a cell,
+// NaN in component 0, are skipped).
+template <int NC>
+__device__ __forceinline__ void nth_hint(const double* H, int nh, int64_t n, int64_t c, int m, double* w) {
     int seen = -1;
-    for (int h = 0; h < P.nhint; ++h) {
-        if (__builtin_isnan(P.hints[(h * NC + 0) * P.n + c])) { continue; }
+    for (int h = 0; h < nh; ++h) {
+        if (__builtin_isnan(H[(h * NC + 0) * n + c])) { continue; }
         if (++seen == m) {
 #pragma unroll
-            for (int i = 0; i < NC; ++i) { w[i] = P.hints[(h * NC + i) * P.n + c]; }
+            for (int i = 0; i < NC; ++i) { w[i] = H[(h * NC + i) * n + c]; }
             return;
         }
     }
+}
+
+template <int NC>
+__device__ __forceinline__ void hint_start(const Pipe<NC>& P, int64_t c, int m, double* w) {
+    nth_hint<NC>(P.hints, P.nhint, P.n, c, m, w);
+}
+
+// Review trial count of the unstable cell with info word `info`: the generated starts, the
+// final-review hints, the selected pair's liquid and vapor compositions.
+template <int NC>
+__device__ __forceinline__ int review_trials(int32_t info) {
+    const int active = active_count<NC>(info);
+    return (active > 1 ? active + 2 : 1) + final_hint_count<NC>(info) + 2;
 }
 
 // Start composition of trial k (pt_stability.hpp:442-466): feed, uniform over the active
@@ -231,19 +252,22 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
         uint32_t prop = 0;
         int32_t info = active << 16;
         // warm-start hints: validated before any provider call, as the reference validates
-        // every supplied start (pt_stability.hpp:264-271)
-        int nh = 0;
-        for (int h = 0; h < P.nhint; ++h) {
-            if (__builtin_isnan(P.hints[(h * NC + 0) * n + c])) { continue; }
+        // every supplied start (pt_stability.hpp:264-271; final starts pt_split.hpp:440-446)
+        int nh = 0, nfh = 0;
+        for (int h = 0; h < P.nhint + P.nfhint; ++h) {
+            const bool fin = h >= P.nhint;
+            const double* H = fin ? P.fhints : P.hints;
+            const int hh = fin ? h - P.nhint : h;
+            if (__builtin_isnan(H[(hh * NC + 0) * n + c])) { continue; }
             double w[NC];
 #pragma unroll
-            for (int i = 0; i < NC; ++i) { w[i] = P.hints[(h * NC + i) * n + c]; }
+            for (int i = 0; i < NC; ++i) { w[i] = H[(hh * NC + i) * n + c]; }
             (void)check_composition<NC>(w, rejected);
 #pragma unroll
             for (int i = 0; i < NC; ++i) { if ((z[i] > 0) != (w[i] > 0)) { rejected = true; } }
-            ++nh;
+            if (fin) { ++nfh; } else { ++nh; }
         }
-        info |= nh << 24;
+        info |= (nh << 24) | (nfh << 26);
         Activity<NC> ref;
         if (!rejected && !lost) {
             ++prop;
@@ -323,9 +347,7 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int
         if (sel < 0 || !finite(P.u_ftol[slot])) { st_arr[k * n + slot] = -1; return false; }
         c = P.u_cell[slot];
         if (P.flags[c] & CF_REJECTED) { st_arr[k * n + slot] = -1; return false; }
-        const int active = active_count<NC>(P.refinfo[c]);
-        const int ntr = (active > 1 ? active + 2 : 1) + 2;
-        if (k >= ntr) { st_arr[k * n + slot] = -1; return false; }
+        if (k >= review_trials<NC>(P.refinfo[c])) { st_arr[k * n + slot] = -1; return false; }
     }
     L.slot = slot;
     L.cell = c;
@@ -371,9 +393,12 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int
         trial_start<NC>(L.z, active, k, L.cand);
     } else if (stage == 0) {
         hint_start<NC>(P, c, k - gen, L.cand);
+    } else if (k - gen < final_hint_count<NC>(P.refinfo[c])) {
+        nth_hint<NC>(P.fhints, P.nfhint, n, c, k - gen, L.cand);
     } else {
         const int sel = P.u_sel[slot];
-        const int q = k - gen; // 0: liquid x, 1: vapor y of the selected pair
+        // 0: liquid x, 1: vapor y of the selected pair, after the final-review hints
+        const int q = k - gen - final_hint_count<NC>(P.refinfo[c]);
 #pragma unroll
         for (int i = 0; i < NC; ++i) { L.cand[i] = P.a_v[((sel * 5 + q) * NC + i) * n + slot]; }
     }
@@ -442,7 +467,7 @@ template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage,
                                                               int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
-    const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4;
+    const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4 + P.nfhint;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
     unsigned long long* counter =
@@ -826,8 +851,7 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
         if (P.u_v[u] != -1) { continue; }
         const int64_t c = P.u_cell[u];
         if (P.flags[c] & (CF_REJECTED | CF_FALLBACK)) { continue; }
-        const int active = active_count<NC>(P.refinfo[c]);
-        const int ntr = (active > 1 ? active + 2 : 1) + 2;
+        const int ntr = review_trials<NC>(P.refinfo[c]);
         uint64_t evals = 0;
         bool neg = false, all_stat = true, ambiguous = false;
         for (int k = 0; k < ntr; ++k) {
@@ -872,8 +896,7 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
             y[i] = P.a_v[((sel * 5 + 1) * NC + i) * n + u];
         }
         const double beta = P.a_s[(sel * 6 + 1) * n + u];
-        const int active = active_count<NC>(P.refinfo[c]);
-        const int ntr = (active > 1 ? active + 2 : 1) + 2;
+        const int ntr = review_trials<NC>(P.refinfo[c]);
         int nwit = 0;
         for (int k = 0; k < ntr; ++k) { nwit += (P.fst[k * n + u] & 0xFF) == T_NEGATIVE; }
         Counters cnt{0, 0, 0, 0};
@@ -1063,24 +1086,31 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
         Counters cnt{0, 0, 0, 0};
         bool rejected = false;
         PhaseView<NC> ph[3];
-        double hw[kMaxHints][NC];
+        double hw[kMaxHints][NC], fw[kMaxHints][NC];
         const double* hs[kMaxHints];
-        int nh = 0;
+        const double* fs[kMaxHints];
+        int nh = 0, nfh = 0;
         for (int h = 0; h < P.nhint; ++h) {
             if (__builtin_isnan(P.hints[(h * NC + 0) * n + c])) { continue; }
             for (int q = 0; q < NC; ++q) { hw[nh][q] = P.hints[(h * NC + q) * n + c]; }
             hs[nh] = hw[nh];
             ++nh;
         }
+        for (int h = 0; h < P.nfhint; ++h) {
+            if (__builtin_isnan(P.fhints[(h * NC + 0) * n + c])) { continue; }
+            for (int q = 0; q < NC; ++q) { fw[nfh][q] = P.fhints[(h * NC + q) * n + c]; }
+            fs[nfh] = fw[nfh];
+            ++nfh;
+        }
         if (P.max_phases == 2) {
             VleOut<NC> V;
-            solve_vle<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, hs, nh, V, cnt, rejected, nullptr,
-                              0);
+            solve_vle<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, hs, nh, V, cnt, rejected, fs, nfh);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_vle<NC>(P.out, n, c, V, 0xF, cnt); }
         } else {
             Max3Out<NC> M;
-            solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected, hs, nh);
+            solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected, hs, nh, fs,
+                               nfh);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_max3<NC>(P.out, n, c, M, cnt); }
         }
@@ -1125,9 +1155,9 @@ __host__ size_t pipe_carve(Pipe<NC>& P, char* base, int64_t n) {
     take(P.u_v, N);
     take(P.u_ftol, N);
     take(P.u_dgbad, N);
-    take(P.fst, (NC + 4) * N);
-    take(P.fev, (NC + 4) * N);
-    take(P.fw, (NC + 4) * NC * N);
+    take(P.fst, (NC + 4 + kMaxHints) * N);
+    take(P.fev, (NC + 4 + kMaxHints) * N);
+    take(P.fw, (NC + 4 + kMaxHints) * NC * N);
     take(P.m_st, N);
     take(P.m_s, 6 * N);
     take(P.m_v, 6 * NC * N);
@@ -1254,11 +1284,14 @@ template <int NC>
 hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, void* workspace,
                            size_t workspace_bytes, int64_t n, const double* p, const double* t, const double* z,
                            int max_phases, double fug_tol, int precision, const FlashOut& out, hipStream_t s,
-                           int nhint, const double* hints) {
+                           int nhint, const double* hints, int nfhint, const double* fhints) {
     Pipe<NC> P{};
     if (nhint < 0 || nhint > kMaxHints || (nhint > 0 && !hints)) { return hipErrorInvalidValue; }
+    if (nfhint < 0 || nfhint > kMaxHints || (nfhint > 0 && !fhints)) { return hipErrorInvalidValue; }
     P.nhint = nhint;
     P.hints = hints;
+    P.nfhint = nfhint;
+    P.fhints = fhints;
     const size_t need = pipe_carve<NC>(P, static_cast<char*>(workspace), n);
     if (!workspace || need > workspace_bytes) { return hipErrorInvalidValue; }
     if (nmix < 1 || (nmix > 1 && !mix_of)) { return hipErrorInvalidValue; }
@@ -1279,11 +1312,13 @@ hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, 
     extern template size_t pipeline_bytes<N>(int64_t);                                                            \
     extern template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,     \
                                                   const double*, const double*, const double*, int, double, int,  \
-                                                  const FlashOut&, hipStream_t, int, const double*);
+                                                  const FlashOut&, hipStream_t, int, const double*, int,    \
+                                                  const double*);
 #define MPMC_INSTANTIATE_PIPELINE(N)                                                                              \
     template size_t pipeline_bytes<N>(int64_t);                                                                   \
     template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,            \
                                            const double*, const double*, const double*, int, double, int,         \
-                                           const FlashOut&, hipStream_t, int, const double*);
+                                           const FlashOut&, hipStream_t, int, const double*, int,           \
+                                           const double*);
 
 } // namespace mpmc::dev

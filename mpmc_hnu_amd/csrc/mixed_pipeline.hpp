@@ -166,8 +166,7 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
         if (sel < 0 || !finite(P.u_ftol[slot])) { st_arr[k * n + slot] = -1; return false; }
         c = P.u_cell[slot];
         if (P.flags[c] & (CF_REJECTED | CF_FALLBACK)) { st_arr[k * n + slot] = -1; return false; }
-        const int active = active_count<NC>(P.refinfo[c]);
-        ntr = (active > 1 ? active + 2 : 1) + 2;
+        ntr = review_trials<NC>(P.refinfo[c]);
         if (k >= ntr) { st_arr[k * n + slot] = -1; return false; }
     }
     L.slot = slot;
@@ -202,10 +201,13 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
         trial_start<NC>(z, active, k, w0);
     } else if (stage == 0) {
         hint_start<NC>(P, c, k - gen, w0);
+    } else if (k - gen < final_hint_count<NC>(P.refinfo[c])) {
+        nth_hint<NC>(P.fhints, P.nfhint, n, c, k - gen, w0);
     } else {
         const int sel = P.u_sel[slot];
+        const int q = k - gen - final_hint_count<NC>(P.refinfo[c]);
 #pragma unroll
-        for (int i = 0; i < NC; ++i) { w0[i] = P.a_v[((sel * 5 + (k - gen)) * NC + i) * n + slot]; }
+        for (int i = 0; i < NC; ++i) { w0[i] = P.a_v[((sel * 5 + q) * NC + i) * n + slot]; }
     }
     const double s = ksum<NC>(w0);
 #pragma unroll
@@ -281,7 +283,7 @@ __device__ __forceinline__ int tpd32_check(const Pipe<NC>& P, int stage, Tpd32La
 template <int NC>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd32(Pipe<NC> P, const double* __restrict__ mixc, int stage) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
-    const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4;
+    const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4 + P.nfhint;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
     unsigned long long* counter = &P.g[stage == 0 ? G_TASK_INIT32 : G_TASK_REVIEW32];

@@ -102,12 +102,13 @@ def allocate_outputs(nc: int, ncell: int, device, counters: bool = True) -> dict
 
 def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, max_phases: int = 3,
                 precision: int = 0, stream=None, out: dict | None = None, counters: bool = True,
-                schedule: int = 0, workspace=None, hints=None) -> dict:
+                schedule: int = 0, workspace=None, hints=None, final_hints=None) -> dict:
     """Flash every cell (p[ncell] Pa, t[ncell] K, z[nc, ncell]); returns device tensors.
 
     ``hints`` ([nhint, nc, ncell], nhint <= 2; NaN in component 0 marks an absent hint)
     are warm-start compositions appended to each cell's initial stability starts
-    (mpmc_flash_batch_hinted).
+    (mpmc_flash_batch_hinted); ``final_hints`` (same layout) are extra starts of the
+    common-tangent review (mpmc_flash_batch_warm).
 
     schedule 0 runs the staged batch (one lane per stability trial / split attempt),
     schedule 1 one lane per cell. ``workspace`` (a uint8 device tensor of at least
@@ -136,6 +137,27 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
         out["branch"].data_ptr(), out["x"].data_ptr(), out["ln_phi"].data_ptr(),
         out["counters"].data_ptr() if out.get("counters") is not None else None)
     V = ctypes.c_void_p
+    if final_hints is not None:
+        hs = []
+        for h in (hints, final_hints):
+            if h is None:
+                hs.append(None)
+                continue
+            h = _dev(h, torch.float64, device)
+            if h.dim() != 3 or h.shape[1:] != (mixture.nc, n) or h.shape[0] > 2:
+                raise ValueError("flash_batch: expected hints[nhint <= 2, nc, ncell]")
+            hs.append(h)
+        hints, final_hints = hs
+        ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
+            else (None, 0)
+        nh = 0 if hints is None else int(hints.shape[0])
+        rc = lib.mpmc_flash_batch_warm(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()), nh,
+                                       V(hints.data_ptr()) if nh else None, int(final_hints.shape[0]),
+                                       V(final_hints.data_ptr()), ctypes.byref(opts), ctypes.byref(o), ws, wsb,
+                                       _stream_ptr(stream))
+        native.check(rc, "flash_batch_warm")
+        out["_inputs"] = (p, t, z, hints, final_hints)
+        return out
     if hints is not None:
         hints = _dev(hints, torch.float64, device)
         if hints.dim() != 3 or hints.shape[1:] != (mixture.nc, n) or hints.shape[0] > 2:

@@ -28,6 +28,7 @@ EXPORTED = (
     "mpmc_flash_multi_workspace_bytes",
     "mpmc_flash_multi",
     "mpmc_flash_batch_hinted",
+    "mpmc_flash_batch_warm",
     "mpmc_flash_plan_create",
     "mpmc_flash_plan_launch",
     "mpmc_flash_plan_info",
@@ -189,6 +190,9 @@ def library() -> ctypes.CDLL:
                                         ctypes.POINTER(FlashOutputs), P, ctypes.c_size_t, P]
     lib.mpmc_flash_batch_hinted.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int, P, ctypes.POINTER(FlashOptions),
                                             ctypes.POINTER(FlashOutputs), P, ctypes.c_size_t, P]
+    lib.mpmc_flash_batch_warm.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int, P, ctypes.c_int, P,
+                                          ctypes.POINTER(FlashOptions), ctypes.POINTER(FlashOutputs), P,
+                                          ctypes.c_size_t, P]
     lib.mpmc_flash_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64]
     lib.mpmc_flash_workspace_bytes.restype = ctypes.c_size_t
     lib.mpmc_flash_multi_max_mixtures.argtypes = [ctypes.c_int]

@@ -861,6 +861,11 @@ void solve_vle(const Mixture& m, double p, double t, const double* feed, const S
     const int n = m.n;
     if (!std::isfinite(p) || p <= 0 || !std::isfinite(t) || t <= 0) { throw Rejected{}; }
     (void)check_composition(feed, n);
+    // final starts are validated before any property call (pt_split.hpp:440-446)
+    for (int e = 0; e < nfinal; ++e) {
+        (void)check_composition(final_starts[e], n);
+        for (int i = 0; i < n; ++i) { if ((feed[i] > 0) != (final_starts[e][i] > 0)) { throw Rejected{}; } }
+    }
     for (int e = 0; e < nstarts; ++e) {
         (void)check_composition(starts[e], n);
         for (int i = 0; i < n; ++i) { if ((feed[i] > 0) != (starts[e][i] > 0)) { throw Rejected{}; } }
@@ -1209,14 +1214,16 @@ bool log_ratio(int n, const double* num, const double* den, const double* z, dou
 }
 
 // solve_pr76_pt_max3, pr76_three_phase.hpp:328-561 (no caller continuation hints).
-// initial: extra starts of the initial stability test (solve_pr76_pt_max3 initial_starts,
+// initial / final: extra starts of the initial stability test and of the base split's
+// common-tangent review (solve_pr76_pt_max3 initial_starts / final_starts,
 // pr76_three_phase.hpp:328-341).
 void solve_max3(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
-                Max3Result& M, Counters& cnt, const double* const* initial = nullptr, int ninitial = 0) {
+                Max3Result& M, Counters& cnt, const double* const* initial = nullptr, int ninitial = 0,
+                const double* const* final_starts = nullptr, int nfinal = 0) {
     const int n = m.n;
     (void)check_composition(feed, n);
     M.use_neighbor = false;
-    solve_vle(m, p, t, feed, so, initial, ninitial, M.base, cnt, nullptr, 0);
+    solve_vle(m, p, t, feed, so, initial, ninitial, M.base, cnt, final_starts, nfinal);
     switch (M.base.status) {
     case V_SINGLE: M.status = M_SINGLE; return;
     case V_TWO: M.status = M_TWO; return;
@@ -2298,14 +2305,18 @@ extern "C" int cpuref_flash_batch(int nc, const double* tc, const double* pc, co
 }
 
 // Warm-start hints: hints[(h*nc + i)*ncell + c], h < nhint, appended to the initial stability
-// starts (Pr76PtFlashBackendOptions::initial_starts); NaN in component 0: no hint h for cell c.
-extern "C" int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
-                                         const double* kij, int64_t ncell, const double* p, const double* t,
-                                         const double* z, int nhint, const double* hints,
-                                         double fugacity_tolerance, int max_phases, int threads, int32_t* status,
-                                         int32_t* phase_count, double* beta, double* zfactor, int32_t* branch,
-                                         double* x, double* ln_phi, uint32_t* counters) {
+// starts (Pr76PtFlashBackendOptions::initial_starts); final_hints likewise, the extra starts
+// of the common-tangent review (final_starts, PtSolveHints::final_two_phase_stability_starts).
+// NaN in component 0: no hint h for cell c.
+extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* pc, const double* omega,
+                                       const double* kij, int64_t ncell, const double* p, const double* t,
+                                       const double* z, int nhint, const double* hints, int nfinal,
+                                       const double* final_hints, double fugacity_tolerance, int max_phases,
+                                       int threads, int32_t* status, int32_t* phase_count, double* beta,
+                                       double* zfactor, int32_t* branch, double* x, double* ln_phi,
+                                       uint32_t* counters) {
     if (nhint < 0 || nhint > 2 || (nhint > 0 && !hints)) { return 1; }
+    if (nfinal < 0 || nfinal > 2 || (nfinal > 0 && !final_hints)) { return 1; }
     if (nc < 1 || nc > NCMAX || ncell < 0 || (max_phases != 2 && max_phases != 3)) { return 1; }
     const Mixture m = make_mixture(nc, tc, pc, omega, kij);
     const Out o{status, phase_count, beta, zfactor, branch, x, ln_phi, counters};
@@ -2314,23 +2325,30 @@ extern "C" int cpuref_flash_batch_hinted(int nc, const double* tc, const double*
     parallel_cells(ncell, threads, [&](int64_t c) {
         double feed[NCMAX];
         for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
-        double hw[2][NCMAX];
+        double hw[2][NCMAX], fw[2][NCMAX];
         const double* hs[2];
-        int nh = 0;
+        const double* fs[2];
+        int nh = 0, nf = 0;
         for (int h = 0; h < nhint; ++h) {
             if (std::isnan(hints[(h * nc + 0) * ncell + c])) { continue; }
             for (int i = 0; i < nc; ++i) { hw[nh][i] = hints[(h * nc + i) * ncell + c]; }
             hs[nh] = hw[nh];
             ++nh;
         }
+        for (int h = 0; h < nfinal; ++h) {
+            if (std::isnan(final_hints[(h * nc + 0) * ncell + c])) { continue; }
+            for (int i = 0; i < nc; ++i) { fw[nf][i] = final_hints[(h * nc + i) * ncell + c]; }
+            fs[nf] = fw[nf];
+            ++nf;
+        }
         Counters cnt;
         auto M = std::make_unique<Max3Result>();
         try {
             if (max_phases == 2) {
-                solve_vle(m, p[c], t[c], feed, so, hs, nh, M->base, cnt, nullptr, 0);
+                solve_vle(m, p[c], t[c], feed, so, hs, nh, M->base, cnt, fs, nf);
                 publish_vle(o, nc, ncell, c, M->base, 0xF, cnt);
             } else {
-                solve_max3(m, p[c], t[c], feed, so, *M, cnt, hs, nh);
+                solve_max3(m, p[c], t[c], feed, so, *M, cnt, hs, nh, fs, nf);
                 publish_max3(o, nc, ncell, c, *M, cnt);
             }
         } catch (const Rejected&) {
@@ -2339,6 +2357,17 @@ extern "C" int cpuref_flash_batch_hinted(int nc, const double* tc, const double*
         }
     });
     return 0;
+}
+
+extern "C" int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
+                                         const double* kij, int64_t ncell, const double* p, const double* t,
+                                         const double* z, int nhint, const double* hints,
+                                         double fugacity_tolerance, int max_phases, int threads, int32_t* status,
+                                         int32_t* phase_count, double* beta, double* zfactor, int32_t* branch,
+                                         double* x, double* ln_phi, uint32_t* counters) {
+    return cpuref_flash_batch_warm(nc, tc, pc, omega, kij, ncell, p, t, z, nhint, hints, 0, nullptr,
+                                   fugacity_tolerance, max_phases, threads, status, phase_count, beta, zfactor,
+                                   branch, x, ln_phi, counters);
 }
 
 extern "C" int cpuref_phase_batch(int nc, const double* tc, const double* pc, const double* omega,

@@ -30,6 +30,16 @@ int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const 
                               double* beta, double* zfactor, int32_t* branch, double* x, double* ln_phi,
                               uint32_t* counters);
 
+/* cpuref_flash_batch_hinted plus final-review hints: final_hints[(h*nc + i)*ncell + c]
+ * (h < nfinal <= 2) placed before the pair's compositions in the common-tangent review
+ * (final_starts); NaN in component 0: no hint. */
+int cpuref_flash_batch_warm(int nc, const double* tc, const double* pc, const double* omega,
+                            const double* kij, int64_t ncell, const double* p, const double* t,
+                            const double* z, int nhint, const double* hints, int nfinal, const double* final_hints,
+                            double fugacity_tolerance, int max_phases, int threads, int32_t* status,
+                            int32_t* phase_count, double* beta, double* zfactor, int32_t* branch, double* x,
+                            double* ln_phi, uint32_t* counters);
+
 int cpuref_phase_batch(int nc, const double* tc, const double* pc, const double* omega,
                        const double* kij, int64_t ncell, const double* p, const double* t,
                        const double* x, int32_t* root_status, int32_t* root_count, double* zroot,

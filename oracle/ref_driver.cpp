@@ -139,7 +139,7 @@ void publish(const fl::PtPhaseSetResult& set, int nc, int64_t ncell, int64_t c, 
 void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evaluator, int nc,
                 int64_t ncell, int64_t c, const double* p, const double* t, const double* z,
                 double fugacity_tolerance, int max_phases, const Outputs& out, int nhint = 0,
-                const double* hints = nullptr) {
+                const double* hints = nullptr, int nfinal = 0, const double* final_hints = nullptr) {
     std::vector<double> feed(nc);
     for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
     // warm-start hints as the backend's initial_starts (PtSolveHints::initial_stability_starts)
@@ -149,12 +149,19 @@ void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evalua
         initial.emplace_back(nc);
         for (int i = 0; i < nc; ++i) { initial.back()[i] = hints[(h * nc + i) * ncell + c]; }
     }
+    // final-review hints as the backend's final_starts (PtSolveHints::final_two_phase_stability_starts)
+    std::vector<std::vector<double>> finals;
+    for (int h = 0; h < nfinal; ++h) {
+        if (std::isnan(final_hints[(h * nc + 0) * ncell + c])) { continue; }
+        finals.emplace_back(nc);
+        for (int i = 0; i < nc; ++i) { finals.back()[i] = final_hints[(h * nc + i) * ncell + c]; }
+    }
     Counts counts;
     try {
         if (max_phases == 2) {
             fl::PtSplitOptions options;
             options.iteration.fugacity_tolerance = fugacity_tolerance;
-            const auto legacy = fl::solve_pr76_pt_vle(p[c], t[c], feed, evaluator, options, initial);
+            const auto legacy = fl::solve_pr76_pt_vle(p[c], t[c], feed, evaluator, options, initial, finals);
             counts.split(legacy.solution);
             const auto published = fl::project_pr76_pt_phase_set(legacy);
             publish(published.solution, nc, ncell, c, 0xF, static_cast<int>(legacy.solution.status),
@@ -162,7 +169,7 @@ void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evalua
         } else {
             fl::Pr76PtMax3Options options;
             options.two_phase.iteration.fugacity_tolerance = fugacity_tolerance;
-            const auto source = fl::solve_pr76_pt_max3(p[c], t[c], feed, evaluator, options, initial);
+            const auto source = fl::solve_pr76_pt_max3(p[c], t[c], feed, evaluator, options, initial, finals);
             counts.split(source.base.solution);
             for (const auto& attempt : source.attempts) {
                 counts.prop += attempt.equilibrium.evaluations;
@@ -327,12 +334,14 @@ void parallel_cells(int64_t ncell, int threads, Body body) {
 
 extern "C" {
 
-int ref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
-                           const double* kij, int64_t ncell, const double* p, const double* t,
-                           const double* z, int nhint, const double* hints, double fugacity_tolerance,
-                           int max_phases, int threads, int32_t* status, int32_t* phase_count, double* beta,
-                           double* zfactor, int32_t* branch, double* x, double* ln_phi, uint32_t* counters) {
+int ref_flash_batch_warm(int nc, const double* tc, const double* pc, const double* omega,
+                         const double* kij, int64_t ncell, const double* p, const double* t,
+                         const double* z, int nhint, const double* hints, int nfinal, const double* final_hints,
+                         double fugacity_tolerance, int max_phases, int threads, int32_t* status,
+                         int32_t* phase_count, double* beta, double* zfactor, int32_t* branch, double* x,
+                         double* ln_phi, uint32_t* counters) {
     if (nhint < 0 || nhint > 2 || (nhint > 0 && !hints)) { return 1; }
+    if (nfinal < 0 || nfinal > 2 || (nfinal > 0 && !final_hints)) { return 1; }
     try {
         const auto parameters = make_parameters(nc, tc, pc, omega, kij);
         const auto model = th::Pr76Phase<double>::from_parameters(parameters);
@@ -341,13 +350,22 @@ int ref_flash_batch_hinted(int nc, const double* tc, const double* pc, const dou
             fl::Pr76VleEvaluator evaluator(model); // one evaluator per thread
             run([&](int64_t c) {
                 solve_cell(model, evaluator, nc, ncell, c, p, t, z, fugacity_tolerance, max_phases, out, nhint,
-                           hints);
+                           hints, nfinal, final_hints);
             });
         });
         return 0;
     } catch (const std::exception&) {
         return 1;
     }
+}
+
+int ref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
+                           const double* kij, int64_t ncell, const double* p, const double* t,
+                           const double* z, int nhint, const double* hints, double fugacity_tolerance,
+                           int max_phases, int threads, int32_t* status, int32_t* phase_count, double* beta,
+                           double* zfactor, int32_t* branch, double* x, double* ln_phi, uint32_t* counters) {
+    return ref_flash_batch_warm(nc, tc, pc, omega, kij, ncell, p, t, z, nhint, hints, 0, nullptr, fugacity_tolerance,
+                                max_phases, threads, status, phase_count, beta, zfactor, branch, x, ln_phi, counters);
 }
 
 int ref_flash_batch(int nc, const double* tc, const double* pc, const double* omega,

@@ -21,6 +21,8 @@ _FLASH_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.
 _PHASE_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P] + [_P] * 5
 _FLASH_HINT_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P, ctypes.c_double,
                     ctypes.c_int, ctypes.c_int] + [_P] * 8
+_FLASH_WARM_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P, ctypes.c_int, _P,
+                    ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_P] * 8
 
 
 def _ptr(a: np.ndarray):
@@ -33,6 +35,7 @@ def _load(path: Path, prefix: str):
     lib = ctypes.CDLL(str(path))
     getattr(lib, prefix + "_flash_batch").argtypes = _FLASH_ARGS
     getattr(lib, prefix + "_flash_batch_hinted").argtypes = _FLASH_HINT_ARGS
+    getattr(lib, prefix + "_flash_batch_warm").argtypes = _FLASH_WARM_ARGS
     getattr(lib, prefix + "_phase_batch").argtypes = _PHASE_ARGS
     return lib
 
@@ -57,10 +60,12 @@ def _mix_arrays(mix):
             np.ascontiguousarray(mix["kij"], dtype=np.float64))
 
 
-def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which="cpuref", hints=None):
+def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which="cpuref", hints=None,
+          final_hints=None):
     """Run the checker flash on host arrays; returns a dict of SoA arrays. ``hints``
     ([nhint <= 2, nc, ncell], NaN in component 0 = absent) are warm-start compositions
-    appended to the initial stability starts."""
+    appended to the initial stability starts; ``final_hints`` (same layout) are the extra
+    starts of the common-tangent review."""
     L = lib(which)
     if L is None:
         raise RuntimeError(f"checker library '{which}' is not built")
@@ -79,7 +84,18 @@ def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which
         "ln_phi": np.zeros((3, nc, n)),
         "counters": np.zeros((4, n), np.uint32),
     }
-    if hints is not None:
+    if final_hints is not None:
+        hints = np.zeros((0, nc, n)) if hints is None else np.ascontiguousarray(hints, dtype=np.float64)
+        final_hints = np.ascontiguousarray(final_hints, dtype=np.float64)
+        for h in (hints, final_hints):
+            assert h.ndim == 3 and h.shape[1:] == (nc, n) and h.shape[0] <= 2
+        fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_warm")
+        rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), int(hints.shape[0]),
+                _ptr(hints), int(final_hints.shape[0]), _ptr(final_hints), float(fugacity_tolerance), int(max_phases),
+                int(threads), _ptr(out["status"]), _ptr(out["phase_count"]), _ptr(out["beta"]),
+                _ptr(out["zfactor"]), _ptr(out["branch"]), _ptr(out["x"]), _ptr(out["ln_phi"]),
+                _ptr(out["counters"]))
+    elif hints is not None:
         hints = np.ascontiguousarray(hints, dtype=np.float64)
         assert hints.ndim == 3 and hints.shape[1:] == (nc, n) and hints.shape[0] <= 2
         fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_hinted")

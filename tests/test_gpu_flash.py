@@ -218,3 +218,46 @@ def test_warm_start_hints(gpu, precision):
         for k in range(2):
             a, b = g["counters"][k, same].sum(dtype=np.float64), o["counters"][k, same].sum(dtype=np.float64)
             assert abs(a - b) <= 0.01 * b, (k, a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,max_phases", [(0, 3), (0, 2), (1, 3)])
+def test_final_review_hints(gpu, precision, max_phases):
+    """mpmc_flash_batch_warm: initial and final-review hints (the phase compositions of a
+    flash at 3% lower pressure), a final hint with the wrong support rejecting its cell.
+    FP64 matches the oracle's warm flash at the usual bar, with the review's extra trials
+    counted identically; mixed precision at the mixed bar."""
+    from mpmc_hnu_amd import flash
+
+    nc = 6
+    mix = W.mixture_linspaced(nc)
+    p, t, z = W.cells_uniform(nc, 2048, seed=79)
+    prev = ob.flash(mix, p * 0.97, t, z, 1e-11, max_phases)
+    fh = W.warm_start_hints(prev, z)
+    ih = W.warm_start_hints(prev, z, nhint=1)
+    bad_cell = int(np.nonzero(np.isfinite(fh[1, 0]))[0][0])
+    fh[1, :, bad_cell] = 0.0
+    fh[1, 0, bad_cell] = 1.0
+    print(f"[case] warm flash precision={precision} max_phases={max_phases}, "
+          f"{int(np.isfinite(fh[0, 0]).sum())} cells with final hints", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        g = flash.to_host(flash.flash_batch(m, p, t, z, max_phases=max_phases, precision=precision, hints=ih,
+                                            final_hints=fh))
+    finally:
+        m.close()
+    o = ob.flash(mix, p, t, z, 1e-11, max_phases, hints=ih, final_hints=fh)
+    assert g["status"][bad_cell] & 0x2000 and o["status"][bad_cell] & 0x2000
+    rel = 1e-9 if precision == 0 else 1e-8
+    rep, bad = parity.compare(g, o, rel=rel, abs_lnphi=rel)
+    print("warm", rep, flush=True)
+    assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
+    assert rep["value_mismatch"] <= MAX_LOOSE * p.size, rep
+    res = parity.fugacity_residual(g)
+    acc = (g["status"] & 0xF) == 0
+    assert np.all(res[acc] <= 1e-10), float(res[acc].max())
+    if precision == 0:
+        same = ~bad
+        for k in range(2):
+            a, b = g["counters"][k, same].sum(dtype=np.float64), o["counters"][k, same].sum(dtype=np.float64)
+            assert abs(a - b) <= 0.01 * b, (k, a, b)

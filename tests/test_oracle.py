@@ -219,3 +219,34 @@ def test_warm_start_hints_match_reference(nc):
     hinted = np.isfinite(hints[0, 0])
     hinted[bad_cell] = False
     assert np.all(o["counters"][1][hinted] > plain["counters"][1][hinted])
+
+
+@pytest.mark.parametrize("max_phases", [2, 3])
+def test_final_review_hints_match_reference(max_phases):
+    """Final-review warm starts (PtSolveHints::final_two_phase_stability_starts through
+    final_starts): the phase compositions of a flash at 3% lower pressure handed to the
+    common-tangent review of the same feeds, together with initial hints. The oracle and
+    the compiled reference agree bit for bit, the review runs the extra trials, and a final
+    hint with the wrong support rejects the cell."""
+    nc = 6
+    mix = W.mixture_linspaced(nc, seed=77)
+    p, t, z = W.cells_uniform(nc, 384, seed=78)
+    prev = ob.flash(mix, p * 0.97, t, z, 1e-11, max_phases, which="ref")
+    fh = W.warm_start_hints(prev, z)
+    ih = W.warm_start_hints(prev, z, nhint=1)
+    assert np.isfinite(fh[0, 0]).sum() > 10
+    bad_cell = int(np.nonzero(np.isfinite(fh[1, 0]))[0][0])
+    fh[1, :, bad_cell] = 0.0
+    fh[1, 0, bad_cell] = 1.0  # support differs from the feed's
+    o = ob.flash(mix, p, t, z, 1e-11, max_phases, hints=ih, final_hints=fh)
+    r = ob.flash(mix, p, t, z, 1e-11, max_phases, which="ref", hints=ih, final_hints=fh)
+    rep, bad = parity.compare(o, r, rel=0.0, abs_lnphi=0.0)
+    assert not bad.any(), rep
+    assert np.array_equal(o["counters"][:2], r["counters"][:2])
+    assert o["status"][bad_cell] & 0x2000 and r["status"][bad_cell] & 0x2000
+    # the review of two-phase cells with a final hint runs more trials than without
+    plain = ob.flash(mix, p, t, z, 1e-11, max_phases, hints=ih)
+    two = (plain["phase_count"] == 2) & np.isfinite(fh[0, 0])
+    two[bad_cell] = False
+    assert two.sum() > 0
+    assert np.all(o["counters"][1][two] > plain["counters"][1][two])
