@@ -633,12 +633,16 @@ int mpmc_flash_plan_create(int nseg, const mpmc_flash_segment* segs, const mpmc_
         return e == hipErrorOutOfMemory ? fail(MPMC_E_NO_MEMORY, "flash_plan_create: device allocation failed")
                                         : hip_fail(e, "flash_plan_create");
     }
-    // one eager run settles every lazily computed launch parameter, then the sequence is
-    // captured once into a graph
+    // One eager run settles every lazily computed launch parameter, then the sequence is
+    // captured once into a graph. The eager run goes to the null stream, not to the capture
+    // stream: a capture stream only records, so its hardware queue never has to back the
+    // pipeline's large private segments (the serial fallback and the three-phase tail carry
+    // up to 22 KB per lane), which the runtime has refused on an additional queue while
+    // another queue held its scratch.
     hipStream_t cs = nullptr;
-    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-    if (e == hipSuccess) { e = plan_record(pl.get(), cs); }
-    if (e == hipSuccess) { e = hipStreamSynchronize(cs); }
+    e = plan_record(pl.get(), nullptr);
+    if (e == hipSuccess) { e = hipStreamSynchronize(nullptr); }
+    if (e == hipSuccess) { e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking); }
     if (e == hipSuccess) { e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal); }
     if (e == hipSuccess) {
         const hipError_t rec = plan_record(pl.get(), cs);
@@ -1075,11 +1079,12 @@ int mpmc_assembly_plan_create(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_a
         mpmc_assembly_plan_destroy(pl.release());
         return fail(MPMC_E_NO_MEMORY, "assembly_plan_create: device allocation failed");
     }
-    // one eager evaluation, then the same launch sequence captured once
+    // one eager evaluation (on the null stream: the capture stream only records, as in
+    // mpmc_flash_plan_create), then the same launch sequence captured once
     hipStream_t cs = nullptr;
-    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-    if (e == hipSuccess) { e = assembly_launch(m, g, md, q, branch, prev, dt, out, pl->rec, pl->flag, false, cs); }
-    if (e == hipSuccess) { e = hipStreamSynchronize(cs); }
+    e = assembly_launch(m, g, md, q, branch, prev, dt, out, pl->rec, pl->flag, false, nullptr);
+    if (e == hipSuccess) { e = hipStreamSynchronize(nullptr); }
+    if (e == hipSuccess) { e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking); }
     if (e == hipSuccess) { e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal); }
     if (e == hipSuccess) {
         const hipError_t rec = assembly_launch(m, g, md, q, branch, prev, dt, out, pl->rec, pl->flag, false, cs);

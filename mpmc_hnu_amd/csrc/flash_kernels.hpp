@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 namespace mpmc::dev {
@@ -129,21 +130,22 @@ __global__ __launch_bounds__(64) void flash_vle_kernel(const double* __restrict_
     __shared__ Mix<NC> smix;
     load_mix<NC>(mixc, smix);
     __syncthreads();
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= ncell) { return; }
-    double feed[NC];
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double feed[NC];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
-    Counters cnt{0, 0, 0, 0};
-    bool rejected = false;
-    VleOut<NC> V;
-    solve_vle<NC>(smix, p[c], t[c], feed, fug_tol, nullptr, 0, V, cnt, rejected);
-    if (rejected) {
-        PhaseView<NC> ph[3];
-        write_cell<NC>(out, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
-        return;
+        for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
+        Counters cnt{0, 0, 0, 0};
+        bool rejected = false;
+        VleOut<NC> V;
+        solve_vle<NC>(smix, p[c], t[c], feed, fug_tol, nullptr, 0, V, cnt, rejected);
+        if (rejected) {
+            PhaseView<NC> ph[3];
+            write_cell<NC>(out, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+            continue;
+        }
+        publish_vle<NC>(out, ncell, c, V, 0xF, cnt);
     }
-    publish_vle<NC>(out, ncell, c, V, 0xF, cnt);
 }
 
 template <int NC>
@@ -153,21 +155,22 @@ __global__ __launch_bounds__(64) void flash_max3_kernel(const double* __restrict
     __shared__ Mix<NC> smix;
     load_mix<NC>(mixc, smix);
     __syncthreads();
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= ncell) { return; }
-    double feed[NC];
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double feed[NC];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
-    Counters cnt{0, 0, 0, 0};
-    bool rejected = false;
-    Max3Out<NC> M;
-    solve_max3<NC>(smix, p[c], t[c], feed, fug_tol, M, cnt, rejected);
-    if (rejected) {
-        PhaseView<NC> ph[3];
-        write_cell<NC>(out, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
-        return;
+        for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
+        Counters cnt{0, 0, 0, 0};
+        bool rejected = false;
+        Max3Out<NC> M;
+        solve_max3<NC>(smix, p[c], t[c], feed, fug_tol, M, cnt, rejected);
+        if (rejected) {
+            PhaseView<NC> ph[3];
+            write_cell<NC>(out, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+            continue;
+        }
+        publish_max3<NC>(out, ncell, c, M, cnt);
     }
-    publish_max3<NC>(out, ncell, c, M, cnt);
 }
 
 template <int NC>
@@ -245,13 +248,35 @@ namespace mpmc::dev {
 
 constexpr int kBlock = 64;
 
+// Grid of a one-lane-per-cell kernel with `lane_bytes` of private memory per lane: every
+// cell's block, capped so that one dispatch's scratch (lane bytes x 64 x resident waves,
+// backed on the dispatch's hardware queue) stays within kDispatchScratch; the kernels
+// stride over the cells. At least 64 workgroups.
+constexpr size_t kDispatchScratch = size_t(256) << 20;
+inline unsigned capped_cell_blocks(int64_t ncell, size_t lane_bytes) {
+    const int64_t want = (ncell + kBlock - 1) / kBlock;
+    const int64_t cap = lane_bytes == 0 ? want
+                                        : std::max<int64_t>(64, static_cast<int64_t>(kDispatchScratch / (lane_bytes * 64)));
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min(want, cap)));
+}
+
+template <typename K>
+inline size_t kernel_lane_scratch(K kernel) {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess) { return 64 * 1024; }
+    return a.localSizeBytes;
+}
+
 template <int NC>
 hipError_t launch_flash(const double* constants, int64_t ncell, const double* p, const double* t, const double* z,
                         int max_phases, double fug_tol, const FlashOut& out, hipStream_t s) {
-    const dim3 grid(static_cast<unsigned>((ncell + kBlock - 1) / kBlock));
+    static const size_t vle_lane = kernel_lane_scratch(flash_vle_kernel<NC>);
+    static const size_t max3_lane = kernel_lane_scratch(flash_max3_kernel<NC>);
     if (max_phases == 2) {
+        const dim3 grid(capped_cell_blocks(ncell, vle_lane));
         hipLaunchKernelGGL(flash_vle_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, z, fug_tol, out);
     } else {
+        const dim3 grid(capped_cell_blocks(ncell, max3_lane));
         hipLaunchKernelGGL(flash_max3_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, z, fug_tol, out);
     }
     return hipGetLastError();

@@ -1196,7 +1196,15 @@ inline unsigned persistent_blocks(K kernel, size_t lds) {
 // once; a multi-mixture table is sized per call).
 struct PipeGrids {
     unsigned tpd, split, tail, fb, tpd32, split32;
+    unsigned prep, publish, polish;   // caps of the grid-stride kernels with private memory
 };
+
+// Largest grid up to 16384 workgroups whose scratch reservation (private bytes per lane x
+// 64 x workgroups) stays within kDispatchScratch (at least 64 workgroups).
+template <typename K>
+inline unsigned scratch_blocks(K kernel) {
+    return capped_cell_blocks(int64_t{16384} * kBlock, kernel_lane_scratch(kernel));
+}
 template <int NC>
 PipeGrids pipe_grids(size_t lds) {
     PipeGrids g{};
@@ -1210,6 +1218,9 @@ PipeGrids pipe_grids(size_t lds) {
     g.fb = std::min(persistent_blocks(k_fallback<NC, false>, lds), 16u);
     g.tpd32 = persistent_blocks(k_tpd32<NC>, lds);
     g.split32 = persistent_blocks(k_split32<NC>, lds);
+    g.prep = scratch_blocks(k_prep<NC, false>);
+    g.publish = scratch_blocks(k_publish<NC>);
+    g.polish = scratch_blocks(k_polish<NC>);
     return g;
 }
 
@@ -1222,7 +1233,7 @@ void launch_fp64(const Pipe<NC>& P, const double* mixc, int max_phases, const Pi
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
     // a multi-mixture table is staged in LDS by every workgroup: fewer, longer-lived
     // workgroups for the cell-parallel preparation
-    const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
+    const unsigned prep_blocks = std::min(cell_blocks, P.nmix > 1 ? std::min(b.prep, 2048u) : b.prep);
     hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0, 0);
     hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
@@ -1231,7 +1242,7 @@ void launch_fp64(const Pipe<NC>& P, const double* mixc, int max_phases, const Pi
     hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 0);
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, false>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
-    hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_publish<NC>, dim3(std::min(cell_blocks, b.publish)), dim3(64), 0, s, P);
     hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
 }
 
@@ -1241,7 +1252,7 @@ template <int NC>
 void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds, hipStream_t s) {
     const int64_t n = P.n;
     const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
-    const unsigned prep_blocks = P.nmix > 1 ? std::min(cell_blocks, 2048u) : cell_blocks;
+    const unsigned prep_blocks = std::min(cell_blocks, P.nmix > 1 ? std::min(b.prep, 2048u) : b.prep);
     P.mixed = 1;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
     hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
@@ -1251,7 +1262,7 @@ void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrid
     hipLaunchKernelGGL(k_split32<NC>, dim3(b.split32), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL(k_split_gate<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     hipLaunchKernelGGL((k_split<NC, false>), dim3(b.split), dim3(64), lds, s, P, mixc, 1);
-    hipLaunchKernelGGL(k_polish<NC>, dim3(cell_blocks), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL(k_polish<NC>, dim3(std::min(cell_blocks, b.polish)), dim3(64), lds, s, P, mixc);
     hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 1);
     hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 1);
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
@@ -1259,7 +1270,7 @@ void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrid
     P.mixed = 2;
     hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
     if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, false>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
-    hipLaunchKernelGGL(k_publish<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_publish<NC>, dim3(std::min(cell_blocks, b.publish)), dim3(64), 0, s, P);
     hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
 }
 
