@@ -332,6 +332,22 @@ def serve_leg(seconds: float) -> dict:
         print(f"[bench] serving {name}: p50/p99 small {out[name]['p50_ms_small']:.2f}/{out[name]['p99_ms_small']:.2f} ms, "
               f"all {out[name]['p50_ms_all']:.1f}/{out[name]['p99_ms_all']:.1f} ms, {out[name]['cells_per_s']:.3e} cells/s",
               file=sys.stderr, flush=True)
+    # the latency floor: the same job mix at a tenth of the rate (20 jobs/s), mixed precision
+    light = S.serving_jobs(seconds, rate_per_s=20.0, seed=8)
+    srv = S.Server(slots=2, precision=1)
+    res = S.run_load(srv, light, mixtures)
+    srv.close()
+    n = np.array([r[0] for r in res])
+    lat = np.array([r[1] for r in res]) * 1e3
+    small = n <= 65536
+    out["mixed_at_20_jobs_per_s"] = {
+        "jobs": len(light), "small_jobs": int(small.sum()),
+        "p50_ms_small": float(np.percentile(lat[small], 50)) if small.any() else None,
+        "p99_ms_small": float(np.percentile(lat[small], 99)) if small.any() else None,
+        "p50_ms_all": float(np.percentile(lat, 50)), "p99_ms_all": float(np.percentile(lat, 99))}
+    print(f"[bench] serving mixed at 20 jobs/s: p50/p99 small {out['mixed_at_20_jobs_per_s']['p50_ms_small']:.2f}/"
+          f"{out['mixed_at_20_jobs_per_s']['p99_ms_small']:.2f} ms ({int(small.sum())} small of {len(light)} jobs)",
+          file=sys.stderr, flush=True)
     return out
 
 
