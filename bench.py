@@ -70,6 +70,9 @@ def cpu_baseline(mix, p, t, z, sample: int):
 # HBM bytes of one Newton evaluation at nc=6 on 100x100x50, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE counters (k_asm_cell + k_asm_rows), each scaled by what the counter reads for
 # a plain 1 GiB streaming kernel (profiles/assembly_2p_nc6_100x100x50_kernel_stats.md).
+# HBM bytes of one FP64 flash of the 1M-cell nc=6 grid: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# summed over the launch sequence (profiles/flash_nc6_1M_hbm_traffic.md).
+FLASH_FP64_TRAFFIC_BYTES = 2369085 * 2.0 * 1024 + 12783116 * 1.0 * 1024
 ASSEMBLY_TRAFFIC_BYTES = (36615 + 1072624) * 2.0 * 1024 + (728516 + 2578166) * 1.0 * 1024
 
 
@@ -519,7 +522,8 @@ def main(argv=None):
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": FLASH_FP64_TRAFFIC_BYTES if ncell == 1048576 and nc == 6 else None,
+            "traffic_source": "profiles/flash_nc6_1M_hbm_traffic.md",
             "kernel": "staged FP64 flash (k_tpd + k_split + reduces), whole launch sequence",
             "kernel_ms": kernel_ms,
             "algorithmic_flop_per_launch": flops,
