@@ -130,12 +130,13 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     from mpmc_hnu_amd import scan as S
 
     scan = {}
-    for prec, name in ((0, "fp64"), (1, "mixed")):
-        res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec)
+    # density mode 1 also resolves the single-phase targets (Z of the feed's provider root)
+    for prec, dmode, name in ((0, 0, "fp64"), (1, 0, "mixed"), (1, 1, "mixed_density_mode_1")):
+        res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec, density_mode=dmode)
         times = []
         for _ in range(max(1, min(steps, 3))):
             t0 = time.perf_counter()
-            res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec)
+            res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec, density_mode=dmode)
             times.append((time.perf_counter() - t0) * 1e3)
         codes = np.bincount(res["code"].cpu().numpy(), minlength=4)
         scan[name] = {"ms": float(np.mean(times)), "cells_per_s": grid["ncell"] / (np.mean(times) * 1e-3),

@@ -407,7 +407,15 @@ int mpmc_assembly_plan_destroy(mpmc_assembly_plan_t plan);
  * returns MPMC_E_INVALID_ARGUMENT when it is an error. Every cell's code is written either
  * way. The transition evidence report and the trigger of the reference's candidate
  * (phase_set_transition_flash_adapter.hpp:132-220) are not produced; the caller builds the
- * candidate from the record. The call synchronises the stream to fill the summary. */
+ * candidate from the record. The call synchronises the stream to fill the summary.
+ *
+ * density_mode 0 takes each target phase's Z from the flash record, as the density
+ * resolver of the reference's scanner test does (scanner_test.cpp:26-57); the record of a
+ * single accepted phase publishes no Z (pt_vle_phase_set.hpp:68-73), so every proposal to
+ * one phase is indeterminate. density_mode 1 (no counterpart in the reference: a resolver
+ * for phase-disappearance proposals) gives that phase the Z of its provider root at the
+ * feed (the root the branch word names, from mpmc_phase_batch); it allocates
+ * 8 (4 + 3 nc) bytes per cell for the duration of the call. */
 enum mpmc_scan_code { MPMC_SCAN_SAME = 0, MPMC_SCAN_PROPOSAL = 1, MPMC_SCAN_INDETERMINATE = 2,
                       MPMC_SCAN_ERROR = 3 };
 /* PostSnesPhaseTransitionScanStatus3D (post_snes_phase_transition_controller.hpp:78-81) */
@@ -432,8 +440,8 @@ typedef struct mpmc_scan_summary {
  * scan_post_snes_pt_flash_source_cell_3d call. */
 int mpmc_transition_scan(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t, const double* z,
                          const int32_t* source_phase_count, const mpmc_flash_options* options,
-                         const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan, mpmc_scan_summary* summary,
-                         void* stream);
+                         int density_mode, const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan,
+                         mpmc_scan_summary* summary, void* stream);
 
 /* Scan of a converged two-phase state q[Q][ncell] (the assembly's natural variables) on
  * `grid`: each cell's source is its current pore-volume inventory (mpmc_inventory_2p, into
@@ -444,7 +452,7 @@ int mpmc_transition_scan(mpmc_mixture_t mixture, int64_t ncell, const double* p,
  * converged. */
 int mpmc_transition_scan_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
                             const double* q, const int32_t* branch, double* inventory, double* feed,
-                            const mpmc_flash_options* options, const mpmc_flash_outputs* out,
+                            const mpmc_flash_options* options, int density_mode, const mpmc_flash_outputs* out,
                             const mpmc_scan_outputs* scan, mpmc_scan_summary* summary, void* stream);
 
 /* ---------------------------------------------------------------- serving

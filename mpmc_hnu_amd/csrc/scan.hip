@@ -108,11 +108,16 @@ __global__ void k_scan_source_2p(int nc, int64_t n, const double* __restrict__ q
 // accepted phases, the density resolver of the reference's scanner test
 // (post_snes_pt_flash_phase_transition_scanner_test.cpp:26-57): a density that is not
 // finite and positive leaves the cell indeterminate.
+// zroot (density mode 1, else null): Z of every root of the cubic at the cell's feed
+// (mpmc_phase_batch); a single accepted phase, whose record publishes no Z (as the
+// reference's single-phase projection, pt_vle_phase_set.hpp:68-73), then takes the Z of
+// its provider root (branch word of phase 0).
 // stop[0] / stop[1]: lowest index of an indeterminate / error cell; stop[2]: proposals.
 __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const double* __restrict__ t,
                               const int32_t* __restrict__ count, int32_t count_all,
                               const int32_t* __restrict__ status, const int32_t* __restrict__ phase_count,
-                              const double* __restrict__ zfactor, int32_t* __restrict__ code,
+                              const double* __restrict__ zfactor, const int32_t* __restrict__ branch,
+                              const double* __restrict__ zroot, int32_t* __restrict__ code,
                               int32_t* __restrict__ target_count, double* __restrict__ density,
                               unsigned long long* __restrict__ stop) {
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
@@ -134,7 +139,11 @@ __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const dou
                     k = MPMC_SCAN_PROPOSAL;
                     for (int ph = 0; ph < MPMC_MAX_PHASES; ++ph) {
                         if (ph >= target) { continue; }
-                        const double zf = zfactor[ph * n + c];
+                        double zf = zfactor[ph * n + c];
+                        if (zroot && target == 1 && !finite(zf)) {
+                            const int root = MPMC_BRANCH_INDEX(branch[c]);
+                            zf = root >= 0 && root < 3 ? zroot[root * n + c] : __builtin_nan("");
+                        }
                         const double den = zf * kGasConstant * t[c];
                         const double r = p[c] / den;
                         if (!finite(zf) || !(zf > 0) || !finite(r) || !(r > 0)) { k = MPMC_SCAN_INDETERMINATE; }
@@ -170,21 +179,57 @@ unsigned grid_for(int64_t n) {
 int scan_finish(mpmc_mixture_t mixture, int64_t n, const double* p, const double* t, const double* z,
                 const int32_t* count, int32_t count_all, const mpmc_flash_options* options,
                 const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan, mpmc_scan_summary* summary,
-                hipStream_t s) {
+                int density_mode, hipStream_t s) {
     int rc = mpmc_flash_batch(mixture, n, p, t, z, options, out, s);
     if (rc != MPMC_OK) { return rc; }
+    // density mode 1: every root of the cubic at the cell's feed (root status and count, Z
+    // and flags of 3 roots, ln phi of 3 roots), in one device buffer
+    char* phase = nullptr;
+    const double* zroot = nullptr;
+    if (density_mode == 1 && n > 0) {
+        const int nc = mpmc_mixture_components(mixture);
+        if (nc < 2) { return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: bad mixture"); }
+        const size_t bytes = static_cast<size_t>(n) * (2 * 4 + 3 * 8 + 3 * 4 + 3 * 8 * static_cast<size_t>(nc)) + 16;
+        if (hipMalloc(reinterpret_cast<void**>(&phase), bytes) != hipSuccess) {
+            return scan_fail(MPMC_E_NO_MEMORY, "transition_scan: device allocation failed");
+        }
+        auto* rs = reinterpret_cast<int32_t*>(phase);
+        auto* rcnt = rs + n;
+        auto* zr = reinterpret_cast<double*>(rcnt + n);  // 2n int32: 8-byte aligned
+        auto* rf = reinterpret_cast<int32_t*>(zr + 3 * n);
+        auto* lp = reinterpret_cast<double*>(rf + 3 * n + ((3 * n) & 1));
+        rc = mpmc_phase_batch(mixture, n, p, t, z, rs, rcnt, zr, rf, lp, s);
+        if (rc != MPMC_OK) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(phase);
+            return rc;
+        }
+        zroot = zr;
+    }
     unsigned long long* stop = nullptr;
     hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&stop), 3 * sizeof(unsigned long long), s);
-    if (e != hipSuccess) { return scan_fail(MPMC_E_NO_MEMORY, "transition_scan: device allocation failed"); }
+    if (e != hipSuccess) {
+        if (phase) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(phase);
+        }
+        return scan_fail(MPMC_E_NO_MEMORY, "transition_scan: device allocation failed");
+    }
     unsigned long long host[3] = {~0ull, ~0ull, 0};
     hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, s, stop);
     hipLaunchKernelGGL(k_scan_decide, dim3(grid_for(n)), dim3(256), 0, s, n, p, t, count, count_all, out->status,
-                       out->phase_count, out->zfactor, scan->code, scan->target_count, scan->target_density, stop);
+                       out->phase_count, out->zfactor, out->branch, zroot, scan->code, scan->target_count,
+                       scan->target_density, stop);
     e = hipGetLastError();
     if (e == hipSuccess) { e = hipMemcpyAsync(host, stop, sizeof(host), hipMemcpyDeviceToHost, s); }
     const hipError_t ef = hipFreeAsync(stop, s);
     if (e == hipSuccess) { e = ef; }
-    if (e == hipSuccess) { e = hipStreamSynchronize(s); }
+    const hipError_t es = hipStreamSynchronize(s);
+    if (e == hipSuccess) { e = es; }
+    if (phase) {
+        const hipError_t ep = hipFree(phase);
+        if (e == hipSuccess) { e = ep; }
+    }
     if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan: ") + hipGetErrorString(e)); }
     // the reference returns at the first indeterminate cell (status indeterminate) or at the
     // first cell that raises an error (the error code, status untouched)
@@ -200,15 +245,18 @@ int scan_finish(mpmc_mixture_t mixture, int64_t n, const double* p, const double
 }
 
 int scan_check(mpmc_mixture_t mixture, int64_t n, const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan,
-               const mpmc_scan_summary* summary) {
+               const mpmc_scan_summary* summary, int density_mode) {
     if (!mixture || !out || !scan || !summary || n < 0) {
         return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: null argument or negative cell count");
+    }
+    if (density_mode != 0 && density_mode != 1) {
+        return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: density_mode must be 0 or 1");
     }
     if (mpmc_mixture_components(mixture) < 2) {
         return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: a PT source needs at least two components");
     }
     if (n > 0 && (!scan->code || !scan->target_count || !scan->target_density || !out->status ||
-                  !out->phase_count || !out->zfactor)) {
+                  !out->phase_count || !out->zfactor || !out->branch)) {
         return scan_fail(MPMC_E_INVALID_ARGUMENT, "transition_scan: null output buffer");
     }
     return MPMC_OK;
@@ -220,9 +268,9 @@ extern "C" {
 
 int mpmc_transition_scan(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t, const double* z,
                          const int32_t* source_phase_count, const mpmc_flash_options* options,
-                         const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan, mpmc_scan_summary* summary,
-                         void* stream) {
-    int rc = scan_check(mixture, ncell, out, scan, summary);
+                         int density_mode, const mpmc_flash_outputs* out, const mpmc_scan_outputs* scan,
+                         mpmc_scan_summary* summary, void* stream) {
+    int rc = scan_check(mixture, ncell, out, scan, summary, density_mode);
     if (rc != MPMC_OK) { return rc; }
     *summary = mpmc_scan_summary{MPMC_SCAN_COMPLETE, -1, 0};
     if (ncell == 0) { return MPMC_OK; }
@@ -233,17 +281,17 @@ int mpmc_transition_scan(mpmc_mixture_t mixture, int64_t ncell, const double* p,
                        p, t, z, source_phase_count, max_phases, scan->code);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan: ") + hipGetErrorString(e)); }
-    return scan_finish(mixture, ncell, p, t, z, source_phase_count, 0, options, out, scan, summary, s);
+    return scan_finish(mixture, ncell, p, t, z, source_phase_count, 0, options, out, scan, summary, density_mode, s);
 }
 
 int mpmc_transition_scan_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
                             const double* q, const int32_t* branch, double* inventory, double* feed,
-                            const mpmc_flash_options* options, const mpmc_flash_outputs* out,
+                            const mpmc_flash_options* options, int density_mode, const mpmc_flash_outputs* out,
                             const mpmc_scan_outputs* scan, mpmc_scan_summary* summary, void* stream) {
     int64_t ncell = 0;
     int rc = mpmc_tpfa_grid_cells(grid, &ncell);
     if (rc != MPMC_OK) { return rc; }
-    rc = scan_check(mixture, ncell, out, scan, summary);
+    rc = scan_check(mixture, ncell, out, scan, summary, density_mode);
     if (rc != MPMC_OK) { return rc; }
     *summary = mpmc_scan_summary{MPMC_SCAN_COMPLETE, -1, 0};
     if (ncell == 0) { return MPMC_OK; }
@@ -257,7 +305,7 @@ int mpmc_transition_scan_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const
                        scan->code);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan_2p: ") + hipGetErrorString(e)); }
-    return scan_finish(mixture, ncell, q, q + ncell, feed, nullptr, 2, options, out, scan, summary, s);
+    return scan_finish(mixture, ncell, q, q + ncell, feed, nullptr, 2, options, out, scan, summary, density_mode, s);
 }
 
 } // extern "C"

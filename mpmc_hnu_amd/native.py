@@ -228,10 +228,10 @@ def library() -> ctypes.CDLL:
     lib.mpmc_assembly_plan_domain_error.argtypes = [P, P, ctypes.POINTER(ctypes.c_int32)]
     lib.mpmc_assembly_plan_destroy.argtypes = [P]
     lib.mpmc_transition_scan.argtypes = [P, ctypes.c_int64, P, P, P, P, ctypes.POINTER(FlashOptions),
-                                         ctypes.POINTER(FlashOutputs), ctypes.POINTER(ScanOutputs),
+                                         ctypes.c_int, ctypes.POINTER(FlashOutputs), ctypes.POINTER(ScanOutputs),
                                          ctypes.POINTER(ScanSummary), P]
     lib.mpmc_transition_scan_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, P,
-                                            ctypes.POINTER(FlashOptions), ctypes.POINTER(FlashOutputs),
+                                            ctypes.POINTER(FlashOptions), ctypes.c_int, ctypes.POINTER(FlashOutputs),
                                             ctypes.POINTER(ScanOutputs), ctypes.POINTER(ScanSummary), P]
     lib.mpmc_server_default_options.argtypes = [ctypes.POINTER(ServerOptions)]
     lib.mpmc_server_default_options.restype = None

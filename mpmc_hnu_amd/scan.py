@@ -7,7 +7,11 @@ post_snes_pt_flash_phase_transition_scanner.hpp).
 Per cell the scan flashes the cell's current PT feed and compares the accepted phase count
 with the cell's current one: SAME (no proposal), PROPOSAL (another accepted count, with the
 target phases' molar densities p / (Z R T)), INDETERMINATE, ERROR. The summary is the
-reference's stop-at-first-cell outcome over the batch in cell order."""
+reference's stop-at-first-cell outcome over the batch in cell order.
+
+density_mode 0 reads Z from the flash record, as the reference's scanner test resolver does,
+so a proposal to a single phase (whose record publishes no Z) is indeterminate; 1 gives that
+phase the Z of its provider root at the feed (an extension with no reference counterpart)."""
 from __future__ import annotations
 
 import ctypes
@@ -55,7 +59,7 @@ def _finish(rc: int, what: str, res: dict, summary: native.ScanSummary) -> dict:
 
 
 def transition_scan(mixture: Mixture, p, t, z, source_phase_count, fugacity_tolerance: float = 1e-11,
-                    max_phases: int = 3, precision: int = 0, stream=None) -> dict:
+                    max_phases: int = 3, precision: int = 0, density_mode: int = 0, stream=None) -> dict:
     """Scan caller-given sources: p[ncell] Pa, t[ncell] K, z[nc, ncell], the cells' current
     phase counts source_phase_count[ncell]. Returns the scan codes, target counts and
     densities, the flash record ("flash") and the summary (status, first_stop, proposals,
@@ -77,7 +81,8 @@ def transition_scan(mixture: Mixture, p, t, z, source_phase_count, fugacity_tole
     summary = native.ScanSummary(0, -1, 0)
     fo = _outputs_struct(out)
     rc = lib.mpmc_transition_scan(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
-                                  V(cnt.data_ptr()), ctypes.byref(opts), ctypes.byref(fo), ctypes.byref(so),
+                                  V(cnt.data_ptr()), ctypes.byref(opts), int(density_mode), ctypes.byref(fo),
+                                  ctypes.byref(so),
                                   ctypes.byref(summary), _stream_ptr(stream))
     res["flash"] = out
     return _finish(rc, "transition_scan", res, summary)
@@ -85,7 +90,7 @@ def transition_scan(mixture: Mixture, p, t, z, source_phase_count, fugacity_tole
 
 def transition_scan_2p(mixture: Mixture, grid: Grid, model: native.AssemblyModel, q, branch,
                        fugacity_tolerance: float = 1e-11, max_phases: int = 3, precision: int = 0,
-                       stream=None) -> dict:
+                       density_mode: int = 0, stream=None) -> dict:
     """Scan a converged two-phase state q[2nc+1, ncell] (with the phase root branches of the
     assembly) on `grid`: each cell's feed is its normalised pore-volume inventory, its
     current phase count 2. Returns the scan result of transition_scan plus "inventory"
@@ -109,7 +114,7 @@ def transition_scan_2p(mixture: Mixture, grid: Grid, model: native.AssemblyModel
     fo = _outputs_struct(out)
     rc = lib.mpmc_transition_scan_2p(mixture.handle, grid.h, ctypes.byref(model), V(q.data_ptr()),
                                      V(branch.data_ptr()), V(res["inventory"].data_ptr()), V(res["feed"].data_ptr()),
-                                     ctypes.byref(opts), ctypes.byref(fo), ctypes.byref(so), ctypes.byref(summary),
-                                     _stream_ptr(stream))
+                                     ctypes.byref(opts), int(density_mode), ctypes.byref(fo), ctypes.byref(so),
+                                     ctypes.byref(summary), _stream_ptr(stream))
     res["flash"] = out
     return _finish(rc, "transition_scan_2p", res, summary)

@@ -9,7 +9,8 @@ post_snes_pt_flash_phase_transition_scanner.hpp:
   source_feed       make_source_snapshot (:160-200), the feed from the current inventory
   decide            scan_post_snes_pt_flash_source_cell_3d (:203-301) on a flash record,
                     with the density resolver of the reference's scanner test
-                    (tests/flow_discretization/petsc/post_snes_pt_flash_phase_transition_scanner_test.cpp:26-57)
+                    (tests/flow_discretization/petsc/post_snes_pt_flash_phase_transition_scanner_test.cpp:26-57),
+                    or (single_z) with a single phase's Z supplied by the caller
   summary           the cell loop of scan_post_snes_pt_flash_phase_transitions_3d (:330-420):
                     the first indeterminate cell ends the scan as indeterminate, the first
                     error cell ends it with the error
@@ -65,9 +66,13 @@ def source_feed(inventory):
     return feed, ok
 
 
-def decide(source_count, p, t, status, phase_count, zfactor, code_in=None):
+def decide(source_count, p, t, status, phase_count, zfactor, code_in=None, single_z=None):
     """Per-cell code, target count (the accepted phase count, 0 without an accepted set)
-    and target densities (proposals only) from a flash record (:229-292)."""
+    and target densities (proposals only) from a flash record (:229-292).
+
+    single_z (density mode 1, an extension with no reference counterpart): per cell, the Z
+    that a single accepted phase takes when the record publishes none (the Z of its
+    provider root at the feed)."""
     status = np.asarray(status)
     n = status.size
     src = np.broadcast_to(np.asarray(source_count), (n,))
@@ -91,6 +96,8 @@ def decide(source_count, p, t, status, phase_count, zfactor, code_in=None):
         rho = np.full(3, np.nan)
         for ph in range(target[c]):
             zf = float(zfactor[ph][c])
+            if single_z is not None and target[c] == 1 and not np.isfinite(zf):
+                zf = float(single_z[c])
             den = zf * GAS_CONSTANT * float(t[c])
             r = float(p[c]) / den if den != 0 else np.inf
             if not (np.isfinite(zf) and zf > 0 and np.isfinite(r) and r > 0):

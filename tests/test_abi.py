@@ -60,11 +60,11 @@ def test_invalid_arguments_are_rejected_without_device_work():
     # summary is left untouched
     summary = native.ScanSummary(7, 5, 3)
     scan = native.ScanOutputs(None, None, None)
-    rc = lib.mpmc_transition_scan(None, 4, None, None, None, None, None, None, ctypes.byref(scan),
+    rc = lib.mpmc_transition_scan(None, 4, None, None, None, None, None, 0, None, ctypes.byref(scan),
                                   ctypes.byref(summary), None)
     assert rc == 1 and b"transition_scan" in lib.mpmc_last_error()
     assert (summary.status, summary.first_stop, summary.proposals) == (7, 5, 3)
-    rc = lib.mpmc_transition_scan_2p(None, None, None, None, None, None, None, None, None, ctypes.byref(scan),
+    rc = lib.mpmc_transition_scan_2p(None, None, None, None, None, None, None, None, 0, None, ctypes.byref(scan),
                                      ctypes.byref(summary), None)
     assert rc == 1
     n = ctypes.c_int64(-1)

@@ -6,7 +6,9 @@ Bar: the scan rules applied to the device's own flash record give bit-identical 
 target counts and target densities (the decision kernel is integer logic plus one IEEE
 quotient); the flash record itself meets the flash's parity bar against the checker
 flash; the two-phase feeds are within one ulp of the long double normalisation of the
-device inventory, and the inventory within the assembly's 1e-8 of the oracle's."""
+device inventory, and the inventory within the assembly's 1e-8 of the oracle's. Density
+mode 1 takes a single target phase's Z from the oracle's phase routine at the feed (bit-
+identical to the device's, as the phase batch test shows) at the record's branch."""
 import numpy as np
 import pytest
 
@@ -23,11 +25,12 @@ def _host(t):
     return t.cpu().numpy()
 
 
-def _check_rules(g, p, t, cnt, code0):
+def _check_rules(g, p, t, cnt, code0, single_z=None):
     from mpmc_hnu_amd import flash
 
     rec = flash.to_host(g["flash"])
-    code, target, dens = S.decide(cnt, p, t, rec["status"], rec["phase_count"], rec["zfactor"], code0)
+    code, target, dens = S.decide(cnt, p, t, rec["status"], rec["phase_count"], rec["zfactor"], code0,
+                                  single_z=single_z(rec) if single_z else None)
     assert np.array_equal(_host(g["code"]), code)
     assert np.array_equal(_host(g["target_count"]), target)
     gd = _host(g["target_density"])
@@ -146,3 +149,42 @@ def test_mixed_precision_scan_applies_the_same_rules(gpu):
         m.close()
     _, code = _check_rules(g, p, t, cnt, np.zeros(p.size, np.int32))
     assert np.any(code == S.PROPOSAL)
+
+
+def _provider_z(mix, p, t, z):
+    """Z of the provider root (the branch word of phase 0) of each cell's feed, from the
+    oracle's phase routine."""
+    roots = ob.phase(mix, p, t, z)["zroot"]
+
+    def pick(rec):
+        idx = rec["branch"][0] & 0xFF
+        ok = (rec["branch"][0] >= 0) & (idx < 3)
+        return np.where(ok, roots[np.minimum(idx, 2), np.arange(p.size)], np.nan)
+    return pick
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_single_phase_density_mode(gpu, precision):
+    """density_mode 1: 2 -> 1 proposals get p / (Z R T) of the feed's provider root; every
+    other decision is the one of density mode 0."""
+    from mpmc_hnu_amd import flash, scan
+
+    mix, p, t, z = W.sour_gas_sweep(1024, seed=4)
+    cnt = np.full(p.size, 2, np.int32)
+    print(f"[case] density mode 1 scan of 1024 sour-gas sources, precision {precision}", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        g1 = scan.transition_scan(m, p, t, z, cnt, precision=precision, density_mode=1)
+        g0 = scan.transition_scan(m, p, t, z, cnt, precision=precision)
+    finally:
+        m.close()
+    zero = np.zeros(p.size, np.int32)
+    _, code1 = _check_rules(g1, p, t, cnt, zero, single_z=_provider_z(mix, p, t, z))
+    _, code0 = _check_rules(g0, p, t, cnt, zero)
+    one = _host(g1["target_count"]) == 1
+    assert np.sum((code1 == S.PROPOSAL) & one) > 10
+    if precision == 0:
+        # the FP64 flash is deterministic: the two calls share their record
+        assert np.array_equal(code0[~one], code1[~one])
+        assert np.all(code0[one] == S.INDETERMINATE)

@@ -108,3 +108,18 @@ def test_pr76_two_to_three_proposal_with_the_checker_flash():
     code, _, _ = S.decide([3], p[c:c + 1], t[c:c + 1], o["status"][c:c + 1], o["phase_count"][c:c + 1],
                           o["zfactor"][:, c:c + 1])
     assert code[0] == S.SAME
+
+
+def test_single_phase_density_from_the_supplied_root():
+    """density mode 1: a 2 -> 1 proposal takes p / (Z R T) of the supplied Z; without it
+    (the record publishes no single-phase Z) the cell stays indeterminate; a supplied Z that
+    is not positive is indeterminate too; multi-phase targets ignore it."""
+    p, t = np.array([2.0e6, 2.0e6, 2.0e6]), np.array([320.0, 320.0, 320.0])
+    status = np.array([_accepted(1), _accepted(1), _accepted(2)])
+    zf = np.array([[np.nan, np.nan, 0.9], [np.nan, np.nan, 0.1], [np.nan] * 3])
+    code, _, _ = S.decide([2, 2, 1], p, t, status, [1, 1, 2], zf)
+    assert code.tolist() == [S.INDETERMINATE, S.INDETERMINATE, S.PROPOSAL]
+    code, target, dens = S.decide([2, 2, 1], p, t, status, [1, 1, 2], zf, single_z=[0.85, -0.2, 5.0])
+    assert code.tolist() == [S.PROPOSAL, S.INDETERMINATE, S.PROPOSAL] and target.tolist() == [1, 1, 2]
+    assert dens[0, 0] == 2.0e6 / (0.85 * R * 320.0) and np.all(np.isnan(dens[1:, 0]))
+    assert dens[0, 2] == 2.0e6 / (0.9 * R * 320.0)
