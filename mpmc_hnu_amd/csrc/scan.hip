@@ -157,9 +157,15 @@ __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const dou
         for (int ph = 0; ph < MPMC_MAX_PHASES; ++ph) {
             density[ph * n + c] = k == MPMC_SCAN_PROPOSAL ? rho[ph] : __builtin_nan("");
         }
-        if (k == MPMC_SCAN_INDETERMINATE) { atomicMin(&stop[0], static_cast<unsigned long long>(c)); }
-        if (k == MPMC_SCAN_ERROR) { atomicMin(&stop[1], static_cast<unsigned long long>(c)); }
-        if (k == MPMC_SCAN_PROPOSAL) { atomicAdd(&stop[2], 1ull); }
+        // one atomic per wave: a wave's cells are consecutive in lane order, so its lowest
+        // flagged lane holds its lowest flagged cell
+        const int lane = static_cast<int>(__lane_id());
+        const unsigned long long indet = __ballot(k == MPMC_SCAN_INDETERMINATE);
+        const unsigned long long err = __ballot(k == MPMC_SCAN_ERROR);
+        const unsigned long long prop = __ballot(k == MPMC_SCAN_PROPOSAL);
+        if (indet && lane == __builtin_ctzll(indet)) { atomicMin(&stop[0], static_cast<unsigned long long>(c)); }
+        if (err && lane == __builtin_ctzll(err)) { atomicMin(&stop[1], static_cast<unsigned long long>(c)); }
+        if (prop && lane == __builtin_ctzll(prop)) { atomicAdd(&stop[2], static_cast<unsigned long long>(__popcll(prop))); }
     }
 }
 
