@@ -188,3 +188,22 @@ def test_single_phase_density_mode(gpu, precision):
         # the FP64 flash is deterministic: the two calls share their record
         assert np.array_equal(code0[~one], code1[~one])
         assert np.all(code0[one] == S.INDETERMINATE)
+
+
+@pytest.mark.gpu
+def test_density_mode_outside_0_1_is_an_argument_error(gpu):
+    """An unknown density_mode is rejected before any launch, and the summary is left as
+    the caller initialised it."""
+    from mpmc_hnu_amd import flash, native, scan
+
+    mix = W.mixture_linspaced(3)
+    p, t, z = W.cells_uniform(3, 64, seed=2)
+    print("[case] density_mode 2 rejected, 64 cells nc=3", flush=True)
+    m = flash.Mixture.from_dict(mix)
+    try:
+        with pytest.raises(native.MpmcError, match="density_mode"):
+            scan.transition_scan(m, p, t, z, np.full(p.size, 2, np.int32), density_mode=2)
+        g = scan.transition_scan(m, p, t, z, np.full(p.size, 2, np.int32), density_mode=1)
+    finally:
+        m.close()
+    _check_rules(g, p, t, 2, np.zeros(p.size, np.int32), single_z=_provider_z(mix, p, t, z))
