@@ -130,8 +130,7 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     from mpmc_hnu_amd import scan as S
 
     scan = {}
-    # density mode 1 also resolves the single-phase targets (Z of the feed's provider root)
-    for prec, dmode, name in ((0, 0, "fp64"), (1, 0, "mixed"), (1, 1, "mixed_density_mode_1")):
+    for prec, dmode, name in ((0, 0, "fp64"), (1, 0, "mixed")):
         res = S.transition_scan_2p(m, g, md, qd, bd, precision=prec, density_mode=dmode)
         times = []
         for _ in range(max(1, min(steps, 3))):
@@ -166,17 +165,14 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     return out
 
 
-def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
+def uneven_workload(scale: float = 1.0):
     """The uneven multi-mixture batch: 256 mixtures, nc ~ U{2..12}, cells per mixture
-    log-uniform on [1e3, 2e5] (about 9.7M cells), 30% of each mixture's cells within +-2%
-    in P of a phase boundary found by scan and bisection on P with the GPU flash. Timed as
-    one flash plan (ragged launches grouped by nc, captured in a HIP graph) against the same
-    segments flashed one mixture at a time; both must give identical records."""
-    import torch
-
+    log-uniform on [1e3, 2e5] times ``scale`` (about 9.7M cells at scale 1), 30% of each
+    mixture's cells within +-2% in P of a phase boundary found by scan and bisection on P
+    with the GPU flash. Returns (layout, mixtures, [(p, t, z)] per mixture, cells,
+    near-boundary cells)."""
     from mpmc_hnu_amd import flash, workloads as W
 
-    t0 = time.perf_counter()
     layout = W.uneven_layout(scale=scale)
     mixes = [flash.Mixture.from_dict(m["mixture"]) for m in layout]
     base = W.uneven_base_points(layout)
@@ -203,11 +199,24 @@ def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
 
     pb = W.boundary_pressures(phase_count, base)
     cells = W.uneven_cells(layout, base, pb)
+    ncell = int(sum(m["ncell"] for m in layout))
+    near = int(sum(round(0.3 * m["ncell"]) if np.isfinite(b).any() else 0 for m, b in zip(layout, pb)))
+    return layout, mixes, cells, ncell, near
+
+
+def uneven_leg(steps: int, warmup: int, scale: float = 1.0) -> dict:
+    """The uneven multi-mixture batch (uneven_workload). Timed as one flash plan (ragged
+    launches grouped by nc, captured in a HIP graph) against the same segments flashed one
+    mixture at a time; both must give identical records."""
+    import torch
+
+    from mpmc_hnu_amd import flash
+
+    t0 = time.perf_counter()
+    layout, mixes, cells, ncell, near = uneven_workload(scale)
     dev = torch.device("cuda", 0)
     segments = [(mixes[k], torch.as_tensor(p, device=dev), torch.as_tensor(t, device=dev),
                  torch.as_tensor(z, device=dev)) for k, (p, t, z) in enumerate(cells)]
-    ncell = int(sum(m["ncell"] for m in layout))
-    near = int(sum(round(0.3 * m["ncell"]) if np.isfinite(b).any() else 0 for m, b in zip(layout, pb)))
     print(f"[bench] uneven: {len(layout)} mixtures, {ncell} cells ({near} near a boundary), "
           f"built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()

@@ -87,15 +87,23 @@ typedef struct mpmc_flash_options {
      * 3: Pr76PtFlashBackend::solve (max3, pr76_pt_flash_backend.hpp:190-237) */
     int32_t max_phases;
     /* 0: every evaluation in FP64 with the reference's bisection roots.
-     * 1: mixed precision. TPD descents and split attempts run with mixing, ln(phi), the
-     *    TPD and successive substitution in FP32, Rachford-Rice in FP32 away from the ends
-     *    of (0, 1); cubic brackets, roots and root selection stay FP64. A search whose
-     *    outcome is within FP32 rounding of a threshold (or slow enough to meet the
-     *    iteration limit) is continued in FP64 under the reference's rules; the selected
-     *    split is polished by FP64 Newton and every published value is FP64. Decisions
-     *    can differ from precision 0 where the reference's own search fails (line-search
-     *    failure, iteration limit); results within 1e-8 relative otherwise.
-     *    Staged schedule only. */
+     * 1: decision-exact mixed precision. FP32 descents (mixing, ln(phi), TPD and the
+     *    line search in FP32; cubic brackets, roots and root selection in FP64) screen
+     *    every stability trial. Every search whose outcome or end point bears on a
+     *    decision is run in FP64 from its start: trials FP32 cannot classify beyond
+     *    rounding reach, the negative trials that seed the splits, every split attempt,
+     *    and the whole review of a cell whose review finds instability. Only trials FP32
+     *    classifies as stationary beyond rounding reach keep their FP32 status. Phase
+     *    counts, statuses and labels are those of precision 0; values are FP64.
+     * 2: fast mixed precision. TPD descents and split attempts run in FP32 (Rachford-Rice
+     *    in FP32 away from the ends of (0, 1)); searches within FP32 rounding of a
+     *    threshold, or slow enough to meet the iteration limit, are continued in FP64 from
+     *    their current point; the selected split is polished by FP64 Newton and every
+     *    published value is FP64. Decisions can differ from precision 0 on cells where the
+     *    reference's own split search ends on a line-search failure or the iteration
+     *    limit (269 of 1,048,576 cells of the nc=6 grid workload); values of the other
+     *    cells agree within 1e-7 relative.
+     *    Precisions 1 and 2 run the staged schedule only. */
     int32_t precision;
     /* 0: staged batch (one lane per stability trial / split attempt, persistent kernels)
      * 1: serial (one lane per cell runs the whole search) */
@@ -394,28 +402,34 @@ int mpmc_assembly_plan_destroy(mpmc_assembly_plan_t plan);
  *   MPMC_SCAN_PROPOSAL       accepted set of another size: a transition proposal, with
  *                            target_count phases (the flash record holds their beta, x, Z,
  *                            ln phi) and target phase molar densities p / (Z R T)
- *   MPMC_SCAN_INDETERMINATE  no accepted phase set, a target density that is not finite and
- *                            positive, or (two-phase state scan) a cell whose state or
- *                            inventory cannot define a PT feed
- *   MPMC_SCAN_ERROR          a malformed source or an unsupported source phase count
- *                            (invalid_argument / PETSC_ERR_ARG_INCOMP), or a feed the flash
- *                            rejects
+ *   MPMC_SCAN_INDETERMINATE  no accepted phase set, a feed the flash rejects (its
+ *                            std::domain_error, caught by the scanner, :292-299), a target
+ *                            density that is not finite and positive, or (two-phase state
+ *                            scan) a cell whose state or inventory cannot define a PT feed
+ *   MPMC_SCAN_ERROR          a malformed source or an unsupported source phase count, or a
+ *                            lower target without the backend's phase-disappearance evidence
+ *                            (invalid_argument / PETSC_ERR_ARG_INCOMP; the PR76 backend
+ *                            reports such evidence only for 3 -> 2 through a fresh two-phase
+ *                            neighbour solve, pr76_pt_flash_backend.hpp:138-146;
+ *                            phase_set_transition_flash_adapter.hpp:171-183)
  * The reference walks the cells in order and stops at the first indeterminate cell (scan
  * status indeterminate, proposals dropped) or the first error (the error returned). The
  * summary reports that outcome: first_stop = index of the first such cell (-1: none),
- * status = MPMC_SCAN_STATUS_INDETERMINATE when that cell is indeterminate, and the call
- * returns MPMC_E_INVALID_ARGUMENT when it is an error. Every cell's code is written either
- * way. The transition evidence report and the trigger of the reference's candidate
+ * status = MPMC_SCAN_STATUS_INDETERMINATE when that cell is indeterminate (proposals 0),
+ * and the call returns MPMC_E_INVALID_ARGUMENT when it is an error (proposals: those
+ * before the error cell). Every cell's code is written either way. The transition evidence report and the trigger of the reference's candidate
  * (phase_set_transition_flash_adapter.hpp:132-220) are not produced; the caller builds the
  * candidate from the record. The call synchronises the stream to fill the summary.
  *
  * density_mode 0 takes each target phase's Z from the flash record, as the density
  * resolver of the reference's scanner test does (scanner_test.cpp:26-57); the record of a
  * single accepted phase publishes no Z (pt_vle_phase_set.hpp:68-73), so every proposal to
- * one phase is indeterminate. density_mode 1 (no counterpart in the reference: a resolver
- * for phase-disappearance proposals) gives that phase the Z of its provider root at the
- * feed (the root the branch word names, from mpmc_phase_batch); it allocates
- * 8 (4 + 3 nc) bytes per cell for the duration of the call. */
+ * one phase is indeterminate. density_mode 1 (a resolver with no counterpart in the
+ * reference) gives that phase the Z of its provider root at the feed (the root the branch
+ * word names, from mpmc_phase_batch); the candidate then meets the evidence rule, and since
+ * the PR76 backend reports no disappearance evidence for a single-phase target, such cells
+ * are errors, as the reference's adapter raises for them. It allocates 8 (4 + 3 nc) bytes
+ * per cell for the duration of the call. */
 enum mpmc_scan_code { MPMC_SCAN_SAME = 0, MPMC_SCAN_PROPOSAL = 1, MPMC_SCAN_INDETERMINATE = 2,
                       MPMC_SCAN_ERROR = 3 };
 /* PostSnesPhaseTransitionScanStatus3D (post_snes_phase_transition_controller.hpp:78-81) */
@@ -430,7 +444,9 @@ typedef struct mpmc_scan_outputs {
 typedef struct mpmc_scan_summary {
     int32_t status;          /* mpmc_scan_status */
     int64_t first_stop;      /* first indeterminate or error cell, -1 when none */
-    int64_t proposals;       /* cells with MPMC_SCAN_PROPOSAL */
+    int64_t proposals;       /* proposals the reference's cell loop holds when it returns:
+                                the MPMC_SCAN_PROPOSAL cells before first_stop, 0 when the
+                                status is indeterminate */
 } mpmc_scan_summary;
 
 /* Scan of caller-given PT sources (PostSnesPtFlashSourceCellSnapshot3D, :30-37): p, t,

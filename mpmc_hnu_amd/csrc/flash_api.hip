@@ -174,7 +174,7 @@ static int flash_options(const mpmc_flash_options* options, mpmc_flash_options& 
     if (!std::isfinite(o.fugacity_tolerance) || !(o.fugacity_tolerance > 0)) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: fugacity tolerance must be finite and positive");
     }
-    if (o.precision != 0 && o.precision != 1) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: precision must be 0 or 1"); }
+    if (o.precision < 0 || o.precision > 2) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: precision must be 0, 1 or 2"); }
     if (o.schedule != 0 && o.schedule != 1) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: schedule must be 0 or 1"); }
     if (o.schedule == 1 && o.precision != 0) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: the serial schedule runs FP64 only");

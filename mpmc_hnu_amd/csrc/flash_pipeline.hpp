@@ -70,7 +70,8 @@ enum CellFlag : int32_t { CF_REJECTED = 1, CF_FALLBACK = 2 };
 enum GlobalSlot : int { G_NU = 0, G_NFALLBACK = 1, G_TASK_INIT = 2, G_TASK_SPLIT = 3, G_TASK_REVIEW = 4,
                         G_TASK_MAX3 = 5, G_TASK_FALLBACK = 6, G_TASK_INIT32 = 7, G_TASK_REVIEW32 = 8,
                         G_TASK_SPLIT32 = 9, G_TASK_INIT_RESUME = 10, G_TASK_REVIEW_RESUME = 11,
-                        G_TASK_SPLIT_RESUME = 12, G_TASK_REVIEW_REDO = 13, G_SLOTS = 16 };
+                        G_TASK_SPLIT_RESUME = 12, G_TASK_REVIEW_REDO = 13, G_TASK_REVIEW_REDO2 = 14,
+                        G_SLOTS = 16 };
 
 // Device workspace carved from one allocation (pipe_carve). n = cells; arrays
 // indexed by the compact unstable index u use capacity n.
@@ -121,7 +122,8 @@ struct Pipe {
     // table of nmix mixtures (all with NC components); mix_of == nullptr: one mixture.
     const int32_t* mix_of;
     int nmix;
-    int mixed;          // mixed-precision schedule (precision 1): 1, then 2 after the review redo
+    int mixed;          // mixed-precision schedules: 1, then 2 after the review redo
+    int exact;          // precision 1: every decision-bearing search is (re)run in FP64 from its start
     // warm-start hints: nhint compositions per cell, hints[(h*NC + i)*n + c], appended to the
     // initial stability test's starts in order (NaN in component 0: no hint h for cell c)
     const double* hints;
@@ -470,9 +472,13 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
     const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4 + P.nfhint;
     const int64_t nslot = stage == 0 ? P.n : static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = tk * nslot;
+    // resume 0: every trial; 1: FP64 continuation of T_RESUME trials; 2 and 3: FP64 redo of
+    // T_REDO trials from their start (3: a second redo pass of the review, own counter)
     unsigned long long* counter =
         &P.g[stage == 0 ? (resume ? G_TASK_INIT_RESUME : G_TASK_INIT)
-                        : (resume == 1 ? G_TASK_REVIEW_RESUME : (resume == 2 ? G_TASK_REVIEW_REDO : G_TASK_REVIEW))];
+                        : (resume == 1 ? G_TASK_REVIEW_RESUME
+                                       : (resume == 2 ? G_TASK_REVIEW_REDO : (resume == 3 ? G_TASK_REVIEW_REDO2 : G_TASK_REVIEW)))];
+    const int mode = resume == 3 ? 2 : resume;
     TpdLane<NC> L;
     bool active = false, exhausted = false;
     for (;;) {
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
         const int64_t task = fetch_task(counter, want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else { active = tpd_begin<NC>(P, stage, static_cast<int>(task / nslot), task % nslot, resume, L); }
+            else { active = tpd_begin<NC>(P, stage, static_cast<int>(task / nslot), task % nslot, mode, L); }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
@@ -593,6 +599,25 @@ __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
             const int nw = nwit < kWitnesses ? nwit : kWitnesses;
             P.u_nwit[u] = nw;
             for (int j = 0; j < nw; ++j) { P.u_wit[j * n + u] = wit[j]; }
+        }
+    }
+}
+
+// Decision-exact mixed precision: the trials an FP32 descent ended clearly negative are
+// searched again in FP64 from their start in every cell that has one, so that the cell's
+// witnesses (the split seeds) are the FP64 schedule's end points bit for bit.
+template <int NC>
+__global__ __launch_bounds__(64) void k_witness_redo(Pipe<NC> P) {
+    const int64_t n = P.n;
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (P.flags[c] & CF_REJECTED) { continue; }
+        const int32_t info = P.refinfo[c];
+        if (!(info & 0x200)) { continue; }
+        const int active = active_count<NC>(info);
+        const int ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info);
+        for (int k = 0; k < ntr; ++k) {
+            if ((P.tst[k * n + c] & 0xFF) == T_NEGATIVE) { P.tst[k * n + c] = T_REDO; }
         }
     }
 }
@@ -1246,8 +1271,9 @@ void launch_fp64(const Pipe<NC>& P, const double* mixc, int max_phases, const Pi
     hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
 }
 
-// Mixed precision: reduced-precision searches, FP64 continuation of the searches they
-// leave ambiguous, FP64 polish of the selected split, FP64 three-phase tail.
+// Fast mixed precision (precision 2): reduced-precision searches, FP64 continuation of
+// the searches they leave ambiguous, FP64 polish of the selected split, FP64 three-phase
+// tail. Decisions can differ from FP64 where the reference's own split search fails.
 template <int NC>
 void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds, hipStream_t s) {
     const int64_t n = P.n;
@@ -1274,20 +1300,56 @@ void launch_mixed(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrid
     hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
 }
 
+// Decision-exact mixed precision (precision 1). FP32 descents screen every stability trial;
+// each one whose outcome bears on a decision is searched in FP64 from its start: the
+// trials FP32 cannot classify beyond rounding reach, the negative trials of every cell
+// that has one (their end points seed the splits), every split attempt, and the whole
+// review of a cell whose review finds instability. The trials FP32 classifies as
+// stationary beyond rounding reach keep that status. Every search whose end point is
+// used is therefore the FP64 schedule's own, and so is every decision built on them.
+template <int NC>
+void launch_mixed_exact(Pipe<NC> P, const double* mixc, int max_phases, const PipeGrids& b, size_t lds,
+                        hipStream_t s) {
+    const int64_t n = P.n;
+    const unsigned cell_blocks = static_cast<unsigned>(std::min<int64_t>((n + 63) / 64, 16384));
+    const unsigned prep_blocks = std::min(cell_blocks, P.nmix > 1 ? std::min(b.prep, 2048u) : b.prep);
+    P.mixed = 1;
+    P.exact = 1;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.g);
+    hipLaunchKernelGGL((k_prep<NC, false>), dim3(prep_blocks), dim3(64), lds, s, P, mixc);
+    hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 0);
+    hipLaunchKernelGGL(k_witness_redo<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 0, 2);
+    hipLaunchKernelGGL(k_init_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_split<NC, false>), dim3(b.split), dim3(64), lds, s, P, mixc, 0);
+    hipLaunchKernelGGL(k_split_select<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_tpd32<NC>, dim3(b.tpd32), dim3(64), lds, s, P, mixc, 1);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 2);
+    hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_tpd<NC, false>), dim3(b.tpd), dim3(64), lds, s, P, mixc, 1, 3);
+    P.mixed = 2;
+    hipLaunchKernelGGL(k_review_reduce<NC>, dim3(cell_blocks), dim3(64), 0, s, P);
+    if (max_phases == 3) { hipLaunchKernelGGL((k_max3_tail<NC, false>), dim3(b.tail), dim3(64), lds, s, P, mixc); }
+    hipLaunchKernelGGL(k_publish<NC>, dim3(std::min(cell_blocks, b.publish)), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_fallback<NC, false>), dim3(b.fb), dim3(64), lds, s, P, mixc);
+}
+
 template <int NC>
 hipError_t launch_pipeline_prec(const Pipe<NC>& P, const double* mixc, int max_phases, int precision,
                                 hipStream_t s) {
     const size_t lds = static_cast<size_t>(P.nmix > 1 ? P.nmix : 1) * sizeof(Mix<NC>);
     static const PipeGrids one = pipe_grids<NC>(sizeof(Mix<NC>));
     const PipeGrids b = P.nmix > 1 ? pipe_grids<NC>(lds) : one;
-    if (precision == 1) { launch_mixed<NC>(P, mixc, max_phases, b, lds, s); }
+    if (precision == 1) { launch_mixed_exact<NC>(P, mixc, max_phases, b, lds, s); }
+    else if (precision == 2) { launch_mixed<NC>(P, mixc, max_phases, b, lds, s); }
     else { launch_fp64<NC>(P, mixc, max_phases, b, lds, s); }
     return hipGetLastError();
 }
 
 // precision 0: FP64 evaluations with the reference's bisection roots (bit-level
-// parity); 1: mixed precision (reduced-precision searches, FP64 roots, decisions and
-// polish, FP64 schedule for every cell the searches cannot settle).
+// parity); 1: decision-exact mixed precision (FP32 screening, every decision-bearing
+// search in FP64); 2: fast mixed precision (reduced-precision searches, FP64 roots,
+// decisions and polish, FP64 schedule for every cell the searches cannot settle).
 //
 // mixc: nmix consecutive Mix<NC> records; mix_of[n] (device) names each cell's record,
 // nullptr when nmix == 1.
@@ -1316,6 +1378,7 @@ hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, 
     P.zin = z;
     P.out = out;
     P.mixed = 0;
+    P.exact = 0;
     return launch_pipeline_prec<NC>(P, mixc, max_phases, precision, s);
 }
 

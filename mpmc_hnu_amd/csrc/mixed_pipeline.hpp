@@ -47,6 +47,7 @@ constexpr uint32_t kTrialEvals32 = 300;
 // A descent slower than this many steps may end at the reference's 512-step limit in FP64
 // (an indeterminate trial); its outcome is left to FP64.
 constexpr int kSlowIt32 = 128;
+constexpr float kSlowTotal = 384.0f;
 constexpr uint32_t kAttemptEvals32 = 400;
 
 // ------------------------------------------------------------------ TPD (FP32)
@@ -64,6 +65,7 @@ struct Tpd32Lane {
     int branch;
     float dir[NC], cand[NC], candlw[NC];  // candidate w and ln w (w may underflow, ln w is kept)
     float slope, scale, alpha;
+    float lrate;        // running mean of ln(stat_k / stat_k-1) over accepted steps
     uint32_t zmask;
     bool has_pt;        // an accepted point exists (else the candidate is the start point)
 };
@@ -132,6 +134,14 @@ __device__ __forceinline__ float log_gap32(const Tpd32Lane<NC>& L, const float* 
 template <int NC>
 __device__ __forceinline__ int tpd32_classify(const Pipe<NC>& P, int stage, const Tpd32Lane<NC>& L) {
     if (L.it >= kSlowIt32) { return T_AMBIG; }
+    // Decision-exact schedule: the reference stops a descent only at stationarity 1e-8 or
+    // after 512 steps. A descent whose recent contraction rate, extrapolated from the FP32
+    // end point down to 1e-8, would need more than kSlowTotal steps in all may end at the
+    // iteration limit in FP64: FP64 decides it.
+    if (P.exact && L.it > 0) {
+        const float need = __logf(fmaxf(L.stat, 1e-8f) * 1e8f);
+        if (!(L.lrate < 0.0f) || static_cast<float>(L.it) - need / L.lrate > kSlowTotal) { return T_AMBIG; }
+    }
     const float margin = 4.0f * L.guard + 1e-6f;
     if (L.value < -(static_cast<float>(L.tol) + margin)) { return T_NEGATIVE; }
     if (L.value > margin) { return T_STATIONARY; }
@@ -223,6 +233,7 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
     L.has_pt = false;
     L.evals = 0;
     L.it = 0;
+    L.lrate = 0.0f;
     L.mode = TM_START;
     if (!fin) {
         st_arr[k * n + slot] = T_AMBIG;
@@ -232,11 +243,19 @@ __device__ bool tpd32_begin(const Pipe<NC>& P, const Mix<NC>* smix, int stage, i
     return true;
 }
 
-// An ambiguous trial is handed to the FP64 continuation (T_RESUME) from its current point.
+// An ambiguous trial is handed to the FP64 continuation (T_RESUME) from its current point,
+// or, in the decision-exact schedule, to an FP64 search from its start (T_REDO).
 template <int NC>
 __device__ void tpd32_finish(const Pipe<NC>& P, int stage, const Tpd32Lane<NC>& L, int status) {
     const int64_t n = P.n;
     const int64_t idx = L.k * n + L.slot;
+    if (P.exact && status == T_AMBIG) {
+        (stage == 0 ? P.tst : P.fst)[idx] = T_REDO;
+        (stage == 0 ? P.tev : P.fev)[idx] = L.evals;
+        atomicAdd(&P.cnt[0 * n + L.cell], L.evals);
+        atomicAdd(&P.cnt[1 * n + L.cell], L.evals);
+        return;
+    }
     const bool resume = status == T_AMBIG;
     (stage == 0 ? P.tst : P.fst)[idx] = resume ? (T_RESUME | (L.it << 16)) : (status | (L.branch << 8));
     (stage == 0 ? P.tev : P.fev)[idx] = L.evals;
@@ -358,6 +377,8 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd32(Pipe<NC> P, const
                 }
             }
             if (ok) {
+                const float lr = nst > 0.0f && L.stat > 0.0f ? __logf(nst / L.stat) : 0.0f;
+                L.lrate = L.it == 0 ? lr : 0.7f * L.lrate + 0.3f * lr;
 #pragma unroll
                 for (int i = 0; i < NC; ++i) { L.w[i] = L.cand[i]; L.lw[i] = L.candlw[i]; L.res[i] = nres[i]; }
                 L.value = nval; L.guard = ngd; L.stat = nst; L.branch = br;

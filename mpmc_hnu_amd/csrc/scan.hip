@@ -102,17 +102,27 @@ __global__ void k_scan_source_2p(int nc, int64_t n, const double* __restrict__ q
 }
 
 // Decision (:236-292) on the flash record. A cell already marked (2, 3) keeps its code.
-// The flash rejecting the input is the reference's invalid_argument (code 3); no accepted
-// phase set is indeterminate; an accepted set of the source's size is no transition;
-// otherwise a proposal whose target phase molar densities are p / (Z R T) of the
+// The flash rejecting the feed of a validated source is the std::domain_error of the
+// reference's stability and mixing checks, which the scanner catches as indeterminate
+// (:292-299); no accepted phase set is indeterminate; an accepted set of the source's size
+// is no transition. Otherwise the target phase molar densities are p / (Z R T) of the
 // accepted phases, the density resolver of the reference's scanner test
 // (post_snes_pt_flash_phase_transition_scanner_test.cpp:26-57): a density that is not
-// finite and positive leaves the cell indeterminate.
+// finite and positive leaves the cell indeterminate. With the densities resolved, the
+// candidate must carry the backend's transition evidence
+// (make_phase_set_transition_candidate_from_flash, phase_set_transition_flash_adapter.hpp:
+// 171-205, over project_pr76_pt_max3_transition_report, pr76_pt_flash_backend.hpp:94-174):
+// a higher target always does (the base split or the three-phase solve challenged the
+// source and resolved the target); a lower target only through a fresh phase-disappearance
+// solve, which the PR76 backend reports for 3 -> 2 alone, when the three-phase tail
+// resolved a two-phase neighbour (max3 two-phase after an unstable base split). Any other
+// lower target is the adapter's std::invalid_argument: an error.
 // zroot (density mode 1, else null): Z of every root of the cubic at the cell's feed
 // (mpmc_phase_batch); a single accepted phase, whose record publishes no Z (as the
 // reference's single-phase projection, pt_vle_phase_set.hpp:68-73), then takes the Z of
-// its provider root (branch word of phase 0).
-// stop[0] / stop[1]: lowest index of an indeterminate / error cell; stop[2]: proposals.
+// its provider root (branch word of phase 0), and meets the evidence rule as any lower
+// target does.
+// stop[0] / stop[1]: lowest index of an indeterminate / error cell.
 __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const double* __restrict__ t,
                               const int32_t* __restrict__ count, int32_t count_all,
                               const int32_t* __restrict__ status, const int32_t* __restrict__ phase_count,
@@ -130,7 +140,7 @@ __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const dou
         if (k == MPMC_SCAN_SAME) {
             const int32_t st = status[c];
             if (st & MPMC_FLAG_REJECTED) {
-                k = MPMC_SCAN_ERROR;
+                k = MPMC_SCAN_INDETERMINATE;
             } else if (MPMC_STATUS_SET(st) != MPMC_SET_ACCEPTED) {
                 k = MPMC_SCAN_INDETERMINATE;
             } else {
@@ -149,6 +159,9 @@ __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const dou
                         if (!finite(zf) || !(zf > 0) || !finite(r) || !(r > 0)) { k = MPMC_SCAN_INDETERMINATE; }
                         rho[ph] = r;
                     }
+                    const bool neighbour = src == 3 && target == 2 && MPMC_STATUS_MAX3(st) == MPMC_MAX3_TWO &&
+                                           MPMC_STATUS_SPLIT(st) == MPMC_SPLIT_UNSTABLE;
+                    if (k == MPMC_SCAN_PROPOSAL && target < src && !neighbour) { k = MPMC_SCAN_ERROR; }
                 }
             }
         }
@@ -162,10 +175,22 @@ __global__ void k_scan_decide(int64_t n, const double* __restrict__ p, const dou
         const int lane = static_cast<int>(__lane_id());
         const unsigned long long indet = __ballot(k == MPMC_SCAN_INDETERMINATE);
         const unsigned long long err = __ballot(k == MPMC_SCAN_ERROR);
-        const unsigned long long prop = __ballot(k == MPMC_SCAN_PROPOSAL);
         if (indet && lane == __builtin_ctzll(indet)) { atomicMin(&stop[0], static_cast<unsigned long long>(c)); }
         if (err && lane == __builtin_ctzll(err)) { atomicMin(&stop[1], static_cast<unsigned long long>(c)); }
-        if (prop && lane == __builtin_ctzll(prop)) { atomicAdd(&stop[2], static_cast<unsigned long long>(__popcll(prop))); }
+    }
+}
+
+// Proposals the reference's cell loop has collected when it stops: those before the first
+// indeterminate or error cell (:366-407), counted one atomic per wave into stop[2].
+__global__ void k_scan_count(int64_t n, const int32_t* __restrict__ code, unsigned long long* __restrict__ stop) {
+    const unsigned long long first = stop[0] < stop[1] ? stop[0] : stop[1];
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const bool prop = code[c] == MPMC_SCAN_PROPOSAL && static_cast<unsigned long long>(c) < first;
+        const unsigned long long b = __ballot(prop);
+        if (b && static_cast<int>(__lane_id()) == __builtin_ctzll(b)) {
+            atomicAdd(&stop[2], static_cast<unsigned long long>(__popcll(b)));
+        }
     }
 }
 
@@ -226,6 +251,7 @@ int scan_finish(mpmc_mixture_t mixture, int64_t n, const double* p, const double
     hipLaunchKernelGGL(k_scan_decide, dim3(grid_for(n)), dim3(256), 0, s, n, p, t, count, count_all, out->status,
                        out->phase_count, out->zfactor, out->branch, zroot, scan->code, scan->target_count,
                        scan->target_density, stop);
+    hipLaunchKernelGGL(k_scan_count, dim3(grid_for(n)), dim3(256), 0, s, n, scan->code, stop);
     e = hipGetLastError();
     if (e == hipSuccess) { e = hipMemcpyAsync(host, stop, sizeof(host), hipMemcpyDeviceToHost, s); }
     const hipError_t ef = hipFreeAsync(stop, s);
@@ -237,12 +263,13 @@ int scan_finish(mpmc_mixture_t mixture, int64_t n, const double* p, const double
         if (e == hipSuccess) { e = ep; }
     }
     if (e != hipSuccess) { return scan_fail(MPMC_E_DEVICE, std::string("transition_scan: ") + hipGetErrorString(e)); }
-    // the reference returns at the first indeterminate cell (status indeterminate) or at the
-    // first cell that raises an error (the error code, status untouched)
+    // the reference returns at the first indeterminate cell (status indeterminate, the
+    // collected proposals cleared) or at the first cell that raises an error (the error
+    // code, status untouched, the proposals before that cell kept)
     const unsigned long long first = host[0] < host[1] ? host[0] : host[1];
     summary->first_stop = first == ~0ull ? -1 : static_cast<int64_t>(first);
-    summary->proposals = static_cast<int64_t>(host[2]);
     summary->status = host[0] < host[1] ? MPMC_SCAN_STATUS_INDETERMINATE : MPMC_SCAN_COMPLETE;
+    summary->proposals = summary->status == MPMC_SCAN_STATUS_INDETERMINATE ? 0 : static_cast<int64_t>(host[2]);
     if (host[1] < host[0]) {
         return scan_fail(MPMC_E_INVALID_ARGUMENT,
                          "transition_scan: the first stopping cell has a malformed source or a rejected feed");

@@ -58,6 +58,7 @@ struct Job {
     int64_t cells_done = 0;
     bool small = false;
     bool done = false;
+    std::string error;       // first failure of a chunk holding cells of this job
     double latency_s = 0;
 };
 
@@ -113,7 +114,6 @@ struct mpmc_server {
     int64_t next_id = 1;
     bool stop = false;
     std::thread worker;
-    std::string error;
     size_t device_bytes = 0;
     std::atomic<int64_t> cells_served{0};
 
@@ -250,17 +250,19 @@ struct mpmc_server {
         return mix;
     }
 
-    void complete(Slot& s, bool ok) { // caller holds mu
+    // A chunk has ended (caller holds mu). On failure its jobs carry the message and their
+    // wait/poll report MPMC_E_DEVICE; other jobs are unaffected.
+    void complete(Slot& s, const std::string& failure) {
         for (const auto& g : s.segs) {
             auto& j = g.job;
             j->cells_done += g.len;
-            cells_served += g.len;
+            if (failure.empty()) { cells_served += g.len; }
+            else if (j->error.empty()) { j->error = failure; }
             if (j->cells_done >= j->n && !j->done) {
                 j->done = true;
                 j->latency_s = std::chrono::duration<double>(Clock::now() - j->submitted).count();
             }
         }
-        (void)ok;
         s.segs.clear();
         s.len = 0;
         cv_done.notify_all();
@@ -275,14 +277,15 @@ struct mpmc_server {
                 if (!s.busy()) { continue; }
                 const hipError_t q = hipEventQuery(s.done);
                 if (q == hipErrorNotReady) { continue; }
+                std::string failure;
                 if (q != hipSuccess) {
-                    error = std::string("server: device failure: ") + hipGetErrorString(q);
+                    failure = std::string("server: device failure: ") + hipGetErrorString(q);
                 } else {
                     lk.unlock();
                     scatter(s);
                     lk.lock();
                 }
-                complete(s, q == hipSuccess);
+                complete(s, failure);
                 progressed = true;
             }
             for (auto& s : slots) {
@@ -292,11 +295,13 @@ struct mpmc_server {
                 if (!mix) { break; }
                 lk.unlock();
                 const bool ok = launch(s, mix, nc);
-                lk.lock();
                 if (!ok) {
-                    error = std::string("server: launch failed: ") + fail_msg;
-                    complete(s, false);
+                    // copies already queued on the stream read and write this slot's pinned
+                    // staging: let them end before the slot is refilled
+                    (void)hipStreamSynchronize(stream);
                 }
+                lk.lock();
+                if (!ok) { complete(s, std::string("server: launch failed: ") + fail_msg); }
                 progressed = true;
             }
             bool busy = false;
@@ -346,7 +351,7 @@ int mpmc_server_create(const mpmc_server_options* options, mpmc_server_t* out) {
     mpmc_server_default_options(&o);
     if (options) { o = *options; }
     if (o.slots < 1 || o.slots > 32 || o.chunk_cells < 1 || o.chunk_cells > (int64_t{1} << 24) ||
-        o.small_job_cells < 0 || (o.max_phases != 2 && o.max_phases != 3) || (o.precision != 0 && o.precision != 1) ||
+        o.small_job_cells < 0 || (o.max_phases != 2 && o.max_phases != 3) || (o.precision < 0 || o.precision > 2) ||
         !(o.fugacity_tolerance > 0)) {
         return server_fail(MPMC_E_INVALID_ARGUMENT, "server_create: invalid options");
     }
@@ -468,7 +473,7 @@ int mpmc_server_wait(mpmc_server_t s, int64_t job_id, double* latency_s) {
     s->cv_done.wait(lk, [&] { return j->done; });
     if (latency_s) { *latency_s = j->latency_s; }
     s->jobs.erase(job_id);
-    if (!s->error.empty()) { return server_fail(MPMC_E_DEVICE, s->error); }
+    if (!j->error.empty()) { return server_fail(MPMC_E_DEVICE, j->error); }
     return MPMC_OK;
 }
 
@@ -479,7 +484,7 @@ int mpmc_server_poll(mpmc_server_t s, int64_t job_id, int32_t* done, double* lat
     if (it == s->jobs.end()) { return server_fail(MPMC_E_INVALID_ARGUMENT, "server_poll: unknown job"); }
     *done = it->second->done ? 1 : 0;
     if (it->second->done && latency_s) { *latency_s = it->second->latency_s; }
-    if (!s->error.empty()) { return server_fail(MPMC_E_DEVICE, s->error); }
+    if (it->second->done && !it->second->error.empty()) { return server_fail(MPMC_E_DEVICE, it->second->error); }
     return MPMC_OK;
 }
 

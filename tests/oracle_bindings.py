@@ -29,6 +29,19 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_P)
 
 
+def host_threads(threads: int = 0) -> int:
+    """Worker threads of a checker call: the given count, else the CPUs this process may
+    run on, at most 16 (a GPU box gives one GPU's jobs a 16-CPU share whatever the
+    machine reports)."""
+    if threads > 0:
+        return host_threads(threads)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
 def _load(path: Path, prefix: str):
     if not path.exists():
         return None
@@ -92,7 +105,7 @@ def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which
         fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_warm")
         rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), int(hints.shape[0]),
                 _ptr(hints), int(final_hints.shape[0]), _ptr(final_hints), float(fugacity_tolerance), int(max_phases),
-                int(threads), _ptr(out["status"]), _ptr(out["phase_count"]), _ptr(out["beta"]),
+                host_threads(threads), _ptr(out["status"]), _ptr(out["phase_count"]), _ptr(out["beta"]),
                 _ptr(out["zfactor"]), _ptr(out["branch"]), _ptr(out["x"]), _ptr(out["ln_phi"]),
                 _ptr(out["counters"]))
     elif hints is not None:
@@ -100,13 +113,13 @@ def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which
         assert hints.ndim == 3 and hints.shape[1:] == (nc, n) and hints.shape[0] <= 2
         fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_hinted")
         rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), int(hints.shape[0]),
-                _ptr(hints), float(fugacity_tolerance), int(max_phases), int(threads), _ptr(out["status"]),
+                _ptr(hints), float(fugacity_tolerance), int(max_phases), host_threads(threads), _ptr(out["status"]),
                 _ptr(out["phase_count"]), _ptr(out["beta"]), _ptr(out["zfactor"]), _ptr(out["branch"]),
                 _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]))
     else:
         fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch")
         rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z),
-                float(fugacity_tolerance), int(max_phases), int(threads), _ptr(out["status"]),
+                float(fugacity_tolerance), int(max_phases), host_threads(threads), _ptr(out["status"]),
                 _ptr(out["phase_count"]), _ptr(out["beta"]), _ptr(out["zfactor"]), _ptr(out["branch"]),
                 _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]))
     if rc != 0:
@@ -180,7 +193,7 @@ def closure(mix, p, t, z, fugacity_tolerance=1e-11, min_fraction=1.4901161193847
     fn = getattr(L, prefix + "_closure_batch")
     fn.argtypes = _CLOSURE_ARGS
     rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), float(fugacity_tolerance),
-            float(min_fraction), float(min_rcond), int(threads), _ptr(out["status"]), _ptr(out["primal"]),
+            float(min_fraction), float(min_rcond), host_threads(threads), _ptr(out["status"]), _ptr(out["primal"]),
             _ptr(out["jac"]), _ptr(out["log_k_jac"]), _ptr(out["diag"]))
     if rc != 0:
         raise RuntimeError(f"{prefix}_closure_batch failed rc={rc}")
@@ -211,7 +224,7 @@ def assembly(mix, model, q, branch, grid, prev, dt, threads=0, which="cpuref"):
                    ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_int, _P, _P, _P, _P]
     rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), _ptr(model), n, _ptr(q), _ptr(br), _ptr(g["bulk"]),
             _ptr(g["poro"]), _ptr(prev), float(dt), nf, _ptr(g["owner"]), _ptr(g["neighbour"]), _ptr(g["trans"]),
-            _ptr(g["gdot"]), _ptr(g["gth"]), int(threads), _ptr(out["cell_status"]), _ptr(out["residual"]),
+            _ptr(g["gdot"]), _ptr(g["gth"]), host_threads(threads), _ptr(out["cell_status"]), _ptr(out["residual"]),
             _ptr(out["diag"]), _ptr(out["offdiag"]))
     out["rc"] = rc
     return out
@@ -228,7 +241,7 @@ def inventory(mix, model, q, branch, poro, threads=0):
     fn.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P]
     rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), _ptr(np.ascontiguousarray(model, dtype=np.float64)), n,
             _ptr(q), _ptr(np.ascontiguousarray(branch, dtype=np.int32)), _ptr(np.ascontiguousarray(poro)),
-            int(threads), _ptr(acc))
+            host_threads(threads), _ptr(acc))
     if rc != 0:
         raise RuntimeError(f"cpuref_accumulation rc={rc}")
     return acc

@@ -74,3 +74,47 @@ def fugacity_residual(res: dict) -> np.ndarray:
         d = np.where(have[None, :], np.abs(mu - mu0), 0.0).max(axis=0)
         out = np.maximum(out, d)
     return out
+
+
+def _value_err(a: dict, b: dict, cells: np.ndarray) -> np.ndarray:
+    """Largest relative difference of beta, x and Z of a and b at ``cells`` (same decisions)."""
+    out = np.zeros(cells.size)
+    npc = np.minimum(a["phase_count"][cells], b["phase_count"][cells])
+    for ph in range(3):
+        have = npc > ph
+        for key in ("beta", "zfactor"):
+            va, vb = a[key][ph][cells], b[key][ph][cells]
+            e = np.abs(va - vb) / np.maximum(np.abs(vb), 1e-300)
+            out = np.maximum(out, np.where(have & np.isfinite(e), e, 0.0))
+        xa, xb = a["x"][ph][:, cells], b["x"][ph][:, cells]
+        e = (np.abs(xa - xb) / np.maximum(np.abs(xb), 1e-300)).max(axis=0)
+        out = np.maximum(out, np.where(have, e, 0.0))
+    return out
+
+
+def within_stopping_error(g: dict, o: dict, cells: np.ndarray, tight) -> tuple:
+    """Cells whose values differ beyond 1e-9 between the device (g) and the oracle (o), both
+    stopped at the same fugacity tolerance tol, judged against ``tight(cells, factor)``: the
+    oracle at tolerance tol * factor on those cells, with factor 1e-2, or 1e-1 where the
+    tighter solve reaches another decision. The device value is accepted when it is within
+    1e-9 of the tight solution, or no farther from it than twice the oracle's own stopping
+    error. A phase fraction of order 1e-4 is determined only to about 1e-12 by a fugacity
+    residual of 1e-11, so 1e-9 relative can be below the reference's own determinacy there.
+    Returns (bad cells, per-cell (device error, oracle error))."""
+    if cells.size == 0:
+        return cells, []
+    sub = {k: (v[..., cells] if isinstance(v, np.ndarray) else v) for k, v in g.items()}
+    subo = {k: (v[..., cells] if isinstance(v, np.ndarray) else v) for k, v in o.items()}
+    idx = np.arange(cells.size)
+    ok = np.zeros(cells.size, bool)
+    errs = [(np.inf, np.inf)] * cells.size
+    for factor in (1e-2, 1e-1):
+        t = tight(cells, factor)
+        same = ((t["status"] & 0xFFF) == (subo["status"] & 0xFFF)) & (t["phase_count"] == subo["phase_count"])
+        eg = _value_err(sub, t, idx)
+        eo = _value_err(subo, t, idx)
+        now = ~ok & same
+        ok |= now & ((eg <= 1e-9) | (eg <= 2.0 * eo))
+        for i in np.nonzero(now)[0]:
+            errs[i] = (float(eg[i]), float(eo[i]))
+    return cells[~ok], errs

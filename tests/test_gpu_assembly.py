@@ -45,7 +45,10 @@ def _close(a, b, rel=REL):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nc,dims,low_p", [(2, (4, 3, 2), False), (3, (3, 3, 3), False), (6, (5, 4, 3), False),
-                                           (12, (3, 2, 2), False), (3, (4, 4, 3), True)])
+                                           (12, (3, 2, 2), False), (3, (4, 4, 3), True),
+                                           # 4,800 cells = 75 record blocks: the strided block order and
+                                           # the XCD renumbering of several blocks per XCD are exercised
+                                           (6, (24, 20, 10), False)])
 def test_gpu_assembly_matches_oracle(gpu, nc, dims, low_p):
     """low_p: 0.5-3 MPa, where some vapour-like cells take the largest of three roots."""
     from mpmc_hnu_amd import assembly as A

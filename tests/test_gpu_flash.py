@@ -8,10 +8,17 @@ the published phases within 1e-10.
 The device libm (log, exp, log1p, expm1) and glibc can differ in the last ulp, so the
 successive-substitution iterates of the two sides are not bit-identical; both stop
 inside the same fugacity tolerance (1e-11), which bounds the difference of converged
-fractions by tolerance x conditioning. Hence: decision mismatches are reported with
-their cell index and allowed up to ``MAX_FLIP`` of the cells, values beyond 1e-9
-relative up to ``MAX_LOOSE`` of the cells (never beyond 1e-7), and evaluation counters
-(which can differ by an iteration) are compared in total.
+fractions by tolerance x conditioning. Decisions must nevertheless agree on every cell
+(a mismatch is printed with its cell index), every value within 1e-9, and evaluation
+counters (which can differ by an iteration) agree in total within 1%.
+
+Mixed precision: precision 1 (decision-exact) runs every decision-bearing search in FP64
+from its start, so its statuses, phase counts and labels must equal precision 0's on every
+cell, except cells where precision 0's own search ends indeterminate because an FP64 descent
+stalls where the FP32 screen reaches a stationary point (a descent passing a saddle of the
+tangent-plane distance or converging too slowly for the 512-step limit); such cells are
+listed and counted. Precision 2 (fast) is held to values within 1e-7 and a decision
+share, since its FP32 split attempts converge where the reference's line search fails.
 """
 import numpy as np
 import pytest
@@ -20,8 +27,8 @@ import oracle_bindings as ob
 import parity
 from mpmc_hnu_amd import workloads as W
 
-MAX_FLIP = 2e-3
-MAX_LOOSE = 1e-3
+# share of cells whose decision may differ from FP64 in the fast mixed schedule (precision 2)
+FAST_MAX_FLIP = 2e-3
 
 
 def _run_gpu(mix, p, t, z, tol, maxph):
@@ -35,6 +42,19 @@ def _run_gpu(mix, p, t, z, tol, maxph):
     return res
 
 
+def _values_within_stopping_error(name, g, o, bad, mix, p, t, z, tol, maxph):
+    """Every value within 1e-9 relative of the oracle's, or (cells listed) within the
+    oracle's own stopping error at this fugacity tolerance (parity.within_stopping_error)."""
+    cells = np.nonzero(bad)[0]
+    if cells.size == 0:
+        return
+    bad_cells, errs = parity.within_stopping_error(
+        g, o, cells, lambda c, f: ob.flash(mix, p[c], t[c], np.ascontiguousarray(z[:, c]), tol * f, maxph))
+    print(f"  {name}: values beyond 1e-9 at cells {cells[:16].tolist()}, (device, oracle) error against the "
+          f"oracle at a smaller tolerance: {[(f'{a:.1e}', f'{b:.1e}') for a, b in errs[:16]]}", flush=True)
+    assert bad_cells.size == 0, bad_cells.tolist()
+
+
 def _check(name, mix, p, t, z, tol, maxph):
     print(f"[case] {name} nc={z.shape[0]} cells={p.size} max_phases={maxph}", flush=True)
     g = _run_gpu(mix, p, t, z, tol, maxph)
@@ -46,10 +66,8 @@ def _check(name, mix, p, t, z, tol, maxph):
         idx = np.nonzero(bad)[0][:8]
         print("  decision mismatches at cells", idx.tolist(), "gpu", g["status"][idx].tolist(), "oracle",
               o["status"][idx].tolist(), flush=True)
-    assert nbad_dec <= MAX_FLIP * p.size, rep
-    assert rep["value_mismatch"] <= MAX_LOOSE * p.size, rep
-    loose, _ = parity.compare(g, o, rel=1e-7, abs_lnphi=1e-7)
-    assert loose["value_mismatch"] == 0, loose
+    assert nbad_dec == 0, rep
+    _values_within_stopping_error(name, g, o, bad, mix, p, t, z, tol, maxph)
     res = parity.fugacity_residual(g)
     acc = (g["status"] & 0xF) == 0
     assert np.all(res[acc] <= 1e-10), float(res[acc].max())
@@ -71,7 +89,7 @@ def test_c1_vle_4096(gpu):
 @pytest.mark.gpu
 def test_c2_max3_prefix(gpu):
     mix = W.mixture_linspaced(6)
-    p, t, z = W.cells_c2(ncell=16384)
+    p, t, z = W.cells_c2(ncell=131072)
     _check("C2", mix, p, t, z, 1e-11, 3)
 
 
@@ -155,7 +173,7 @@ def _record_bits(out):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_record_is_independent_of_buffer_history(gpu, precision):
     """A cell's record is a function of its inputs alone: the same batch on zeroed
     workspace and output buffers, on the buffers left by another batch, and again on
@@ -199,46 +217,95 @@ def test_record_is_independent_of_buffer_history(gpu, precision):
     assert int(three.sum()) > 100
 
 
-def _mixed_vs_fp64(mix, p, t, z, maxph=3):
+def _mixed_vs_fp64(mix, p, t, z, maxph=3, precision=1):
     from mpmc_hnu_amd import flash
 
     m = flash.Mixture.from_dict(mix)
-    a = flash.to_host(flash.flash_batch(m, p, t, z, max_phases=maxph, precision=1))
+    a = flash.to_host(flash.flash_batch(m, p, t, z, max_phases=maxph, precision=precision))
     b = flash.to_host(flash.flash_batch(m, p, t, z, max_phases=maxph, precision=0))
     m.close()
     return a, b
 
 
+def _exact_bar(name, a, b, max_stalled=0):
+    """Precision 1 against precision 0: identical decisions, except cells that precision 0
+    leaves indeterminate (status set 2) because one of its FP64 descents stalls; those are
+    listed and at most ``max_stalled``. Cells with identical decisions have identical
+    values whenever the same searches ran, and within 1e-9 otherwise."""
+    rep, bad = parity.compare(a, b, rel=1e-9, abs_lnphi=1e-9)
+    dec = ((a["status"] & 0xFFF) != (b["status"] & 0xFFF)) | (a["phase_count"] != b["phase_count"])
+    idx = np.nonzero(dec)[0]
+    print(f"{name} exact mixed vs fp64", rep, "decision cells", idx[:16].tolist(),
+          [hex(v) for v in b["status"][idx[:16]]], [hex(v) for v in a["status"][idx[:16]]], flush=True)
+    assert np.all((b["status"][idx] & 0xF) == 2), "a decision differs on a cell FP64 decides"
+    assert idx.size <= max_stalled, idx.size
+    assert rep["value_mismatch"] == 0, rep
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("nc", [3, 6, 9, 12])
 def test_mixed_precision_matches_fp64(gpu, nc):
-    """Mixed precision: same phase counts and labels as FP64 (reported per cell, at most
-    MAX_FLIP), values within 1e-8 relative of the FP64 oracle, fugacity equality 1e-10."""
+    """Decision-exact mixed precision (precision 1): the same statuses, phase counts and
+    labels as FP64 on every cell, values within 1e-9 of FP64 and within 1e-8 of the FP64
+    oracle, fugacity equality 1e-10."""
     mix = W.mixture_linspaced(nc)
-    p, t, z = W.cells_uniform(nc, 4096, seed=300 + nc)
+    p, t, z = W.cells_uniform(nc, 16384, seed=300 + nc)
     a, b = _mixed_vs_fp64(mix, p, t, z)
+    _exact_bar(f"U{nc}", a, b)
     o = ob.flash(mix, p, t, z, 1e-11, 3)
     rep, bad = parity.compare(a, o, rel=1e-8, abs_lnphi=1e-8)
-    print(f"mixed nc={nc}", rep, flush=True)
-    assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
-    assert rep["value_mismatch"] <= MAX_LOOSE * p.size, rep
+    print(f"exact mixed nc={nc} vs oracle", rep, flush=True)
+    assert rep["decision_cells"] == 0 and rep["value_mismatch"] == 0, rep
     res = parity.fugacity_residual(a)
     acc = (a["status"] & 0xF) == 0
     assert np.all(res[acc] <= 1e-10), float(res[acc].max())
 
 
 @pytest.mark.gpu
+def test_mixed_precision_c2_grid(gpu):
+    """Decision-exact mixed precision on the first 131,072 cells of the nc=6 grid workload:
+    no decision differs from FP64 or from the FP64 oracle."""
+    mix = W.mixture_linspaced(6)
+    p, t, z = W.cells_c2(ncell=131072)
+    a, b = _mixed_vs_fp64(mix, p, t, z)
+    _exact_bar("C2", a, b)
+    o = ob.flash(mix, p, t, z, 1e-11, 3)
+    rep, bad = parity.compare(a, o, rel=1e-8, abs_lnphi=1e-8)
+    print("exact mixed C2 vs oracle", rep, flush=True)
+    assert rep["decision_cells"] == 0 and rep["value_mismatch"] == 0, rep
+
+
+@pytest.mark.gpu
 def test_mixed_precision_three_phase_sweep(gpu):
+    """The sour-gas sweep crosses the three-phase region, where FP64 descents pass saddles of
+    the tangent-plane distance: precision 1 may differ only on cells FP64 leaves
+    indeterminate (at most 1 of 2,048 here, listed)."""
     mix, p, t, z = W.sour_gas_sweep(2048)
     a, b = _mixed_vs_fp64(mix, p, t, z)
-    rep, bad = parity.compare(a, b, rel=1e-8, abs_lnphi=1e-8)
-    print("mixed sour", rep, flush=True)
-    assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
+    _exact_bar("sour", a, b, max_stalled=2)
     assert np.sum((a["status"] >> 4 & 0xF) == 2) > 100
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nc", [3, 6])
+def test_fast_mixed_precision(gpu, nc):
+    """Fast mixed precision (precision 2): values within 1e-7 of FP64 on cells with the same
+    decision; decision differences (cells where the reference's split search ends on a
+    line-search failure that FP32 attempts do not meet) are printed and bounded."""
+    mix = W.mixture_linspaced(nc)
+    p, t, z = W.cells_uniform(nc, 16384, seed=300 + nc)
+    a, b = _mixed_vs_fp64(mix, p, t, z, precision=2)
+    rep, bad = parity.compare(a, b, rel=1e-7, abs_lnphi=1e-7)
+    print(f"fast mixed nc={nc}", rep, flush=True)
+    assert rep["decision_cells"] <= FAST_MAX_FLIP * p.size, rep
+    assert rep["value_mismatch"] == 0, rep
+    res = parity.fugacity_residual(a)
+    acc = (a["status"] & 0xF) == 0
+    assert np.all(res[acc] <= 1e-10), float(res[acc].max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_warm_start_hints(gpu, precision):
     """mpmc_flash_batch_hinted: the phase compositions of a flash at 3% lower pressure as
     warm-start hints. FP64 matches the oracle's hinted flash at the usual bar (hints add
@@ -257,11 +324,11 @@ def test_warm_start_hints(gpu, precision):
     finally:
         m.close()
     o = ob.flash(mix, p, t, z, 1e-11, 3, hints=hints)
-    rel = 1e-9 if precision == 0 else 1e-8
+    rel = 1e-7 if precision == 2 else 1e-9
     rep, bad = parity.compare(g, o, rel=rel, abs_lnphi=rel)
     print("hinted", rep, flush=True)
-    assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
-    assert rep["value_mismatch"] <= MAX_LOOSE * p.size, rep
+    assert rep["decision_cells"] <= (FAST_MAX_FLIP * p.size if precision == 2 else 0), rep
+    assert rep["value_mismatch"] == 0, rep
     res = parity.fugacity_residual(g)
     acc = (g["status"] & 0xF) == 0
     assert np.all(res[acc] <= 1e-10), float(res[acc].max())
@@ -273,7 +340,7 @@ def test_warm_start_hints(gpu, precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,max_phases", [(0, 3), (0, 2), (1, 3)])
+@pytest.mark.parametrize("precision,max_phases", [(0, 3), (0, 2), (1, 3), (2, 3)])
 def test_final_review_hints(gpu, precision, max_phases):
     """mpmc_flash_batch_warm: initial and final-review hints (the phase compositions of a
     flash at 3% lower pressure), a final hint with the wrong support rejecting its cell.
@@ -300,11 +367,11 @@ def test_final_review_hints(gpu, precision, max_phases):
         m.close()
     o = ob.flash(mix, p, t, z, 1e-11, max_phases, hints=ih, final_hints=fh)
     assert g["status"][bad_cell] & 0x2000 and o["status"][bad_cell] & 0x2000
-    rel = 1e-9 if precision == 0 else 1e-8
+    rel = 1e-7 if precision == 2 else 1e-9
     rep, bad = parity.compare(g, o, rel=rel, abs_lnphi=rel)
     print("warm", rep, flush=True)
-    assert rep["decision_cells"] <= MAX_FLIP * p.size, rep
-    assert rep["value_mismatch"] <= MAX_LOOSE * p.size, rep
+    assert rep["decision_cells"] <= (FAST_MAX_FLIP * p.size if precision == 2 else 0), rep
+    assert rep["value_mismatch"] == 0, rep
     res = parity.fugacity_residual(g)
     acc = (g["status"] & 0xF) == 0
     assert np.all(res[acc] <= 1e-10), float(res[acc].max())

@@ -205,3 +205,62 @@ def test_uneven_batch_plan_equals_one_mixture_at_a_time(gpu):
     assert r["near_boundary_cells"] > 0.25 * r["cells"]
     assert r["identical_records"]
     assert set(r["phase_counts"]) >= {1, 2}
+
+
+@pytest.mark.gpu
+def test_uneven_batch_against_the_oracle(gpu):
+    """The uneven batch at 1% of its cells (256 mixtures of nc 2..12, 30% of the cells within
+    2% in P of a phase boundary): every cell of every mixture, near-boundary cells included,
+    flashed by one FP64 plan has the oracle's decisions and values (the flash's bar: 1e-9, or
+    within the oracle's own stopping error at tolerance 1e-11), and
+    the decision-exact mixed plan has the FP64 plan's decisions."""
+    import sys
+    from pathlib import Path
+
+    import torch
+
+    import oracle_bindings as ob
+    import parity
+    from mpmc_hnu_amd import flash
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import bench
+
+    layout, mixes, cells, ncell, near = bench.uneven_workload(scale=0.01)
+    print(f"[case] uneven batch vs oracle: {len(layout)} mixtures, {ncell} cells, {near} near a boundary",
+          flush=True)
+    try:
+        dev = torch.device("cuda", 0)
+        segments = [(mixes[k], torch.as_tensor(p, device=dev), torch.as_tensor(t, device=dev),
+                     torch.as_tensor(z, device=dev)) for k, (p, t, z) in enumerate(cells)]
+        plan = flash.Plan(segments)
+        exact = flash.Plan(segments, precision=1)
+        try:
+            torch.cuda.synchronize()
+            flips = stalled = 0
+            for k, (m, (p, t, z)) in enumerate(zip(layout, cells)):
+                g = _host(plan.outputs[k])
+                o = ob.flash(m["mixture"], p, t, z, 1e-11, 3)
+                rep, bad = parity.compare(g, o)
+                mix = m["mixture"]
+                loose, errs = parity.within_stopping_error(
+                    g, o, np.nonzero(bad)[0],
+                    lambda c, f: ob.flash(mix, p[c], t[c], np.ascontiguousarray(z[:, c]), 1e-11 * f, 3))
+                if rep["decision_cells"] or rep["value_mismatch"]:
+                    print(f"  mixture {k} nc={m['nc']} n={p.size}", rep, np.nonzero(bad)[0][:8].tolist(), errs[:8],
+                          flush=True)
+                flips += rep["decision_cells"] + loose.size
+                e = _host(exact.outputs[k])
+                dec = ((e["status"] & 0xFFF) != (g["status"] & 0xFFF)) | (e["phase_count"] != g["phase_count"])
+                assert np.all((g["status"][dec] & 0xF) == 2), (k, np.nonzero(dec)[0][:8].tolist())
+                stalled += int(dec.sum())
+            print(f"  FP64 plan vs oracle: {flips} cells differ; exact mixed plan: {stalled} stalled-descent cells",
+                  flush=True)
+            assert flips == 0
+            assert stalled <= 1e-4 * ncell
+        finally:
+            exact.close()
+            plan.close()
+    finally:
+        for m in mixes:
+            m.close()

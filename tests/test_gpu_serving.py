@@ -43,6 +43,19 @@ def test_server_results_alone_and_in_the_mix(gpu):
         a = _alone(mixtures[nc], p, t, z)
         for k in KEYS:
             assert np.array_equal(r[k], a[k], equal_nan=True), (nc, p.size, k)
+    # the served records against the oracle: every cell of the jobs up to 20,000 cells
+    import parity
+
+    checked = 0
+    for (nc, p, t, z), r in zip(jobs, results):
+        if p.size > 20_000:
+            continue
+        o = ob.flash(mixtures[nc], p, t, z, 1e-11, 3)
+        rep, bad = parity.compare(r, o)
+        assert rep["decision_cells"] == 0 and rep["value_mismatch"] == 0, (nc, p.size, rep)
+        checked += p.size
+    print(f"  {checked} served cells match the oracle", flush=True)
+    assert checked > 10_000
 
 
 @pytest.mark.gpu

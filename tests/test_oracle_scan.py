@@ -35,7 +35,8 @@ def test_reference_regression_cases_on_synthetic_records():
     # same cardinality: no proposal; indeterminate flash: indeterminate, no proposal
     assert code[1] == S.SAME and target[1] == 3 and np.all(np.isnan(dens[:, 1]))
     assert code[2] == S.INDETERMINATE and target[2] == 0 and np.all(np.isnan(dens[:, 2]))
-    assert S.summary(code) == (S.STATUS_INDETERMINATE, 2, False, 1)
+    # the indeterminate cell ends the scan and clears the proposals collected before it
+    assert S.summary(code) == (S.STATUS_INDETERMINATE, 2, False, 0)
 
 
 def test_target_density_that_is_not_positive_is_indeterminate():
@@ -43,15 +44,35 @@ def test_target_density_that_is_not_positive_is_indeterminate():
     assert code[0] == S.INDETERMINATE and target[0] == 2 and np.all(np.isnan(dens[:, 0]))
 
 
-def test_rejected_feed_is_an_error():
+def test_rejected_feed_is_indeterminate():
+    """A validated source whose feed the flash rejects: the flash's std::domain_error is
+    caught by the scanner's catch-all as indeterminate (:292-299)."""
     code, target, _ = S.decide([2], [1.0e6], [300.0], [2 | S.FLAG_REJECTED], [0], np.full((3, 1), np.nan))
-    assert code[0] == S.ERROR and target[0] == 0
+    assert code[0] == S.INDETERMINATE and target[0] == 0
+
+
+def test_lower_targets_need_disappearance_evidence():
+    """make_phase_set_transition_candidate_from_flash (phase_set_transition_flash_adapter.hpp:
+    171-183) raises invalid_argument for a lower target without accepted phase-disappearance
+    evidence; the PR76 max3 report carries it only for 3 -> 2 through the tail's fresh
+    two-phase neighbour (max3 two-phase after an unstable base split, pr76_pt_flash_backend.hpp:
+    138-146). Higher targets always carry source-challenge and accepted-target evidence."""
+    p, t = np.full(4, 1.0e7), np.full(4, 350.0)
+    zf = np.array([[0.8] * 4, [0.3] * 4, [np.nan] * 4])
+    direct = 1 << 4 | 1 << 8            # accepted, max3 two-phase, base split two-phase
+    neighbour = 1 << 4 | 2 << 8         # accepted, max3 two-phase, base split unstable
+    code, target, dens = S.decide([3, 3, 1, 1], p, t, [direct, neighbour, direct, neighbour], [2] * 4, zf)
+    assert code.tolist() == [S.ERROR, S.PROPOSAL, S.PROPOSAL, S.PROPOSAL]
+    assert np.all(np.isnan(dens[:, 0])) and np.all(dens[:2, 1] > 0)
+    assert S.summary(code) == (S.COMPLETE, 0, True, 0)
 
 
 def test_first_stopping_cell_decides_the_scan():
     assert S.summary([0, 1, 0]) == (S.COMPLETE, -1, False, 1)
+    # an error returns with the proposals before it; an indeterminate cell clears them
     assert S.summary([0, 1, 3, 2]) == (S.COMPLETE, 2, True, 1)
-    assert S.summary([1, 2, 3, 1]) == (S.STATUS_INDETERMINATE, 1, False, 2)
+    assert S.summary([1, 1, 3, 1]) == (S.COMPLETE, 2, True, 2)
+    assert S.summary([1, 2, 3, 1]) == (S.STATUS_INDETERMINATE, 1, False, 0)
 
 
 def test_source_validation():
@@ -111,15 +132,17 @@ def test_pr76_two_to_three_proposal_with_the_checker_flash():
 
 
 def test_single_phase_density_from_the_supplied_root():
-    """density mode 1: a 2 -> 1 proposal takes p / (Z R T) of the supplied Z; without it
-    (the record publishes no single-phase Z) the cell stays indeterminate; a supplied Z that
-    is not positive is indeterminate too; multi-phase targets ignore it."""
+    """density mode 1: a 2 -> 1 candidate takes p / (Z R T) of the supplied Z and then meets
+    the evidence rule, which a single-phase target never passes (an error, as the
+    reference's adapter raises); without a Z (the record publishes no single-phase Z) the
+    cell stays indeterminate; a supplied Z that is not positive is indeterminate too;
+    multi-phase targets ignore it."""
     p, t = np.array([2.0e6, 2.0e6, 2.0e6]), np.array([320.0, 320.0, 320.0])
     status = np.array([_accepted(1), _accepted(1), _accepted(2)])
     zf = np.array([[np.nan, np.nan, 0.9], [np.nan, np.nan, 0.1], [np.nan] * 3])
     code, _, _ = S.decide([2, 2, 1], p, t, status, [1, 1, 2], zf)
     assert code.tolist() == [S.INDETERMINATE, S.INDETERMINATE, S.PROPOSAL]
     code, target, dens = S.decide([2, 2, 1], p, t, status, [1, 1, 2], zf, single_z=[0.85, -0.2, 5.0])
-    assert code.tolist() == [S.PROPOSAL, S.INDETERMINATE, S.PROPOSAL] and target.tolist() == [1, 1, 2]
-    assert dens[0, 0] == 2.0e6 / (0.85 * R * 320.0) and np.all(np.isnan(dens[1:, 0]))
+    assert code.tolist() == [S.ERROR, S.INDETERMINATE, S.PROPOSAL] and target.tolist() == [1, 1, 2]
+    assert np.all(np.isnan(dens[:, 0])) and np.all(np.isnan(dens[:, 1]))
     assert dens[0, 2] == 2.0e6 / (0.9 * R * 320.0)
