@@ -152,6 +152,24 @@ int mpmc_flash_batch_ws(mpmc_mixture_t mixture, int64_t ncell, const double* p, 
                         const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* Bytes of the static scratch of mpmc_flash_batch_static (256 MiB). */
+#define MPMC_STATIC_SCRATCH_BYTES (256u * 1024u * 1024u)
+
+/* mpmc_flash_batch without any allocation: the staged schedule's workspace is a static
+ * device array of MPMC_STATIC_SCRATCH_BYTES that every device running the library holds
+ * until the process ends; the call itself allocates no device memory. No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Limit: ncell with
+ * mpmc_flash_workspace_bytes(nc, ncell) <= MPMC_STATIC_SCRATCH_BYTES (about 40,000 cells
+ * at nc = 6, 20,000 at nc = 12); larger batches are rejected with MPMC_E_INVALID_ARGUMENT
+ * before any device work. Every call on one device shares that device's array: calls
+ * that overlap on one device (other streams, other host threads) race and must be
+ * serialised by the caller. Each call writes every workspace word it reads, so no state
+ * carries over between calls. Results are those of mpmc_flash_batch. */
+int mpmc_flash_batch_static(mpmc_mixture_t mixture, int64_t ncell, const double* p, const double* t,
+                            const double* z, const mpmc_flash_options* options, const mpmc_flash_outputs* out,
+                            void* stream);
+
 /* Warm-started batch flash: up to 2 compositions per cell, hints[(h*nc + i)*ncell + c]
  * (device memory), appended in order to the initial stability test's starts after the
  * generated ones, as Pr76PtFlashBackendOptions::initial_starts does for the

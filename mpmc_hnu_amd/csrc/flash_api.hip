@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <algorithm>
@@ -74,6 +75,40 @@ size_t workspace_bytes_for(int nc, int64_t ncell) {
 #undef MPMC_CASE
     default: return 0;
     }
+}
+
+// Where the staged schedule's workspace comes from: the mixture's own (grown by
+// hipMalloc), the caller's, or the static scratch of mpmc_flash_batch_static.
+enum class Workspace { own, caller, static_scratch };
+
+// Scratch of mpmc_flash_batch_static: the staged schedule's workspace for callers that
+// cannot allocate. Every device that runs the library holds its own copy until the process
+// ends. In host code `scratch` is only a stand-in for the symbol; the device address of
+// each device's copy comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address reaches kernels.
+static __device__ __align__(256) unsigned char scratch[MPMC_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address, looked up on the first call on that device.
+int static_scratch(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { return hip_fail(e, "flash_batch_static: hipGetDevice"); }
+    if (dev < 0 || dev >= kMaxDevices) { return fail(MPMC_E_DEVICE, "flash_batch_static: device index out of range"); }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            return fail(MPMC_E_DEVICE, std::string("flash_batch_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return MPMC_OK;
 }
 
 } // namespace
@@ -197,13 +232,20 @@ static int flash_buffers(const double* p, const double* t, const double* z, cons
 
 static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                             const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
-                            size_t workspace_bytes, bool own_workspace, void* stream, int nhint = 0,
+                            size_t workspace_bytes, Workspace source, void* stream, int nhint = 0,
                             const double* hints = nullptr, int nfhint = 0, const double* fhints = nullptr) {
     if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: null mixture or outputs"); }
     if (ncell < 0 || ncell > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch: bad ncell"); }
     mpmc_flash_options o;
     if (const int rc = flash_options(options, o); rc != MPMC_OK) { return rc; }
     if (ncell == 0) { return MPMC_OK; }
+    if (source == Workspace::static_scratch && o.schedule == 0 &&
+        workspace_bytes_for(m->nc, ncell) > MPMC_STATIC_SCRATCH_BYTES) {
+        return fail(MPMC_E_INVALID_ARGUMENT,
+                    "flash_batch_static: " + std::to_string(ncell) + " cells need " +
+                        std::to_string(workspace_bytes_for(m->nc, ncell)) + " bytes of scratch; the static scratch holds " +
+                        std::to_string(static_cast<size_t>(MPMC_STATIC_SCRATCH_BYTES)) + " bytes");
+    }
     if (const int rc = flash_buffers(p, t, z, out); rc != MPMC_OK) { return rc; }
     const mpmc::dev::FlashOut fo{out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
                                  out->ln_phi, out->counters};
@@ -225,7 +267,10 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
         return e == hipSuccess ? MPMC_OK : hip_fail(e, "flash_batch launch");
     }
     const size_t need = workspace_bytes_for(m->nc, ncell);
-    if (own_workspace) {
+    if (source == Workspace::static_scratch) {
+        if (const int rc = static_scratch(&workspace); rc != MPMC_OK) { return rc; }
+        workspace_bytes = MPMC_STATIC_SCRATCH_BYTES;
+    } else if (source == Workspace::own) {
         if (m->workspace_bytes < need) {
             if (m->workspace) {
                 // the previous workspace may still be read by work queued on this stream
@@ -267,13 +312,18 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
 
 int mpmc_flash_batch(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                      const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* stream) {
-    return flash_batch_impl(m, ncell, p, t, z, options, out, nullptr, 0, true, stream);
+    return flash_batch_impl(m, ncell, p, t, z, options, out, nullptr, 0, Workspace::own, stream);
 }
 
 int mpmc_flash_batch_ws(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                         const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                         size_t workspace_bytes, void* stream) {
-    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, false, stream);
+    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, Workspace::caller, stream);
+}
+
+int mpmc_flash_batch_static(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
+                            const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* stream) {
+    return flash_batch_impl(m, ncell, p, t, z, options, out, nullptr, 0, Workspace::static_scratch, stream);
 }
 
 int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
@@ -286,8 +336,9 @@ int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, cons
     if ((nhint > 0 && ncell > 0 && !is_device_ptr(hints)) || (nfinal > 0 && ncell > 0 && !is_device_ptr(final_hints))) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: hints must be device memory");
     }
-    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, workspace == nullptr,
-                            stream, nhint, hints, nfinal, final_hints);
+    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes,
+                            workspace == nullptr ? Workspace::own : Workspace::caller, stream, nhint, hints, nfinal,
+                            final_hints);
 }
 
 int mpmc_flash_batch_hinted(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
@@ -297,8 +348,8 @@ int mpmc_flash_batch_hinted(mpmc_mixture_t m, int64_t ncell, const double* p, co
     if (nhint > 0 && ncell > 0 && !is_device_ptr(hints)) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: hints must be device memory");
     }
-    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes, workspace == nullptr,
-                            stream, nhint, hints);
+    return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes,
+                            workspace == nullptr ? Workspace::own : Workspace::caller, stream, nhint, hints);
 }
 
 // ---------------------------------------------------------------- ragged multi-mixture batch

@@ -23,6 +23,7 @@ EXPORTED = (
     "mpmc_flash_default_options",
     "mpmc_flash_batch",
     "mpmc_flash_batch_ws",
+    "mpmc_flash_batch_static",
     "mpmc_flash_workspace_bytes",
     "mpmc_flash_multi_max_mixtures",
     "mpmc_flash_multi_workspace_bytes",
@@ -186,6 +187,8 @@ def library() -> ctypes.CDLL:
     lib.mpmc_flash_default_options.restype = None
     lib.mpmc_flash_batch.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOptions),
                                      ctypes.POINTER(FlashOutputs), P]
+    lib.mpmc_flash_batch_static.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOptions),
+                                            ctypes.POINTER(FlashOutputs), P]
     lib.mpmc_flash_batch_ws.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.POINTER(FlashOptions),
                                         ctypes.POINTER(FlashOutputs), P, ctypes.c_size_t, P]
     lib.mpmc_flash_batch_hinted.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int, P, ctypes.POINTER(FlashOptions),
