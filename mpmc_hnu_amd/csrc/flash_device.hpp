@@ -406,9 +406,29 @@ MPMC_HOT int rachford_rice(const double* z, const double* lk, RRState<NC>& r, Co
     r.mass_abs = 0;
     r.mass_rel = 0;
     if (contrast == 0) { return RR_DEGENERATE; }
-    const double ls = rr_log_sum<NC>(z, lk, 1.0), rs = rr_log_sum<NC>(z, lk, -1.0);
-    if (!finite(ls) || !finite(rs)) { return RR_UNREPRESENTABLE; }
-    if (ls <= 64 * kEps || rs <= 64 * kEps) { return RR_NO_INTERIOR; }
+    {
+        // The reference's interior-root test: ls = log(sum z_i K_i) and rs = log(sum z_i / K_i)
+        // in the overflow-safe log-sum-exp form, with no interior root when either is within
+        // 64 eps of zero. With every |ln K_i| below 700 both are finite, and a plain sum
+        // S+ = sum z_i K_i (S- likewise) carries a relative error of a few eps: S > 1 + 1e-9
+        // then certifies ls (rs) > 64 eps, and the 2 nc + 2 logarithms and 2 nc exponentials
+        // of the exact test are evaluated only when a sum is within that band.
+        bool quick = true;
+        double sp = 0, sm = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            if (z[i] == 0) { continue; }
+            quick = quick && __builtin_fabs(lk[i]) < 700.0;
+            const double inv = 1.0 / e[i];
+            sp += z[i] * (lk[i] >= 0 ? inv : e[i]);
+            sm += z[i] * (lk[i] >= 0 ? e[i] : inv);
+        }
+        if (!(quick && sp > 1.0 + 1e-9 && sm > 1.0 + 1e-9)) {
+            const double ls = rr_log_sum<NC>(z, lk, 1.0), rs = rr_log_sum<NC>(z, lk, -1.0);
+            if (!finite(ls) || !finite(rs)) { return RR_UNREPRESENTABLE; }
+            if (ls <= 64 * kEps || rs <= 64 * kEps) { return RR_NO_INTERIOR; }
+        }
+    }
     // Certified signs. With e_i, num_i fixed, the exact F(b) = sum z_i num_i / den_i(b) is
     // strictly decreasing on (0,1) with |F'| >= S2 = sum z_i num_i^2 (den_i <= 1), and the
     // reference's computed value differs from F(b) by at most E(b) = 32 eps A / min(b,1-b)
@@ -420,7 +440,32 @@ MPMC_HOT int rachford_rice(const double* z, const double* lk, RRState<NC>& r, Co
     RRBracket cert = rr_enclose<NC>(z, lk, e, num, guess);
     double left = 0, right = 1, beta = 0.5, value = 0;
     bool resolved = false;
-    for (int it = 0; it < 192; ++it) {
+    int it0 = 0;
+    if (cert.ok) {
+        // Every bisection interval is a dyadic interval [j 2^-k, (j+1) 2^-k] (midpoints of
+        // dyadic endpoints in [0, 1] are exact), and a midpoint farther than m from the
+        // enclosure is skipped by the rule below: every skipped midpoint lies in
+        // (lo/2, (1 + hi)/2), where the rule's bound 32 eps A / min(b, 1 - b) is below
+        // s2 m / 2. So the first k levels, while [lo - m, hi + m] stays inside one dyadic
+        // interval, are taken at once: k = the common leading bits of the two ends (at most
+        // 49, where intervals are still wider than the termination width).
+        const double m = 128.0 * kEps * cert.a / (cert.s2 * smin(cert.lo, 1 - cert.hi));
+        const double lo = cert.lo - m, hi = cert.hi + m;
+        if (finite(m) && lo > 0 && hi < 1) {
+            const uint64_t x = static_cast<uint64_t>(lo * 4503599627370496.0); // floor(lo 2^52)
+            const uint64_t y = static_cast<uint64_t>(hi * 4503599627370496.0);
+            int k = x == y ? 52 : __builtin_clzll(x ^ y) - 12;
+            k = k > 49 ? 49 : k;
+            if (k > 0) {
+                const uint64_t keep = ~((uint64_t{1} << (52 - k)) - 1);
+                left = __builtin_ldexp(static_cast<double>(x & keep), -52);
+                right = left + __builtin_ldexp(1.0, -k);
+                it0 = k;
+                cnt.rr += static_cast<uint32_t>(k);
+            }
+        }
+    }
+    for (int it = it0; it < 192; ++it) {
         ++cnt.rr;
         beta = (left + right) / 2;
         if (!(beta > 0 && beta < 1)) { break; }

@@ -34,7 +34,7 @@ def host_threads(threads: int = 0) -> int:
     run on, at most 16 (a GPU box gives one GPU's jobs a 16-CPU share whatever the
     machine reports)."""
     if threads > 0:
-        return host_threads(threads)
+        return int(threads)
     try:
         avail = len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
