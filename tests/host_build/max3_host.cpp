@@ -32,7 +32,7 @@ int main(int argc, char** argv) {
     for (long c = first; c < last && c < n; ++c) {
         double feed[NC];
         for (int i = 0; i < NC; ++i) { feed[i] = z[i * n + c]; }
-        static Max3Out<NC> M;
+        Max3Out<NC> M;
         Counters cnt{0, 0, 0, 0};
         bool rej = false;
         solve_max3<NC>(m, p[c], t[c], feed, 1e-11, M, cnt, rej);
