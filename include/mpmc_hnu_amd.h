@@ -54,6 +54,21 @@ enum mpmc_split_status { MPMC_SPLIT_SINGLE = 0, MPMC_SPLIT_TWO = 1, MPMC_SPLIT_U
 /* flag: the reference rejects this cell's input with an exception (invalid_argument /
  * domain_error); no phase set is published. */
 #define MPMC_FLAG_REJECTED 0x2000
+/* flags of the search behind the published set, the facts the reference's transition
+ * evidence reads from its solve result (project_pr76_pt_max3_transition_report and
+ * project_pr76_pt_transition_report, pr76_pt_flash_backend.hpp:42-164):
+ *   SPLIT_DISAPPEARANCE  an attempt of the base two-phase split ended at a phase-fraction
+ *                        endpoint (PtSplitAttemptStatus::phase_disappearance)
+ *   SPLIT_CANDIDATE      the base split selected a converged pair (PtSplitResult::candidate())
+ *   MAX3_ATTEMPTED       the three-phase stage ran at least one attempt
+ *                        (Pr76PtMax3Result::attempts not empty)
+ *   START_CONSUMED       an attempt ran from the caller-supplied three-phase start
+ *                        (pr76_pt_continuation_supplied_start_consumed,
+ *                        pr76_pt_continuation.hpp:183-190) */
+#define MPMC_FLAG_SPLIT_DISAPPEARANCE 0x4000
+#define MPMC_FLAG_SPLIT_CANDIDATE 0x8000
+#define MPMC_FLAG_MAX3_ATTEMPTED 0x10000
+#define MPMC_FLAG_START_CONSUMED 0x20000
 
 /* branch word per phase: provider root index | smooth << 8 ; -1 when the slot is empty */
 #define MPMC_BRANCH_INDEX(b) ((b) & 0xFF)
@@ -170,7 +185,7 @@ int mpmc_flash_batch_static(mpmc_mixture_t mixture, int64_t ncell, const double*
                             const double* z, const mpmc_flash_options* options, const mpmc_flash_outputs* out,
                             void* stream);
 
-/* Warm-started batch flash: up to 2 compositions per cell, hints[(h*nc + i)*ncell + c]
+/* Warm-started batch flash: up to 3 compositions per cell, hints[(h*nc + i)*ncell + c]
  * (device memory), appended in order to the initial stability test's starts after the
  * generated ones, as Pr76PtFlashBackendOptions::initial_starts does for the
  * initial_stability_starts of PtSolveHints (pt_solve_hints.hpp:26-43,
@@ -184,7 +199,7 @@ int mpmc_flash_batch_hinted(mpmc_mixture_t mixture, int64_t ncell, const double*
                             const mpmc_flash_options* options, const mpmc_flash_outputs* out,
                             void* workspace, size_t workspace_bytes, void* stream);
 
-/* mpmc_flash_batch_hinted plus final-review warm starts: up to 2 compositions per cell,
+/* mpmc_flash_batch_hinted plus final-review warm starts: up to 3 compositions per cell,
  * final_hints[(h*nc + i)*ncell + c] (device memory), placed before the selected pair's two
  * compositions in the common-tangent review, as Pr76PtFlashBackendOptions::final_starts
  * does for the final_two_phase_stability_starts of PtSolveHints (pt_solve_hints.hpp:34-36,
@@ -194,6 +209,34 @@ int mpmc_flash_batch_warm(mpmc_mixture_t mixture, int64_t ncell, const double* p
                           const double* z, int nhint, const double* hints, int nfinal, const double* final_hints,
                           const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                           size_t workspace_bytes, void* stream);
+
+/*
+ * Ordered-path PT continuation (solve_pr76_pt_continuation, pr76_pt_continuation.hpp:194-310)
+ * of npath paths at once. Path j has the feed z[i*npath + j] and npoint ordered states
+ * p[k*npath + j], t[k*npath + j] (device memory); point k of every path is a max3 flash
+ * (options->max_phases must be 3; staged schedule) of the path feed normalised once, and
+ * the points of one path run in order while the paths run side by side. carry bit 0
+ * (carry_previous_phase_compositions): the previous point's accepted phase compositions
+ * (deduplicated; a single phase carries its normalised feed) are the initial and the
+ * final-review stability starts of the next point; bit 1 (carry_previous_three_phase_state):
+ * a three-phase previous point's compositions and phase-1/2 fractions are the next point's
+ * first three-phase start. After a point that accepts no phase set nothing is carried.
+ * out holds npoint consecutive flash records of npath cells (the mpmc_flash_outputs layout
+ * with ncell = npath, point k at offset k times one record); MPMC_FLAG_START_CONSUMED marks a
+ * point whose three-phase stage ran from the carried start. point_info (device, [npoint]
+ * [npath], may be NULL) = incoming hint phase count (0: none) | carried stability start
+ * count << 4 | carried three-phase start << 8. A path whose feed or states fail the
+ * reference's path check, or one of whose points is rejected, is published as rejected
+ * (MPMC_FLAG_REJECTED) at every point with point_info 0, as the reference rejects the whole
+ * path. The paths' transition brackets follow from consecutive accepted phase counts.
+ * workspace NULL: the mixture's own, else mpmc_pt_continuation_workspace_bytes(nc, npath)
+ * bytes of device memory.
+ */
+size_t mpmc_pt_continuation_workspace_bytes(int nc, int64_t npath);
+int mpmc_pt_continuation(mpmc_mixture_t mixture, int64_t npath, int npoint, const double* p, const double* t,
+                         const double* z, int carry, const mpmc_flash_options* options,
+                         const mpmc_flash_outputs* out, int32_t* point_info, void* workspace,
+                         size_t workspace_bytes, void* stream);
 
 /*
  * Ragged multi-mixture batch: the cells of several mixtures with the same component count

@@ -134,7 +134,7 @@ __device__ __forceinline__ int phase_dual_body(const Mix<NC>& m, double p, doubl
     const RootSet rs = pr76_roots<false>(A.v, B.v);
     if (rs.status != RS_SUCCESS) { return PF_PHASE; }
     if (root_index < 0 || root_index >= rs.count) { return PF_INDEX; }
-    const Root r = root_index == 0 ? rs.r[0] : (root_index == 1 ? rs.r[1] : rs.r[2]);
+    const Root r = root_at(rs, root_index);
     if (!r.dvalid) { return PF_PHASE; }
     const double scale = smax(smax(1.0, B.v), __builtin_sqrt(__builtin_fabs(A.v)));
     const double y0 = r.x / scale;

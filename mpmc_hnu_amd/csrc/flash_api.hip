@@ -3,6 +3,7 @@
 // outcomes are per-cell statuses written by the kernels.
 #include "assembly_device.hpp"
 #include "closure_device.hpp"
+#include "continuation_kernels.hpp"
 #include "flash_pipeline.hpp"
 
 #include "../../include/mpmc_hnu_amd.h"
@@ -230,6 +231,39 @@ static int flash_buffers(const double* p, const double* t, const double* z, cons
     return MPMC_OK;
 }
 
+// The workspace of a call: the mixture's own (grown to `need` bytes; the previous one is
+// freed once the stream has drained) or the caller's, which must be device memory of at
+// least `need` bytes.
+static int acquire_workspace(mpmc_mixture_t m, size_t need, Workspace source, hipStream_t s, void*& workspace,
+                             size_t& workspace_bytes, const char* caller_contract) {
+    if (source == Workspace::own) {
+        if (m->workspace_bytes < need) {
+            if (m->workspace) {
+                // the previous workspace may still be read by work queued on this stream
+                const hipError_t e = hipStreamSynchronize(s);
+                if (e != hipSuccess) { return hip_fail(e, "flash workspace sync"); }
+                (void)hipFree(m->workspace);
+                m->workspace = nullptr;
+                m->workspace_bytes = 0;
+            }
+            if (hipMalloc(&m->workspace, need) != hipSuccess) {
+                (void)hipGetLastError();
+                m->workspace = nullptr;
+                return fail(MPMC_E_NO_MEMORY, std::string("flash: workspace allocation of ") + std::to_string(need) +
+                                                  " bytes failed");
+            }
+            m->workspace_bytes = need;
+        }
+        workspace = m->workspace;
+        workspace_bytes = m->workspace_bytes;
+        return MPMC_OK;
+    }
+    if (!workspace || workspace_bytes < need || !is_device_ptr(workspace)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, caller_contract);
+    }
+    return MPMC_OK;
+}
+
 static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                             const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                             size_t workspace_bytes, Workspace source, void* stream, int nhint = 0,
@@ -270,38 +304,18 @@ static int flash_batch_impl(mpmc_mixture_t m, int64_t ncell, const double* p, co
     if (source == Workspace::static_scratch) {
         if (const int rc = static_scratch(&workspace); rc != MPMC_OK) { return rc; }
         workspace_bytes = MPMC_STATIC_SCRATCH_BYTES;
-    } else if (source == Workspace::own) {
-        if (m->workspace_bytes < need) {
-            if (m->workspace) {
-                // the previous workspace may still be read by work queued on this stream
-                e = hipStreamSynchronize(s);
-                if (e != hipSuccess) { return hip_fail(e, "flash_batch workspace sync"); }
-                (void)hipFree(m->workspace);
-                m->workspace = nullptr;
-                m->workspace_bytes = 0;
-            }
-            e = hipMalloc(&m->workspace, need);
-            if (e != hipSuccess) {
-                m->workspace = nullptr;
-                return fail(MPMC_E_NO_MEMORY, std::string("flash_batch: workspace allocation of ") +
-                                                  std::to_string(need) + " bytes failed");
-            }
-            m->workspace_bytes = need;
-        }
-        workspace = m->workspace;
-        workspace_bytes = m->workspace_bytes;
-    } else {
-        if (!workspace || workspace_bytes < need || !is_device_ptr(workspace)) {
-            return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_ws: workspace must be device memory of "
-                                                 "mpmc_flash_workspace_bytes(nc, ncell) bytes");
-        }
+    } else if (const int rc = acquire_workspace(m, need, source, s, workspace, workspace_bytes,
+                                                "flash_batch_ws: workspace must be device memory of "
+                                                "mpmc_flash_workspace_bytes(nc, ncell) bytes");
+               rc != MPMC_OK) {
+        return rc;
     }
     switch (m->nc) {
 #define MPMC_CASE(N)                                                                                             \
     case N:                                                                                                      \
         e = mpmc::dev::launch_pipeline<N>(m->dev_constants, 1, nullptr, workspace, workspace_bytes, ncell, p, t,  \
                                           z, o.max_phases, o.fugacity_tolerance, o.precision, fo, s, nhint,      \
-                                          hints, nfhint, fhints);                                                \
+                                          hints, nfhint, fhints, nullptr, nullptr);                              \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE
@@ -330,8 +344,8 @@ int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, cons
                           int nhint, const double* hints, int nfinal, const double* final_hints,
                           const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                           size_t workspace_bytes, void* stream) {
-    if (nhint < 0 || nhint > 2 || nfinal < 0 || nfinal > 2) {
-        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: nhint and nfinal must be 0..2");
+    if (nhint < 0 || nhint > 3 || nfinal < 0 || nfinal > 3) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: nhint and nfinal must be 0..3");
     }
     if ((nhint > 0 && ncell > 0 && !is_device_ptr(hints)) || (nfinal > 0 && ncell > 0 && !is_device_ptr(final_hints))) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: hints must be device memory");
@@ -344,12 +358,104 @@ int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, cons
 int mpmc_flash_batch_hinted(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                             int nhint, const double* hints, const mpmc_flash_options* options,
                             const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes, void* stream) {
-    if (nhint < 0 || nhint > 2) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: nhint must be 0..2"); }
+    if (nhint < 0 || nhint > 3) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: nhint must be 0..3"); }
     if (nhint > 0 && ncell > 0 && !is_device_ptr(hints)) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: hints must be device memory");
     }
     return flash_batch_impl(m, ncell, p, t, z, options, out, workspace, workspace_bytes,
                             workspace == nullptr ? Workspace::own : Workspace::caller, stream, nhint, hints);
+}
+
+// ---------------------------------------------------------------- ordered-path continuation
+namespace {
+// Continuation buffers after the pipeline workspace: feed [NC][n], hints [3][NC][n],
+// three-phase start [3][NC][n] + [2][n] (continuation_kernels.hpp).
+size_t continuation_extra(int nc, int64_t n) {
+    const size_t N = static_cast<size_t>(n), NC = static_cast<size_t>(nc);
+    auto a = [](size_t v) { return (v + 255) & ~size_t{255}; };
+    return a(NC * N * 8) + a(3 * NC * N * 8) + a(3 * NC * N * 8) + a(2 * N * 8);
+}
+size_t continuation_bytes(int nc, int64_t n) {
+    return ((workspace_bytes_for(nc, n) + 255) & ~size_t{255}) + continuation_extra(nc, n);
+}
+} // namespace
+
+size_t mpmc_pt_continuation_workspace_bytes(int nc, int64_t npath) {
+    if (nc < 1 || nc > MPMC_MAX_COMPONENTS || npath < 0) { return 0; }
+    return continuation_bytes(nc, npath);
+}
+
+int mpmc_pt_continuation(mpmc_mixture_t m, int64_t npath, int npoint, const double* p, const double* t,
+                         const double* z, int carry, const mpmc_flash_options* options,
+                         const mpmc_flash_outputs* out, int32_t* point_info, void* workspace, size_t workspace_bytes,
+                         void* stream) {
+    if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: null mixture or outputs"); }
+    if (npath < 0 || npath > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: bad npath"); }
+    // Pr76PtContinuationOptions::max_points (pr76_pt_continuation.hpp:33, checked at :164-166)
+    if (npoint < 1 || npoint > 4096) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: npoint must be 1..4096");
+    }
+    if (carry < 0 || carry > 3) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: carry must be 0..3"); }
+    mpmc_flash_options o;
+    if (const int rc = flash_options(options, o); rc != MPMC_OK) { return rc; }
+    if (o.max_phases != 3) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: the path points are max3 flashes"); }
+    if (o.schedule != 0) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: staged schedule only"); }
+    if (npath == 0) { return MPMC_OK; }
+    if (const int rc = flash_buffers(p, t, z, out); rc != MPMC_OK) { return rc; }
+    if (point_info && !is_device_ptr(point_info)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: point_info must be device memory");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nc = m->nc;
+    const size_t need = continuation_bytes(nc, npath);
+    if (const int rc = acquire_workspace(m, need, workspace == nullptr ? Workspace::own : Workspace::caller, s,
+                                         workspace, workspace_bytes,
+                                         "pt_continuation: workspace must be device memory of "
+                                         "mpmc_pt_continuation_workspace_bytes(nc, npath) bytes");
+        rc != MPMC_OK) {
+        return rc;
+    }
+    const size_t pipe_bytes = (workspace_bytes_for(nc, npath) + 255) & ~size_t{255};
+    char* extra = static_cast<char*>(workspace) + pipe_bytes;
+    const size_t N = static_cast<size_t>(npath), NC = static_cast<size_t>(nc);
+    auto a256 = [](size_t v) { return (v + 255) & ~size_t{255}; };
+    double* feed = reinterpret_cast<double*>(extra);
+    double* hints = reinterpret_cast<double*>(extra + a256(NC * N * 8));
+    double* tpz = reinterpret_cast<double*>(extra + a256(NC * N * 8) + a256(3 * NC * N * 8));
+    double* tpf = reinterpret_cast<double*>(extra + a256(NC * N * 8) + 2 * a256(3 * NC * N * 8));
+    const int nh = (carry & 1) ? 3 : 0;
+    const mpmc::dev::FlashOut fo{out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
+                                 out->ln_phi, out->counters};
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((npath + 255) / 256, 2048));
+    hipError_t e = hipSuccess;
+    switch (nc) {
+#define MPMC_CASE(N)                                                                                                  \
+    case N:                                                                                                           \
+        hipLaunchKernelGGL(mpmc::dev::k_cont_feed<N>, dim3(grid), dim3(256), 0, s, npath, z, feed);                   \
+        for (int k = 0; k < npoint && e == hipSuccess; ++k) {                                                         \
+            const int64_t off = static_cast<int64_t>(k) * npath;                                                      \
+            hipLaunchKernelGGL(mpmc::dev::k_cont_hints<N>, dim3(grid), dim3(256), 0, s, npath,                        \
+                               mpmc::dev::point_slice<N>(fo, npath, k > 0 ? k - 1 : 0), k > 0 ? 1 : 0, feed, carry,   \
+                               hints, tpz, tpf, point_info ? point_info + off : nullptr);                             \
+            e = hipGetLastError();                                                                                    \
+            if (e == hipSuccess) {                                                                                    \
+                e = mpmc::dev::launch_pipeline<N>(m->dev_constants, 1, nullptr, workspace, pipe_bytes, npath, p + off, \
+                                                  t + off, feed, 3, o.fugacity_tolerance, o.precision,                \
+                                                  mpmc::dev::point_slice<N>(fo, npath, k), s, nh, hints, nh, hints,   \
+                                                  (carry & 2) ? tpz : nullptr, (carry & 2) ? tpf : nullptr);          \
+            }                                                                                                         \
+        }                                                                                                             \
+        if (e == hipSuccess) {                                                                                        \
+            hipLaunchKernelGGL(mpmc::dev::k_cont_reject<N>, dim3(grid), dim3(256), 0, s, npath, npoint, fo,           \
+                               point_info);                                                                           \
+            e = hipGetLastError();                                                                                    \
+        }                                                                                                             \
+        break;
+        MPMC_FOR_EACH_NC(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: unsupported nc");
+    }
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "pt_continuation launch");
 }
 
 // ---------------------------------------------------------------- ragged multi-mixture batch
@@ -437,7 +543,7 @@ int mpmc_flash_multi(int nmix, const mpmc_mixture_t* mixtures, const int64_t* ce
     case N:                                                                                                      \
         e = mpmc::dev::launch_pipeline<N>(table, nmix, nmix > 1 ? mix_of : nullptr, workspace, pipe_bytes, ncell, \
                                           p, t, z, o.max_phases, o.fugacity_tolerance, o.precision, fo, s, 0,   \
-                                          nullptr, 0, nullptr);                                                 \
+                                          nullptr, 0, nullptr, nullptr, nullptr);                               \
         break;
         MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE
@@ -530,7 +636,7 @@ hipError_t plan_record(mpmc_flash_plan* pl, hipStream_t s) {
         e = mpmc::dev::launch_pipeline<N>(g.table, nmix, nmix > 1 ? g.mix_of : nullptr, pl->workspace,            \
                                           pl->workspace_bytes, g.n, g.p, g.t, g.z, pl->options.max_phases,        \
                                           pl->options.fugacity_tolerance, pl->options.precision, fo, s, 0,        \
-                                          nullptr, 0, nullptr);                                                   \
+                                          nullptr, 0, nullptr, nullptr, nullptr);                                 \
         break;
             MPMC_FOR_EACH_NC(MPMC_CASE)
 #undef MPMC_CASE

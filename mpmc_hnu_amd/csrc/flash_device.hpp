@@ -469,7 +469,7 @@ MPMC_HOT int rachford_rice(const double* z, const double* lk, RRState<NC>& r, Co
         ++cnt.rr;
         beta = (left + right) / 2;
         if (!(beta > 0 && beta < 1)) { break; }
-        const bool term = beta == left || beta == right || right - left <= 8 * kEps * smin(beta, 1 - beta);
+        const bool term = (beta == left) | (beta == right) | (right - left <= 8 * kEps * smin(beta, 1 - beta));
         if (cert.ok && !term) {
             const double dm = smin(beta, 1 - beta);
             if (dm > 1e-200) {
@@ -702,6 +702,7 @@ struct VleOut {
     SplitState<NC> cand;   // selected converged pair
     bool has_final;
     StabOut<NC> fin;       // common-tangent review (witnesses seed max3)
+    bool any_disappearance; // an attempt ended at a phase-fraction endpoint
 };
 
 // solve_pt_vle (pt_split.hpp:414-525) with the PR76 evaluator as both providers.
@@ -718,6 +719,7 @@ __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, con
     V.status = V_INDETERMINATE;
     V.has_candidate = false;
     V.has_final = false;
+    V.any_disappearance = false;
     if (!finite(p) || p <= 0 || !finite(t) || t <= 0) { rejected = true; return; }
     (void)check_composition<NC>(feed, rejected);
     for (int e = 0; e < nstarts; ++e) {
@@ -748,6 +750,7 @@ __device__ __noinline__ void solve_vle(const Mix<NC>& m, double p, double t, con
             }
             split_evals += evals;
             ++attempts;
+            if (status == A_DISAPPEARANCE) { V.any_disappearance = true; }
             if (status == A_CONVERGED && (!V.has_candidate || st.gibbs < V.cand.gibbs)) {
                 V.cand = st;
                 V.has_candidate = true;
@@ -1046,14 +1049,22 @@ template <int NC, bool kN = false>
 __device__ __noinline__ int witness_sides(const Mix<NC>& m, double p, double t, const double* w, int branch) {
     PhaseCtx<NC> ctx;
     if (phase_prepare<NC, kN>(m, p, t, w, ctx) != ISSUE_NONE) { return 0; }
-    int adm[3], na = 0;
-    for (int k = 0; k < ctx.rs.count; ++k) { if (ctx.rs.r[k].slope_sign > 0) { adm[na++] = k; } }
+    // first and last admissible root, in root order
+    int na = 0, first = -1, last = -1;
     bool found = false;
-    for (int k = 0; k < na; ++k) { found = found || adm[k] == branch; }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k < ctx.rs.count && root_at(ctx.rs, k).slope_sign > 0) {
+            if (na == 0) { first = k; }
+            last = k;
+            ++na;
+            found = found || k == branch;
+        }
+    }
     if (na == 0 || !found) { return 0; }
     if (na == 1) { return 3; }
-    if (branch == adm[0]) { return 1; }
-    if (branch == adm[na - 1]) { return 2; }
+    if (branch == first) { return 1; }
+    if (branch == last) { return 2; }
     return 0;
 }
 
@@ -1078,23 +1089,78 @@ enum Max3Status : int { M_SINGLE = 0, M_TWO, M_THREE, M_BOUNDARY, M_HIGHER, M_IN
 template <int NC>
 struct Max3Tail {
     int status;
+    int attempts;          // three-phase attempts run (Pr76PtMax3Result::attempts.size())
+    bool start_consumed;   // an attempt ran from the caller-supplied start
     bool use_neighbor;
     SplitState<NC> neighbor;
     P3State<NC> three;
 };
 
 // The part of solve_pr76_pt_max3 (pr76_three_phase.hpp:360-561) that runs once the
-// two-phase common-tangent review found lower-Gibbs evidence: negative final-review
-// witnesses seed three-phase attempts (witness sides from pr76_witness_root_sides),
-// each followed by the three-phase final review or, on disappearance, a fresh
-// two-phase neighbour solve. `witness(i, w, branch)` yields the i-th negative witness
-// of the review in trial order. No caller continuation hints.
+// two-phase common-tangent review found lower-Gibbs evidence. A caller-supplied three-phase
+// start (Pr76PtMax3Options::three_phase_starts with one entry: the PR76 PT continuation's
+// carried state, pr76_pt_continuation.hpp:240-251) is tried first, over the eight root-side
+// combinations (:472-498); then negative final-review witnesses seed attempts (witness
+// sides from pr76_witness_root_sides), each followed by the three-phase final review or, on
+// disappearance, a fresh two-phase neighbour solve. `witness(i, w, branch)` yields the i-th
+// negative witness of the review in trial order. start: three compositions [3][NC] or
+// nullptr; start_frac: the phase-fraction seed of phases 1 and 2.
 template <int NC, bool kN, typename WitnessFn>
 __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, const double* z, const double* px,
                                        const double* py, double pbeta, int nwit, WitnessFn witness, double fug_tol,
-                                       Max3Tail<NC>& M, Counters& cnt, bool& rejected) {
+                                       Max3Tail<NC>& M, Counters& cnt, bool& rejected, const double* start = nullptr,
+                                       const double* start_frac = nullptr) {
     M.use_neighbor = false;
+    M.attempts = 0;
+    M.start_consumed = false;
     bool saw_higher = false, saw_boundary = false;
+    // one attempt (run_attempt, pr76_three_phase.hpp:377-467); true when it closes the solve
+    // (or the cell is rejected)
+    auto attempt = [&](const double* l1, const double* l2, double b1, double b2, const int* sd) -> bool {
+        ++M.attempts;
+        int disappearing;
+        P3State<NC> E;
+        const int st = iterate_three<NC, kN>(m, p, t, z, l1, l2, b1, b2, sd, E, disappearing, cnt, rejected);
+        if (rejected) { return true; }
+        if (st == P_DISAPPEARANCE) {
+            saw_boundary = true;
+            if (disappearing >= 0) {
+                const double* stv[2];
+                int ns = 0;
+                for (int ph = 0; ph < 3; ++ph) { if (ph != disappearing) { stv[ns++] = E.comp[ph]; } }
+                VleOut<NC> nb;
+                solve_vle<NC, kN>(m, p, t, z, fug_tol, stv, 2, nb, cnt, rejected);
+                if (rejected) { return true; }
+                if (nb.status == V_TWO) {
+                    M.status = M_TWO;
+                    M.use_neighbor = true;
+                    M.neighbor = nb.cand;
+                    return true;
+                }
+            }
+            return false;
+        }
+        if (st != P_CONVERGED) { return false; }
+        if (!(E.norm <= 1e-11 && E.rr_res <= 2e-13 && E.mass_abs <= 1e-12 && E.mass_rel <= 1e-10)) { return false; }
+        const double* stv[3] = {E.comp[0], E.comp[1], E.comp[2]};
+        StabOut<NC> F;
+        stability<NC, kN>(m, p, t, z, E.common, stv, 3, 1e-10 + E.norm, F, cnt, rejected);
+        if (rejected) { return true; }
+        if (F.status == S_UNSTABLE) { saw_higher = true; }
+        else if (F.status == S_NO_INSTABILITY) { M.status = M_THREE; M.three = E; return true; }
+        return false;
+    };
+    if (start) {
+        double l1[NC], l2[NC];
+        if (log_ratio<NC>(start + NC, start, z, l1) && log_ratio<NC>(start + 2 * NC, start, z, l2)) {
+            // side of phase 0 outermost, of phase 2 innermost; 0 = lower, 1 = upper admissible
+            for (int code = 0; code < 8; ++code) {
+                const int sd[3] = {code >> 2, (code >> 1) & 1, code & 1};
+                M.start_consumed = true;
+                if (attempt(l1, l2, start_frac[0], start_frac[1], sd)) { return; }
+            }
+        }
+    }
     int automatic = 0;
     bool auto_limit = false;
     double src1[NC];
@@ -1111,41 +1177,25 @@ __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, con
                 if (automatic >= 16) { auto_limit = true; break; }
                 ++automatic;
                 const int sd[3] = {0, 1, s};
-                int disappearing;
-                P3State<NC> E;
-                const int st = iterate_three<NC, kN>(m, p, t, z, src1, l2, (1.0 - 0.1) * pbeta, 0.1, sd, E, disappearing,
-                                                 cnt, rejected);
-                if (rejected) { return; }
-                if (st == P_DISAPPEARANCE) {
-                    saw_boundary = true;
-                    if (disappearing >= 0) {
-                        const double* stv[2];
-                        int ns = 0;
-                        for (int ph = 0; ph < 3; ++ph) { if (ph != disappearing) { stv[ns++] = E.comp[ph]; } }
-                        VleOut<NC> nb;
-                        solve_vle<NC, kN>(m, p, t, z, fug_tol, stv, 2, nb, cnt, rejected);
-                        if (rejected) { return; }
-                        if (nb.status == V_TWO) {
-                            M.status = M_TWO;
-                            M.use_neighbor = true;
-                            M.neighbor = nb.cand;
-                            return;
-                        }
-                    }
-                    continue;
-                }
-                if (st != P_CONVERGED) { continue; }
-                if (!(E.norm <= 1e-11 && E.rr_res <= 2e-13 && E.mass_abs <= 1e-12 && E.mass_rel <= 1e-10)) { continue; }
-                const double* stv[3] = {E.comp[0], E.comp[1], E.comp[2]};
-                StabOut<NC> F;
-                stability<NC, kN>(m, p, t, z, E.common, stv, 3, 1e-10 + E.norm, F, cnt, rejected);
-                if (rejected) { return; }
-                if (F.status == S_UNSTABLE) { saw_higher = true; }
-                else if (F.status == S_NO_INSTABILITY) { M.status = M_THREE; M.three = E; return; }
+                if (attempt(src1, l2, (1.0 - 0.1) * pbeta, 0.1, sd)) { return; }
             }
         }
     }
     M.status = saw_higher ? M_HIGHER : (saw_boundary ? M_BOUNDARY : M_INDETERMINATE);
+}
+
+// A caller-supplied three-phase start is valid when its fraction seed lies in the simplex
+// and each composition is a composition with the feed's support; otherwise the reference
+// rejects the request (pr76_pt_max3_check_starts, pr76_three_phase.hpp:211-242).
+template <int NC>
+__device__ __forceinline__ void check_three_phase_start(const double* feed, const double* start,
+                                                        const double* start_frac, bool& rejected) {
+    if (!rr3_feasible(start_frac[0], start_frac[1])) { rejected = true; }
+    for (int ph = 0; ph < 3; ++ph) {
+        (void)check_composition<NC>(start + ph * NC, rejected);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { if ((feed[i] > 0) != (start[ph * NC + i] > 0)) { rejected = true; } }
+    }
 }
 
 template <int NC>
@@ -1155,8 +1205,8 @@ struct Max3Out {
     Max3Tail<NC> tail;
 };
 
-// solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), without caller continuation hints,
-// serial in one lane.
+// solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), serial in one lane; start /
+// start_frac: one caller-supplied three-phase start (three_phase_starts), or nullptr.
 template <int NC, bool kN = false>
 // initial / final: extra starts of the initial stability test and of the base split's
 // common-tangent review (solve_pr76_pt_max3 initial_starts / final_starts,
@@ -1164,9 +1214,13 @@ template <int NC, bool kN = false>
 __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, const double* feed, double fug_tol,
                                         Max3Out<NC>& M, Counters& cnt, bool& rejected,
                                         const double* const* initial = nullptr, int ninitial = 0,
-                                        const double* const* final_starts = nullptr, int nfinal = 0) {
+                                        const double* const* final_starts = nullptr, int nfinal = 0,
+                                        const double* start = nullptr, const double* start_frac = nullptr) {
     M.tail.use_neighbor = false;
+    M.tail.attempts = 0;
+    M.tail.start_consumed = false;
     (void)check_composition<NC>(feed, rejected);
+    if (start) { check_three_phase_start<NC>(feed, start, start_frac, rejected); }
     if (rejected) { return; }
     solve_vle<NC, kN>(m, p, t, feed, fug_tol, initial, ninitial, M.base, cnt, rejected, final_starts, nfinal);
     if (rejected) { return; }
@@ -1184,7 +1238,7 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
                       for (int c = 0; c < NC; ++c) { w[c] = fin.wit[i].w[c]; }
                       branch = fin.wit[i].branch;
                   },
-                  fug_tol, M.tail, cnt, rejected);
+                  fug_tol, M.tail, cnt, rejected, start, start_frac);
     M.status = M.tail.status;
 }
 

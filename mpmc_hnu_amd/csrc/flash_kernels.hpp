@@ -72,25 +72,34 @@ __device__ __forceinline__ int pair_view(const SplitState<NC>& s, PhaseView<NC>*
     return 2;
 }
 
+// Status-word flags of the base split (MPMC_FLAG_SPLIT_DISAPPEARANCE, MPMC_FLAG_SPLIT_CANDIDATE):
+// the facts of PtSplitResult that the reference's transition evidence reads
+// (pr76_pt_flash_backend.hpp:42-95).
+template <int NC>
+__device__ __forceinline__ int split_flags(const VleOut<NC>& V) {
+    return (V.any_disappearance ? 0x4000 : 0) | (V.has_candidate ? 0x8000 : 0);
+}
+
 // project_pt_vle_phase_set (pt_vle_phase_set.hpp:204-258)
 template <int NC>
 __device__ void publish_vle(const FlashOut& o, int64_t N, int64_t c, const VleOut<NC>& V, int max3_code,
-                            const Counters& cnt) {
+                            const Counters& cnt, int extra_flags = 0) {
     PhaseView<NC> ph[3];
+    const int fl = split_flags<NC>(V) | extra_flags;
     switch (V.status) {
     case V_SINGLE:
-        if (!V.init.has_ref) { write_cell<NC>(o, N, c, 2, max3_code, V.status, 0, ph, cnt, 0); return; }
+        if (!V.init.has_ref) { write_cell<NC>(o, N, c, 2, max3_code, V.status, 0, ph, cnt, fl); return; }
         ph[0] = {1.0, V.init.feed, &V.init.ref, __builtin_nan("")};
-        write_cell<NC>(o, N, c, 0, max3_code, V.status, 1, ph, cnt, 0);
+        write_cell<NC>(o, N, c, 0, max3_code, V.status, 1, ph, cnt, fl);
         return;
     case V_TWO:
-        write_cell<NC>(o, N, c, 0, max3_code, V.status, pair_view<NC>(V.cand, ph), ph, cnt, 0);
+        write_cell<NC>(o, N, c, 0, max3_code, V.status, pair_view<NC>(V.cand, ph), ph, cnt, fl);
         return;
     case V_UNSTABLE:
-        write_cell<NC>(o, N, c, 1, max3_code, V.status, V.has_candidate ? pair_view<NC>(V.cand, ph) : 0, ph, cnt, 0);
+        write_cell<NC>(o, N, c, 1, max3_code, V.status, V.has_candidate ? pair_view<NC>(V.cand, ph) : 0, ph, cnt, fl);
         return;
     default:
-        write_cell<NC>(o, N, c, 2, max3_code, V.status, V.has_candidate ? pair_view<NC>(V.cand, ph) : 0, ph, cnt, 0);
+        write_cell<NC>(o, N, c, 2, max3_code, V.status, V.has_candidate ? pair_view<NC>(V.cand, ph) : 0, ph, cnt, fl);
         return;
     }
 }
@@ -100,29 +109,41 @@ template <int NC>
 __device__ void publish_max3(const FlashOut& o, int64_t N, int64_t c, const Max3Out<NC>& M, const Counters& cnt) {
     PhaseView<NC> ph[3];
     const int split = M.base.status;
+    const int max3f = (M.tail.attempts > 0 ? 0x10000 : 0) | (M.tail.start_consumed ? 0x20000 : 0);
+    const int fl = split_flags<NC>(M.base) | max3f;
     switch (M.status) {
     case M_SINGLE:
-        publish_vle<NC>(o, N, c, M.base, M.status, cnt);
+        publish_vle<NC>(o, N, c, M.base, M.status, cnt, max3f);
         return;
     case M_TWO:
         write_cell<NC>(o, N, c, 0, M.status, split, pair_view<NC>(M.tail.use_neighbor ? M.tail.neighbor : M.base.cand, ph),
-                       ph, cnt, 0);
+                       ph, cnt, fl);
         return;
     case M_THREE:
 #pragma unroll
         for (int k = 0; k < 3; ++k) { ph[k] = {M.tail.three.frac[k], M.tail.three.comp[k], &M.tail.three.act[k], M.tail.three.z[k]}; }
-        write_cell<NC>(o, N, c, 0, M.status, split, 3, ph, cnt, 0);
+        write_cell<NC>(o, N, c, 0, M.status, split, 3, ph, cnt, fl);
         return;
     case M_HIGHER:
         write_cell<NC>(o, N, c, 1, M.status, split, M.base.has_candidate ? pair_view<NC>(M.base.cand, ph) : 0, ph, cnt,
-                       0);
+                       fl);
         return;
     default:
-        write_cell<NC>(o, N, c, 2, M.status, split, 0, ph, cnt, 0);
+        write_cell<NC>(o, N, c, 2, M.status, split, 0, ph, cnt, fl);
         return;
     }
 }
 
+// One active lane per wave. The serial searches run through non-inlined calls
+// (solve_vle, solve_max3, max3_tail, stability, iterate_three) under divergent control
+// flow. With several cells in one wave, the three-phase outcome of a cell on the
+// sour-gas sweep came to depend on the other cells of its wave: about 5% of the cells
+// that reach the three-phase stage changed outcome with the batch order or with the
+// buffers of a previous call, while the same cells flashed one per call (one active lane)
+// matched the oracle on every cell, and the host build of the same code is bit-identical
+// to the oracle and clean under MemorySanitizer, AddressSanitizer and UBSan. So each
+// wave runs one cell at a time; lane 0 does the work after the workgroup has staged the
+// mixture, and the grid supplies the parallelism across cells.
 template <int NC>
 __global__ __launch_bounds__(64) void flash_vle_kernel(const double* __restrict__ mixc, int64_t ncell,
                                                        const double* __restrict__ p, const double* __restrict__ t,
@@ -130,8 +151,8 @@ __global__ __launch_bounds__(64) void flash_vle_kernel(const double* __restrict_
     __shared__ Mix<NC> smix;
     load_mix<NC>(mixc, smix);
     __syncthreads();
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (threadIdx.x != 0) { return; }
+    for (int64_t c = blockIdx.x; c < ncell; c += gridDim.x) {
         double feed[NC];
 #pragma unroll
         for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
@@ -155,8 +176,8 @@ __global__ __launch_bounds__(64) void flash_max3_kernel(const double* __restrict
     __shared__ Mix<NC> smix;
     load_mix<NC>(mixc, smix);
     __syncthreads();
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (threadIdx.x != 0) { return; }
+    for (int64_t c = blockIdx.x; c < ncell; c += gridDim.x) {
         double feed[NC];
 #pragma unroll
         for (int i = 0; i < NC; ++i) { feed[i] = z[i * ncell + c]; }
@@ -201,9 +222,12 @@ __global__ __launch_bounds__(64) void phase_kernel(const double* __restrict__ mi
     root_status[c] = e == REJECTED ? -1 : ctx.rs.status;
     if (e != ISSUE_NONE) { return; }
     root_count[c] = ctx.rs.count;
-    for (int k = 0; k < ctx.rs.count; ++k) {
-        zroot[k * ncell + c] = ctx.rs.r[k].z;
-        root_flags[k * ncell + c] = (ctx.rs.r[k].slope_sign + 1) | (ctx.rs.r[k].dvalid ? 0x10 : 0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= ctx.rs.count) { break; }
+        const Root r = root_at(ctx.rs, k);
+        zroot[k * ncell + c] = r.z;
+        root_flags[k * ncell + c] = (r.slope_sign + 1) | (r.dvalid ? 0x10 : 0);
         double lp[NC];
         if (phase_lnphi<NC>(smix, p[c], t[c], ctx, k, lp, nullptr) == ISSUE_NONE) {
 #pragma unroll
@@ -248,13 +272,13 @@ namespace mpmc::dev {
 
 constexpr int kBlock = 64;
 
-// Grid of a one-lane-per-cell kernel with `lane_bytes` of private memory per lane: every
-// cell's block, capped so that one dispatch's scratch (lane bytes x 64 x resident waves,
+// Grid of a one-lane-per-cell kernel with `lane_bytes` of private memory per lane (one_lane:
+// one active lane per wave, so one cell per workgroup): every cell's block, capped so that one dispatch's scratch (lane bytes x 64 x resident waves,
 // backed on the dispatch's hardware queue) stays within kDispatchScratch; the kernels
 // stride over the cells. At least 64 workgroups.
 constexpr size_t kDispatchScratch = size_t(256) << 20;
-inline unsigned capped_cell_blocks(int64_t ncell, size_t lane_bytes) {
-    const int64_t want = (ncell + kBlock - 1) / kBlock;
+inline unsigned capped_cell_blocks(int64_t ncell, size_t lane_bytes, bool one_lane = false) {
+    const int64_t want = one_lane ? ncell : (ncell + kBlock - 1) / kBlock;
     const int64_t cap = lane_bytes == 0 ? want
                                         : std::max<int64_t>(64, static_cast<int64_t>(kDispatchScratch / (lane_bytes * 64)));
     return static_cast<unsigned>(std::max<int64_t>(1, std::min(want, cap)));
@@ -273,10 +297,10 @@ hipError_t launch_flash(const double* constants, int64_t ncell, const double* p,
     static const size_t vle_lane = kernel_lane_scratch(flash_vle_kernel<NC>);
     static const size_t max3_lane = kernel_lane_scratch(flash_max3_kernel<NC>);
     if (max_phases == 2) {
-        const dim3 grid(capped_cell_blocks(ncell, vle_lane));
+        const dim3 grid(capped_cell_blocks(ncell, vle_lane, true));
         hipLaunchKernelGGL(flash_vle_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, z, fug_tol, out);
     } else {
-        const dim3 grid(capped_cell_blocks(ncell, max3_lane));
+        const dim3 grid(capped_cell_blocks(ncell, max3_lane, true));
         hipLaunchKernelGGL(flash_max3_kernel<NC>, grid, dim3(kBlock), 0, s, constants, ncell, p, t, z, fug_tol, out);
     }
     return hipGetLastError();

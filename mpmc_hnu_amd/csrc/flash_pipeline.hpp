@@ -53,7 +53,7 @@ namespace mpmc::dev {
 #endif
 
 constexpr int kAttempts = 16;   // PtSplitOptions::max_split_attempts (pt_split.hpp:49)
-constexpr int kMaxHints = 2;    // warm-start compositions per cell (extra initial stability starts)
+constexpr int kMaxHints = 3;    // warm-start compositions per cell (extra initial stability starts)
 constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
 // Outcomes of a reduced-precision trial or split attempt that FP64 must decide: an
@@ -132,6 +132,10 @@ struct Pipe {
     // before the selected pair's two compositions (pt_split.hpp:508-510)
     const double* fhints;
     int nfhint;
+    // one caller-supplied three-phase start per cell: compositions tpz[(ph*NC + i)*n + c] and
+    // phase-fraction seed tpf[k*n + c] of phases 1, 2 (NaN in tpf[c]: none); nullptr: none
+    const double* tpz;
+    const double* tpf;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -270,6 +274,13 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
             if (fin) { ++nfh; } else { ++nh; }
         }
         info |= (nh << 24) | (nfh << 26);
+        if (P.tpz && !__builtin_isnan(P.tpf[c])) {
+            double st3[3 * NC];
+            const double sf[2] = {P.tpf[c], P.tpf[n + c]};
+#pragma unroll
+            for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[k * n + c]; }
+            check_three_phase_start<NC>(z, st3, sf, rejected);
+        }
         Activity<NC> ref;
         if (!rejected && !lost) {
             ++prop;
@@ -834,10 +845,12 @@ __global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
         int best = -1;
         double best_g = 0;
         bool fallback = false;
+        bool disappearance = false;
         for (int j = 0; j < na; ++j) {
             if (cum >= 20000) { fallback = true; break; }
             const uint32_t ev = P.a_ev[j * n + u];
             if (cum + ev > 20000) { fallback = true; }
+            disappearance = disappearance || (P.a_st[j * n + u] & 0xFF) == A_DISAPPEARANCE;
             if ((P.a_st[j * n + u] & 0xFF) == A_CONVERGED) {
                 const double g = P.a_s[(j * 6 + 0) * n + u];
                 if (best < 0 || g < best_g) { best = j; best_g = g; }
@@ -846,6 +859,8 @@ __global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
         }
         if (fallback) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
         P.u_sel[u] = best;
+        // bit 0: the pair's Gibbs energy exceeds the feed's; bit 1: a disappearance attempt
+        P.u_dgbad[u] = disappearance ? 2 : 0;
         if (best < 0) { continue; }
         double feed_g = 0, corr = 0, mag = 1;
 #pragma unroll
@@ -863,7 +878,7 @@ __global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
         if (!finite(dg) || !finite(guard) || !finite(ftol)) { continue; }
         P.u_v[u] = -1;
         P.u_ftol[u] = ftol;
-        P.u_dgbad[u] = dg > guard;
+        P.u_dgbad[u] |= dg > guard ? 1 : 0;
     }
 }
 
@@ -895,19 +910,24 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
             continue;
         }
         P.u_v[u] = fin == S_UNSTABLE ? V_UNSTABLE
-                 : (P.u_dgbad[u] ? V_INDETERMINATE : (fin == S_NO_INSTABILITY ? V_TWO : V_INDETERMINATE));
+                 : ((P.u_dgbad[u] & 1) ? V_INDETERMINATE : (fin == S_NO_INSTABILITY ? V_TWO : V_INDETERMINATE));
     }
 }
 
 // ------------------------------------------------------------------ max3 tail
 template <int NC, bool kN>
+// One active lane per wave: the three-phase tail runs the serial searches through
+// non-inlined calls, whose outcome came to depend on the other cells of the wave when
+// several ran together (flash_kernels.hpp, serial kernels). Lane 0 takes one unstable cell
+// at a time.
 __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __restrict__ mixc) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
+    if (threadIdx.x != 0) { return; }
     const int64_t n = P.n;
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
     bool exhausted = false;
     while (!exhausted) {
-        const int64_t u = fetch_task(&P.g[G_TASK_MAX3], true);
+        const int64_t u = static_cast<int64_t>(atomicAdd(&P.g[G_TASK_MAX3], 1ull));
         if (u >= nu) { exhausted = true; continue; }
         if (P.u_v[u] != V_UNSTABLE || P.max_phases != 3) { continue; }
         const int64_t c = P.u_cell[u];
@@ -927,6 +947,14 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
         Counters cnt{0, 0, 0, 0};
         bool rejected = false;
         Max3Tail<NC> M;
+        const bool has_start = P.tpz && !__builtin_isnan(P.tpf[c]);
+        double st3[3 * NC], sf[2];
+        if (has_start) {
+#pragma unroll
+            for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[k * n + c]; }
+            sf[0] = P.tpf[c];
+            sf[1] = P.tpf[n + c];
+        }
         max3_tail<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], z, x, y, beta, nwit,
                       [&](int i, double* w, int& branch) {
                           int seen = -1, k = 0;
@@ -937,13 +965,14 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
                           for (int q = 0; q < NC; ++q) { w[q] = P.fw[(k * NC + q) * n + u]; }
                           branch = (P.fst[k * n + u] >> 8) & 0xFF;
                       },
-                      P.fug_tol, M, cnt, rejected);
+                      P.fug_tol, M, cnt, rejected, has_start ? st3 : nullptr, sf);
         atomicAdd(&P.cnt[0 * n + c], cnt.prop);
         atomicAdd(&P.cnt[1 * n + c], cnt.tpd);
         atomicAdd(&P.cnt[2 * n + c], cnt.rr);
         atomicAdd(&P.cnt[3 * n + c], cnt.rr3);
         if (rejected) { atomicOr(&P.flags[c], CF_REJECTED); continue; }
-        P.m_st[u] = M.status | (M.use_neighbor ? 0x100 : 0);
+        P.m_st[u] = M.status | (M.use_neighbor ? 0x100 : 0) | (M.attempts > 0 ? 0x200 : 0) |
+                    (M.start_consumed ? 0x400 : 0);
         if (M.status == M_THREE) {
 #pragma unroll
             for (int ph = 0; ph < 3; ++ph) {
@@ -979,8 +1008,8 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
 template <int NC>
 __device__ void put_record(const FlashOut& o, int64_t n, int64_t c, int set, int max3, int split, int np,
                            const double* beta, const double* zf, const int32_t* br, const double* x,
-                           const double* lp) {
-    o.status[c] = set | (max3 << 4) | (split << 8);
+                           const double* lp, int flags = 0) {
+    o.status[c] = set | (max3 << 4) | (split << 8) | flags;
     o.phase_count[c] = np;
     const double nan = __builtin_nan("");
 #pragma unroll
@@ -1039,6 +1068,10 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
         const int64_t u = P.cell_u[c];
         const int v = P.u_v[u];
         const int sel = P.u_sel[u];
+        // MPMC_FLAG_SPLIT_DISAPPEARANCE, MPMC_FLAG_SPLIT_CANDIDATE, MPMC_FLAG_MAX3_ATTEMPTED
+        const int fl = ((P.u_dgbad[u] & 2) ? 0x4000 : 0) | (sel >= 0 ? 0x8000 : 0) |
+                       (max3 && v == V_UNSTABLE && (P.m_st[u] & 0x200) ? 0x10000 : 0) |
+                       (max3 && v == V_UNSTABLE && (P.m_st[u] & 0x400) ? 0x20000 : 0);
         auto load_pair = [&]() {
             const int32_t st = P.a_st[sel * n + u];
             beta[1] = P.a_s[(sel * 6 + 1) * n + u];
@@ -1060,15 +1093,15 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
         if (!max3) {
             const int set = v == V_TWO ? 0 : (v == V_UNSTABLE ? 1 : 2);
             const int np = sel >= 0 ? load_pair() : 0;
-            put_record<NC>(P.out, n, c, set, 0xF, v, np, beta, zf, br, x, lp);
+            put_record<NC>(P.out, n, c, set, 0xF, v, np, beta, zf, br, x, lp, fl);
             continue;
         }
         if (v == V_TWO) {
-            put_record<NC>(P.out, n, c, 0, M_TWO, v, load_pair(), beta, zf, br, x, lp);
+            put_record<NC>(P.out, n, c, 0, M_TWO, v, load_pair(), beta, zf, br, x, lp, fl);
             continue;
         }
         if (v != V_UNSTABLE) {
-            put_record<NC>(P.out, n, c, 2, M_INDETERMINATE, v, 0, beta, zf, br, x, lp);
+            put_record<NC>(P.out, n, c, 2, M_INDETERMINATE, v, 0, beta, zf, br, x, lp, fl);
             continue;
         }
         const int32_t ms = P.m_st[u];
@@ -1085,11 +1118,11 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
                     lp[ph * NC + i] = P.m_v[((3 + ph) * NC + i) * n + u];
                 }
             }
-            put_record<NC>(P.out, n, c, 0, mstat, v, np, beta, zf, br, x, lp);
+            put_record<NC>(P.out, n, c, 0, mstat, v, np, beta, zf, br, x, lp, fl);
         } else if (mstat == M_HIGHER) {
-            put_record<NC>(P.out, n, c, 1, M_HIGHER, v, load_pair(), beta, zf, br, x, lp);
+            put_record<NC>(P.out, n, c, 1, M_HIGHER, v, load_pair(), beta, zf, br, x, lp, fl);
         } else {
-            put_record<NC>(P.out, n, c, 2, mstat, v, 0, beta, zf, br, x, lp);
+            put_record<NC>(P.out, n, c, 2, mstat, v, 0, beta, zf, br, x, lp, fl);
         }
     }
 }
@@ -1098,11 +1131,12 @@ __global__ __launch_bounds__(64) void k_publish(Pipe<NC> P) {
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __restrict__ mixc) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
+    if (threadIdx.x != 0) { return; }   // one active lane per wave, as in k_max3_tail
     const int64_t n = P.n;
     const int64_t nf = static_cast<int64_t>(P.g[G_NFALLBACK]);
     bool exhausted = false;
     while (!exhausted) {
-        const int64_t i = fetch_task(&P.g[G_TASK_FALLBACK], true);
+        const int64_t i = static_cast<int64_t>(atomicAdd(&P.g[G_TASK_FALLBACK], 1ull));
         if (i >= nf) { exhausted = true; continue; }
         const int64_t c = P.fb_list[i];
         double feed[NC];
@@ -1134,8 +1168,16 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
             else { publish_vle<NC>(P.out, n, c, V, 0xF, cnt); }
         } else {
             Max3Out<NC> M;
+            const bool has_start = P.tpz && !__builtin_isnan(P.tpf[c]);
+            double st3[3 * NC], sf[2];
+            if (has_start) {
+#pragma unroll
+                for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[k * n + c]; }
+                sf[0] = P.tpf[c];
+                sf[1] = P.tpf[n + c];
+            }
             solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected, hs, nh, fs,
-                               nfh);
+                               nfh, has_start ? st3 : nullptr, sf);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_max3<NC>(P.out, n, c, M, cnt); }
         }
@@ -1239,8 +1281,8 @@ PipeGrids pipe_grids(size_t lds) {
     // and serve few cells; their grids are capped so that one dispatch reserves at most
     // ~100 MB of scratch (8192 and 1024 lanes), which keeps several launch streams of
     // such dispatches within the runtime's scratch limits.
-    g.tail = std::min(persistent_blocks(k_max3_tail<NC, false>, lds), 128u);
-    g.fb = std::min(persistent_blocks(k_fallback<NC, false>, lds), 16u);
+    g.tail = std::min(persistent_blocks(k_max3_tail<NC, false>, lds), 256u);
+    g.fb = std::min(persistent_blocks(k_fallback<NC, false>, lds), 64u);
     g.tpd32 = persistent_blocks(k_tpd32<NC>, lds);
     g.split32 = persistent_blocks(k_split32<NC>, lds);
     g.prep = scratch_blocks(k_prep<NC, false>);
@@ -1357,14 +1399,18 @@ template <int NC>
 hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, void* workspace,
                            size_t workspace_bytes, int64_t n, const double* p, const double* t, const double* z,
                            int max_phases, double fug_tol, int precision, const FlashOut& out, hipStream_t s,
-                           int nhint, const double* hints, int nfhint, const double* fhints) {
+                           int nhint, const double* hints, int nfhint, const double* fhints, const double* tpz,
+                           const double* tpf) {
     Pipe<NC> P{};
     if (nhint < 0 || nhint > kMaxHints || (nhint > 0 && !hints)) { return hipErrorInvalidValue; }
     if (nfhint < 0 || nfhint > kMaxHints || (nfhint > 0 && !fhints)) { return hipErrorInvalidValue; }
+    if ((tpz == nullptr) != (tpf == nullptr) || (tpz && max_phases != 3)) { return hipErrorInvalidValue; }
     P.nhint = nhint;
     P.hints = hints;
     P.nfhint = nfhint;
     P.fhints = fhints;
+    P.tpz = tpz;
+    P.tpf = tpf;
     const size_t need = pipe_carve<NC>(P, static_cast<char*>(workspace), n);
     if (!workspace || need > workspace_bytes) { return hipErrorInvalidValue; }
     if (nmix < 1 || (nmix > 1 && !mix_of)) { return hipErrorInvalidValue; }
@@ -1387,12 +1433,12 @@ hipError_t launch_pipeline(const double* mixc, int nmix, const int32_t* mix_of, 
     extern template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,     \
                                                   const double*, const double*, const double*, int, double, int,  \
                                                   const FlashOut&, hipStream_t, int, const double*, int,    \
-                                                  const double*);
+                                                  const double*, const double*, const double*);
 #define MPMC_INSTANTIATE_PIPELINE(N)                                                                              \
     template size_t pipeline_bytes<N>(int64_t);                                                                   \
     template hipError_t launch_pipeline<N>(const double*, int, const int32_t*, void*, size_t, int64_t,            \
                                            const double*, const double*, const double*, int, double, int,         \
                                            const FlashOut&, hipStream_t, int, const double*, int,           \
-                                           const double*);
+                                           const double*, const double*, const double*);
 
 } // namespace mpmc::dev

@@ -754,6 +754,9 @@ __global__ __launch_bounds__(64) void k_polish(Pipe<NC> P, const double* __restr
         P.u_dgbad[u] = 0;
         if (P.flags[c] & CF_REJECTED) { continue; }
         const int na = 2 * P.u_nwit[u];
+        bool disappearance = false;
+        for (int j = 0; j < na; ++j) { disappearance = disappearance || (P.a_st[j * n + u] & 0xFF) == A_DISAPPEARANCE; }
+        P.u_dgbad[u] = disappearance ? 2 : 0;
         double z[NC];
         bool support = true;
 #pragma unroll
@@ -829,7 +832,7 @@ __global__ __launch_bounds__(64) void k_polish(Pipe<NC> P, const double* __restr
         if (!finite(dg) || !finite(guard) || !finite(ftol)) { continue; }
         P.u_v[u] = -1;
         P.u_ftol[u] = ftol;
-        P.u_dgbad[u] = dg > guard;
+        P.u_dgbad[u] |= dg > guard ? 1 : 0;
     }
 }
 

@@ -50,11 +50,36 @@ struct Root {
     bool dvalid;
     int slope_sign;
 };
+// The roots are three named members, not an array: a per-lane array accessed with a
+// run-time index lives in private (scratch) memory, and the compiler turns a chain of
+// selects over array elements back into such an index.
 struct RootSet {
-    Root r[3];
+    Root r0, r1, r2;
     int count;
     int status;
 };
+
+// Root k of a set, field by field through value selects (a select between member
+// addresses, or a branch per member store, is merged back into an indexed access).
+__device__ __forceinline__ Root root_at(const RootSet& rs, int k) {
+    Root v;
+    v.z = k == 0 ? rs.r0.z : (k == 1 ? rs.r1.z : rs.r2.z);
+    v.x = k == 0 ? rs.r0.x : (k == 1 ? rs.r1.x : rs.r2.x);
+    v.dvalid = k == 0 ? rs.r0.dvalid : (k == 1 ? rs.r1.dvalid : rs.r2.dvalid);
+    v.slope_sign = k == 0 ? rs.r0.slope_sign : (k == 1 ? rs.r1.slope_sign : rs.r2.slope_sign);
+    return v;
+}
+__device__ __forceinline__ void put_root_member(Root& dst, bool take, const Root& v) {
+    dst.z = take ? v.z : dst.z;
+    dst.x = take ? v.x : dst.x;
+    dst.dvalid = take ? v.dvalid : dst.dvalid;
+    dst.slope_sign = take ? v.slope_sign : dst.slope_sign;
+}
+__device__ __forceinline__ void put_root(RootSet& rs, int k, const Root& v) {
+    put_root_member(rs.r0, k == 0, v);
+    put_root_member(rs.r1, k == 1, v);
+    put_root_member(rs.r2, k == 2, v);
+}
 
 __device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : (b > a ? b : a); }
@@ -90,6 +115,9 @@ __device__ __forceinline__ double poly_scale(double y, const Cubic& c) {
 template <bool kNewton = false>
 MPMC_HOT RootSet pr76_roots(double a, double b) {
     RootSet rs;
+    rs.r0 = Root{0.0, 0.0, false, 0};
+    rs.r1 = rs.r0;
+    rs.r2 = rs.r0;
     rs.count = 0;
     rs.status = RS_UNREPRESENTABLE;
     const double scale = smax(smax(1.0, b), __builtin_sqrt(__builtin_fabs(a)));
@@ -103,13 +131,15 @@ MPMC_HOT RootSet pr76_roots(double a, double b) {
     }
     if (!finite(c.c2) || !finite(c.c1) || !finite(c.c0) || !(c.c0 < 0)) { return rs; }
     const double upper = 1.0 + smax(smax(__builtin_fabs(c.c2), __builtin_fabs(c.c1)), __builtin_fabs(c.c0));
-    double ends[4];
-    int ne = 1;
-    ends[0] = 0.0;
+    // Interval ends 0 < e1 [< e2 [< e3]]: the stationary points inside (0, upper), then
+    // upper; held in scalars (no run-time indexed array).
+    bool use0 = false, use1 = false;
+    double s0 = 0.0, s1 = 0.0;
     const double disc = __builtin_fma(c.c2, c.c2, -3.0 * c.c1);
     if (disc > 0) {
         const double q = -c.c2 - __builtin_copysign(__builtin_sqrt(disc), c.c2);
-        double s0 = q / 3.0, s1 = c.c1 / q;
+        s0 = q / 3.0;
+        s1 = c.c1 / q;
         if (s1 < s0) { const double tmp = s0; s0 = s1; s1 = tmp; }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -119,13 +149,17 @@ MPMC_HOT RootSet pr76_roots(double a, double b) {
                 const double mag = poly_scale(pt, c);
                 if (!finite(v) || !(mag > 0)) { return rs; }
                 if (__builtin_fabs(v) <= 64.0 * kEps * mag) { rs.status = RS_NEAR_MULTIPLE; return rs; }
-                ends[ne++] = pt;
+                if (k == 0) { use0 = true; } else { use1 = true; }
             }
         }
     }
-    ends[ne++] = upper;
+    const int ne = 2 + (use0 ? 1 : 0) + (use1 ? 1 : 0);
+    const double e1 = use0 ? s0 : (use1 ? s1 : upper);
+    const double e2 = use0 && use1 ? s1 : upper;
+    double last_z = 0.0;
     for (int iv = 1; iv < ne; ++iv) {
-        double left = ends[iv - 1], right = ends[iv];
+        double left = iv == 1 ? 0.0 : (iv == 2 ? e1 : e2);
+        double right = iv == 1 ? e1 : (iv == 2 ? e2 : upper);
         double fl = poly(left, c);
         const double fr = poly(right, c);
         if (!finite(fl) || !finite(fr)) { rs.count = 0; rs.status = RS_UNREPRESENTABLE; return rs; }
@@ -133,14 +167,24 @@ MPMC_HOT RootSet pr76_roots(double a, double b) {
         double y = 0;
         bool conv = false;
         if (!kNewton) {
+            // left moves only to points whose value has the sign bit of fl, so that bit
+            // is fixed for the whole bisection.
+            // The stopping tests are combined without short-circuit (each || is a branch
+            // on the wave's execution mask) and the bracket is updated by selects: the
+            // same arithmetic and decisions in about two thirds of the time.
+            const bool lneg = __builtin_signbit(fl);
             for (int it = 0; it < 2048; ++it) {
                 y = (left + right) / 2;
                 const double fy = poly(y, c);
-                if (fy == 0 || y == left || y == right || right - left <= 4.0 * kEps * smax(y, DBL_MIN)) {
+                const bool stop = (fy == 0) | (y == left) | (y == right) |
+                                  (right - left <= 4.0 * kEps * smax(y, DBL_MIN));
+                if (stop) {
                     conv = true;
                     break;
                 }
-                if (__builtin_signbit(fy) == __builtin_signbit(fl)) { left = y; fl = fy; } else { right = y; }
+                const bool to_left = static_cast<bool>(__builtin_signbit(fy)) == lneg;
+                left = to_left ? y : left;
+                right = to_left ? right : y;
             }
         } else {
             y = (left + right) / 2;
@@ -175,14 +219,17 @@ MPMC_HOT RootSet pr76_roots(double a, double b) {
             pres > 512.0 * kEps || rs.count == 3) {
             rs.count = 0; rs.status = RS_UNREPRESENTABLE; return rs;
         }
-        if (rs.count != 0 && !(z > rs.r[rs.count - 1].z)) {
+        if (rs.count != 0 && !(z > last_z)) {
             rs.count = 0; rs.status = RS_NEAR_MULTIPLE; return rs;
         }
         // 8*sqrt(eps) = 2^-23 exactly.
-        rs.r[rs.count].z = z;
-        rs.r[rs.count].x = x;
-        rs.r[rs.count].dvalid = sr > 8.0 * 1.4901161193847656e-08;
-        rs.r[rs.count].slope_sign = dh > 0 ? 1 : (dh < 0 ? -1 : 0);
+        Root nr;
+        nr.z = z;
+        nr.x = x;
+        nr.dvalid = sr > 8.0 * 1.4901161193847656e-08;
+        nr.slope_sign = dh > 0 ? 1 : (dh < 0 ? -1 : 0);
+        put_root(rs, rs.count, nr);
+        last_z = z;
         ++rs.count;
     }
     rs.status = rs.count > 0 ? RS_SUCCESS : RS_UNREPRESENTABLE;
@@ -283,13 +330,13 @@ __device__ __forceinline__ double log1p_over_x(double u) {
     return log1p(u) / u;
 }
 
-// ln(phi) of root k (pr76_phase.hpp:193-258, double arithmetic).
+// ln(phi) of the root with free volume fv = Z - B (pr76_phase.hpp:193-258, double
+// arithmetic).
 template <int NC>
-MPMC_HOT int phase_lnphi(const Mix<NC>& m, double p, double t, const PhaseCtx<NC>& ctx, int k,
-                                        double* out, double* zout) {
+MPMC_HOT int phase_lnphi_x(const Mix<NC>& m, double p, double t, const PhaseCtx<NC>& ctx, double fv,
+                                          double* out, double* zout) {
 #pragma unroll
     for (int i = 0; i < NC; ++i) { if (!finite(ctx.S[i])) { return ROOT_RANGE; } }
-    const double fv = ctx.rs.r[k].x;
     const double z = fv + ctx.B;
     const double s2 = 1.4142135623730951; // sqrt(2), correctly rounded
     const double den = fv + (2.0 - s2) * ctx.B;
@@ -317,6 +364,13 @@ MPMC_HOT int phase_lnphi(const Mix<NC>& m, double p, double t, const PhaseCtx<NC
     return ISSUE_NONE;
 }
 
+// ln(phi) of root k of the context's root set.
+template <int NC>
+MPMC_HOT int phase_lnphi(const Mix<NC>& m, double p, double t, const PhaseCtx<NC>& ctx, int k,
+                                        double* out, double* zout) {
+    return phase_lnphi_x<NC>(m, p, t, ctx, root_at(ctx.rs, k).x, out, zout);
+}
+
 // Activity of one candidate phase: ln(phi), provider branch (root index), smooth flag.
 template <int NC>
 struct Activity {
@@ -326,48 +380,74 @@ struct Activity {
 };
 
 // Stability provider: the least-Gibbs mechanically admissible root at w, with the
-// near-tie smoothness flag (pr76_stability.hpp:36-119).
+// near-tie smoothness flag (pr76_stability.hpp:36-119). The admissible roots (positive
+// slope) are compared in root order. A PR76 cubic with c0 < 0 has one or three roots in
+// y > 0 with slopes +, -, +, so at most two are admissible; they are held in two slots
+// of registers rather than a run-time indexed array. A third admissible root cannot
+// occur and is reported as a root-topology issue.
 template <int NC, bool kNewton = false>
 MPMC_HOT int stability_provider(const Mix<NC>& m, double p, double t, const double* w,
                                                Activity<NC>& out) {
     PhaseCtx<NC> ctx;
     const int pi = phase_prepare<NC, kNewton>(m, p, t, w, ctx);
     if (pi != ISSUE_NONE) { return pi; }
-    double cand[3][NC];
-    int cb[3];
-    int nc = 0;
-    for (int k = 0; k < ctx.rs.count; ++k) {
-        if (ctx.rs.r[k].slope_sign <= 0) { continue; }
-        const int e = phase_lnphi<NC>(m, p, t, ctx, k, cand[nc], nullptr);
+    const RootSet& rs = ctx.rs;
+    const bool a0 = rs.count > 0 && rs.r0.slope_sign > 0;
+    const bool a1 = rs.count > 1 && rs.r1.slope_sign > 0;
+    const bool a2 = rs.count > 2 && rs.r2.slope_sign > 0;
+    if (!a0 && !a1 && !a2) { return NO_ADMISSIBLE; }
+    if (a0 && a1 && a2) { return ROOT_TOPOLOGY; }
+    // first (f) and second (g) admissible root, in root order
+    const int f = a0 ? 0 : (a1 ? 1 : 2);
+    const bool two = (a0 && (a1 || a2)) || (a1 && a2);
+    const int g = a0 ? (a1 ? 1 : 2) : 2;
+    double c0[NC], c1[NC];
+    {
+        const double xf = f == 0 ? rs.r0.x : (f == 1 ? rs.r1.x : rs.r2.x);
+        const int e = phase_lnphi_x<NC>(m, p, t, ctx, xf, c0, nullptr);
         if (e != ISSUE_NONE) { return e; }
-        cb[nc++] = k;
     }
-    if (nc == 0) { return NO_ADMISSIBLE; }
-    int best = 0;
-    for (int k = 1; k < nc; ++k) {
+    if (two) {
+        const double xg = g == 1 ? rs.r1.x : rs.r2.x;
+        const int e = phase_lnphi_x<NC>(m, p, t, ctx, xg, c1, nullptr);
+        if (e != ISSUE_NONE) { return e; }
+    }
+    // selection: the second candidate replaces the first when its Gibbs difference is negative
+    bool second = false;
+    if (two) {
         double s = 0, c = 0;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) { kadd(w[i] * (cand[k][i] - cand[best][i]), s, c); }
+        for (int i = 0; i < NC; ++i) { kadd(w[i] * (c1[i] - c0[i]), s, c); }
         if (!finite(s)) { return NONFINITE; }
-        if (s < 0) { best = k; }
+        second = s < 0;
     }
-    if (!ctx.rs.r[cb[best]].dvalid) { return ILL_CONDITIONED; }
+    const int bidx = second ? g : f;
+    const bool dvalid = bidx == 0 ? rs.r0.dvalid : (bidx == 1 ? rs.r1.dvalid : rs.r2.dvalid);
+    if (!dvalid) { return ILL_CONDITIONED; }
     bool smooth = true;
-    for (int k = 0; k < nc; ++k) {
-        if (k == best) { continue; }
+    if (two) {
+        // the other candidate against the selected one
         double mag = 1;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) { mag += w[i] * (__builtin_fabs(cand[k][i]) + __builtin_fabs(cand[best][i])); }
+        for (int i = 0; i < NC; ++i) {
+            const double other = second ? c0[i] : c1[i];
+            const double best = second ? c1[i] : c0[i];
+            mag += w[i] * (__builtin_fabs(other) + __builtin_fabs(best));
+        }
         if (!finite(mag)) { return NONFINITE; }
         double s = 0, c = 0;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) { kadd(w[i] * (cand[k][i] - cand[best][i]), s, c); }
+        for (int i = 0; i < NC; ++i) {
+            const double other = second ? c0[i] : c1[i];
+            const double best = second ? c1[i] : c0[i];
+            kadd(w[i] * (other - best), s, c);
+        }
         if (!finite(s)) { return NONFINITE; }
         if (__builtin_fabs(s) <= 256.0 * kEps * mag) { smooth = false; }
     }
 #pragma unroll
-    for (int i = 0; i < NC; ++i) { out.lnphi[i] = cand[best][i]; }
-    out.branch = cb[best];
+    for (int i = 0; i < NC; ++i) { out.lnphi[i] = second ? c1[i] : c0[i]; }
+    out.branch = bidx;
     out.smooth = smooth;
     return ISSUE_NONE;
 }
@@ -381,15 +461,16 @@ MPMC_HOT int side_provider(const Mix<NC>& m, double p, double t, const double* w
     PhaseCtx<NC> ctx;
     const int pi = phase_prepare<NC, kNewton>(m, p, t, w, ctx);
     if (pi != ISSUE_NONE) { return pi; }
-    int sel = -1;
-    for (int k = 0; k < ctx.rs.count; ++k) {
-        if (ctx.rs.r[k].slope_sign <= 0) { continue; }
-        sel = k;
-        if (!upper) { break; }
-    }
-    if (sel < 0) { return NO_ADMISSIBLE; }
-    if (!ctx.rs.r[sel].dvalid) { return ILL_CONDITIONED; }
-    const int e = phase_lnphi<NC>(m, p, t, ctx, sel, out.lnphi, &zout);
+    const RootSet& rs = ctx.rs;
+    const bool a0 = rs.count > 0 && rs.r0.slope_sign > 0;
+    const bool a1 = rs.count > 1 && rs.r1.slope_sign > 0;
+    const bool a2 = rs.count > 2 && rs.r2.slope_sign > 0;
+    if (!a0 && !a1 && !a2) { return NO_ADMISSIBLE; }
+    const int sel = upper ? (a2 ? 2 : (a1 ? 1 : 0)) : (a0 ? 0 : (a1 ? 1 : 2));
+    const bool dvalid = sel == 0 ? rs.r0.dvalid : (sel == 1 ? rs.r1.dvalid : rs.r2.dvalid);
+    if (!dvalid) { return ILL_CONDITIONED; }
+    const double xs = sel == 0 ? rs.r0.x : (sel == 1 ? rs.r1.x : rs.r2.x);
+    const int e = phase_lnphi_x<NC>(m, p, t, ctx, xs, out.lnphi, &zout);
     if (e != ISSUE_NONE) { return e; }
     out.branch = sel;
     out.smooth = true;

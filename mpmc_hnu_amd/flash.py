@@ -105,7 +105,7 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
                 schedule: int = 0, workspace=None, hints=None, final_hints=None) -> dict:
     """Flash every cell (p[ncell] Pa, t[ncell] K, z[nc, ncell]); returns device tensors.
 
-    ``hints`` ([nhint, nc, ncell], nhint <= 2; NaN in component 0 marks an absent hint)
+    ``hints`` ([nhint, nc, ncell], nhint <= 3; NaN in component 0 marks an absent hint)
     are warm-start compositions appended to each cell's initial stability starts
     (mpmc_flash_batch_hinted); ``final_hints`` (same layout) are extra starts of the
     common-tangent review (mpmc_flash_batch_warm).
@@ -144,8 +144,8 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
                 hs.append(None)
                 continue
             h = _dev(h, torch.float64, device)
-            if h.dim() != 3 or h.shape[1:] != (mixture.nc, n) or h.shape[0] > 2:
-                raise ValueError("flash_batch: expected hints[nhint <= 2, nc, ncell]")
+            if h.dim() != 3 or h.shape[1:] != (mixture.nc, n) or h.shape[0] > 3:
+                raise ValueError("flash_batch: expected hints[nhint <= 3, nc, ncell]")
             hs.append(h)
         hints, final_hints = hs
         ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
@@ -160,8 +160,8 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
         return out
     if hints is not None:
         hints = _dev(hints, torch.float64, device)
-        if hints.dim() != 3 or hints.shape[1:] != (mixture.nc, n) or hints.shape[0] > 2:
-            raise ValueError("flash_batch: expected hints[nhint <= 2, nc, ncell]")
+        if hints.dim() != 3 or hints.shape[1:] != (mixture.nc, n) or hints.shape[0] > 3:
+            raise ValueError("flash_batch: expected hints[nhint <= 3, nc, ncell]")
         ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
             else (None, 0)
         rc = lib.mpmc_flash_batch_hinted(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),

@@ -149,6 +149,39 @@ def sour_gas_sweep(ncell: int = 2048, seed: int = 11):
     return mixture_sour_gas(), p, t, np.ascontiguousarray(z)
 
 
+def sour_gas_paths(npath: int = 64, npoint: int = 16, seed: int = 13):
+    """Ordered isobaric cooling paths through the acid-gas three-phase region: path j holds
+    p_j ~ U[1.5,3] MPa and T falling from 200 K to 160 K over npoint states (uneven steps),
+    feed perturbed by 10% noise. Returns (mixture, p [npoint, npath], t [npoint, npath], z)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pj = rng.uniform(1.5e6, 3.0e6, npath)
+    steps = rng.uniform(0.5, 1.5, (npoint, npath))
+    frac = np.cumsum(steps, axis=0) - steps[0]
+    frac = frac / np.maximum(frac[-1], 1e-300)
+    t = 200.0 - 40.0 * frac
+    p = np.broadcast_to(pj, (npoint, npath)).copy()
+    z = np.abs(SOUR_GAS_FEED[:, None] * (1.0 + 0.1 * rng.standard_normal((6, npath))))
+    z = z / z.sum(axis=0, keepdims=True)
+    resid = 1.0 - z.sum(axis=0)
+    j = np.argmax(z, axis=0)
+    z[j, np.arange(npath)] += resid
+    return mixture_sour_gas(), np.ascontiguousarray(p), np.ascontiguousarray(t), np.ascontiguousarray(z)
+
+
+def boundary_feeds(p, t, record: dict, eps: float, ncell: int = 512):
+    """Feeds just inside a two-phase boundary: for the first ``ncell`` cells of ``record``
+    (a flash record: status, phase_count, x) that were accepted with two phases, the feed
+    x_L + eps (x_V - x_L) at the same state. The split of such a feed converges to a
+    phase fraction of order eps, the reference's phase-disappearance endpoint for eps
+    below its 1e-10 minimum fraction. Returns (p, t, z)."""
+    two = np.nonzero(((record["status"] & 0xF) == 0) & (record["phase_count"] == 2))[0][:ncell]
+    x0 = record["x"][0][:, two]
+    y0 = record["x"][1][:, two]
+    z = x0 + eps * (y0 - x0)
+    z = z / z.sum(axis=0, keepdims=True)
+    return np.ascontiguousarray(p[two]), np.ascontiguousarray(t[two]), np.ascontiguousarray(z)
+
+
 # ---------------------------------------------------------------- two-phase assembly grid
 MILLIDARCY = 9.869233e-16  # m2
 GRAVITY = 9.80665          # m/s2, acting along -z

@@ -1181,6 +1181,8 @@ void iterate_three(const Mixture& m, double p, double t, const double* feed_in, 
 enum Max3Status { M_SINGLE = 0, M_TWO, M_THREE, M_BOUNDARY, M_HIGHER, M_INDETERMINATE };
 struct Max3Result {
     int status;
+    int attempts;     // three-phase attempts run (Pr76PtMax3Result::attempts.size())
+    bool start_consumed; // an attempt ran from the supplied start (pr76_pt_continuation.hpp:183-190)
     VleResult base;
     bool use_neighbor;
     VleResult neighbor;
@@ -1213,16 +1215,32 @@ bool log_ratio(int n, const double* num, const double* den, const double* z, dou
     return true;
 }
 
-// solve_pr76_pt_max3, pr76_three_phase.hpp:328-561 (no caller continuation hints).
+// solve_pr76_pt_max3, pr76_three_phase.hpp:328-561; start / start_frac: one caller-supplied
+// three-phase start (three_phase_starts), or nullptr.
 // initial / final: extra starts of the initial stability test and of the base split's
 // common-tangent review (solve_pr76_pt_max3 initial_starts / final_starts,
 // pr76_three_phase.hpp:328-341).
 void solve_max3(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
                 Max3Result& M, Counters& cnt, const double* const* initial = nullptr, int ninitial = 0,
-                const double* const* final_starts = nullptr, int nfinal = 0) {
+                const double* const* final_starts = nullptr, int nfinal = 0, const double* start = nullptr,
+                const double* start_frac = nullptr) {
     const int n = m.n;
     (void)check_composition(feed, n);
     M.use_neighbor = false;
+    M.attempts = 0;
+    M.start_consumed = false;
+    // pr76_pt_max3_check_starts (pr76_three_phase.hpp:211-242): the supplied start is validated
+    // before any solve; a violation is the reference's domain_error
+    if (start) {
+        if (!(std::isfinite(start_frac[0]) && std::isfinite(start_frac[1]) && start_frac[0] >= 0.0 &&
+              start_frac[1] >= 0.0 && start_frac[0] + start_frac[1] <= 1.0)) {
+            throw Rejected{};
+        }
+        for (int ph = 0; ph < 3; ++ph) {
+            (void)check_composition(start + ph * n, n);
+            for (int i = 0; i < n; ++i) { if ((feed[i] > 0) != (start[ph * n + i] > 0)) { throw Rejected{}; } }
+        }
+    }
     solve_vle(m, p, t, feed, so, initial, ninitial, M.base, cnt, final_starts, nfinal);
     switch (M.base.status) {
     case V_SINGLE: M.status = M_SINGLE; return;
@@ -1238,10 +1256,18 @@ void solve_max3(const Mixture& m, double p, double t, const double* feed, const 
     int automatic = 0;
     bool auto_limit = false;
     P3Options po;
-    // returns true when an attempt closes the solve
-    auto run_attempt = [&](const double* l1, const double* l2, const double* fr, const int* sides) -> bool {
-        if (automatic >= 16) { auto_limit = true; return false; }
-        ++automatic;
+    int continuation = 0;
+    bool cont_limit = false;
+    // returns true when an attempt closes the solve; supplied: the caller's start (its own
+    // attempt budget, pr76_three_phase.hpp:385-394)
+    auto run_attempt = [&](const double* l1, const double* l2, const double* fr, const int* sides,
+                           bool supplied = false) -> bool {
+        int& count = supplied ? continuation : automatic;
+        bool& limit = supplied ? cont_limit : auto_limit;
+        if (count >= 16) { limit = true; return false; }
+        ++count;
+        ++M.attempts;
+        if (supplied) { M.start_consumed = true; }
         P3Result E;
         iterate_three(m, p, t, z, l1, l2, fr, sides, po, E, cnt);
         if (E.status == P_DISAPPEARANCE) {
@@ -1270,6 +1296,17 @@ void solve_max3(const Mixture& m, double p, double t, const double* feed, const 
         else if (fs == S_NO_INSTABILITY) { M.status = M_THREE; M.three = c; return true; }
         return false;
     };
+    // the caller-supplied start first, over the eight root-side combinations, lower before
+    // upper, phase 0 outermost (pr76_three_phase.hpp:472-498)
+    if (start) {
+        double sl1[NCMAX], sl2[NCMAX];
+        if (log_ratio(n, start + n, start, z, sl1) && log_ratio(n, start + 2 * n, start, z, sl2)) {
+            for (int code = 0; code < 8 && !cont_limit; ++code) {
+                const int sd[3] = {code >> 2, (code >> 1) & 1, code & 1};
+                if (run_attempt(sl1, sl2, start_frac, sd, true)) { return; }
+            }
+        }
+    }
     double src1[NCMAX];
     if (log_ratio(n, pair.fr.y, pair.fr.x, z, src1)) {
         for (int k = 0; k < fin.ntrials && !auto_limit; ++k) {
@@ -1340,22 +1377,30 @@ int pair_phases(const VleResult& V, Phase* ph) {
     return 2;
 }
 
-void publish_vle(const Out& o, int n, int64_t N, int64_t c, const VleResult& V, int max3_code, const Counters& cnt) {
+// Status-word flags of the base split (mpmc_hnu_amd.h MPMC_FLAG_SPLIT_*): the facts of
+// PtSplitResult that the transition evidence reads (pr76_pt_flash_backend.hpp:42-95).
+int split_flags(const VleResult& V) {
+    return (V.any_disappearance ? 0x4000 : 0) | (V.has_candidate ? 0x8000 : 0);
+}
+
+void publish_vle(const Out& o, int n, int64_t N, int64_t c, const VleResult& V, int max3_code, const Counters& cnt,
+                 int extra_flags = 0) {
     Phase ph[3];
+    const int fl = split_flags(V) | extra_flags;
     switch (V.status) {
     case V_SINGLE:
-        if (!V.init.has_ref) { write_cell(o, n, N, c, 2, max3_code, V.status, 0, ph, cnt, 0); return; }
+        if (!V.init.has_ref) { write_cell(o, n, N, c, 2, max3_code, V.status, 0, ph, cnt, fl); return; }
         ph[0] = {1.0, V.init.feed, &V.init.ref, NaN};
-        write_cell(o, n, N, c, 0, max3_code, V.status, 1, ph, cnt, 0);
+        write_cell(o, n, N, c, 0, max3_code, V.status, 1, ph, cnt, fl);
         return;
     case V_TWO:
-        write_cell(o, n, N, c, 0, max3_code, V.status, pair_phases(V, ph), ph, cnt, 0);
+        write_cell(o, n, N, c, 0, max3_code, V.status, pair_phases(V, ph), ph, cnt, fl);
         return;
     case V_UNSTABLE:
-        write_cell(o, n, N, c, 1, max3_code, V.status, V.has_candidate ? pair_phases(V, ph) : 0, ph, cnt, 0);
+        write_cell(o, n, N, c, 1, max3_code, V.status, V.has_candidate ? pair_phases(V, ph) : 0, ph, cnt, fl);
         return;
     default:
-        write_cell(o, n, N, c, 2, max3_code, V.status, V.has_candidate ? pair_phases(V, ph) : 0, ph, cnt, 0);
+        write_cell(o, n, N, c, 2, max3_code, V.status, V.has_candidate ? pair_phases(V, ph) : 0, ph, cnt, fl);
         return;
     }
 }
@@ -1364,24 +1409,25 @@ void publish_vle(const Out& o, int n, int64_t N, int64_t c, const VleResult& V, 
 void publish_max3(const Out& o, int n, int64_t N, int64_t c, const Max3Result& M, const Counters& cnt) {
     Phase ph[3];
     const int split = M.base.status;
+    const int fl = split_flags(M.base) | (M.attempts > 0 ? 0x10000 : 0) | (M.start_consumed ? 0x20000 : 0);
     switch (M.status) {
     case M_SINGLE:
-        publish_vle(o, n, N, c, M.base, M.status, cnt);
+        publish_vle(o, n, N, c, M.base, M.status, cnt, (M.attempts > 0 ? 0x10000 : 0) | (M.start_consumed ? 0x20000 : 0));
         return;
     case M_TWO: {
         const VleResult& V = M.use_neighbor ? M.neighbor : M.base;
-        write_cell(o, n, N, c, 0, M.status, split, pair_phases(V, ph), ph, cnt, 0);
+        write_cell(o, n, N, c, 0, M.status, split, pair_phases(V, ph), ph, cnt, fl);
         return;
     }
     case M_THREE:
         for (int k = 0; k < 3; ++k) { ph[k] = {M.three.frac[k], M.three.comp[k], &M.three.act[k], M.three.z[k]}; }
-        write_cell(o, n, N, c, 0, M.status, split, 3, ph, cnt, 0);
+        write_cell(o, n, N, c, 0, M.status, split, 3, ph, cnt, fl);
         return;
     case M_HIGHER:
-        write_cell(o, n, N, c, 1, M.status, split, M.base.has_candidate ? pair_phases(M.base, ph) : 0, ph, cnt, 0);
+        write_cell(o, n, N, c, 1, M.status, split, M.base.has_candidate ? pair_phases(M.base, ph) : 0, ph, cnt, fl);
         return;
     default:
-        write_cell(o, n, N, c, 2, M.status, split, 0, ph, cnt, 0);
+        write_cell(o, n, N, c, 2, M.status, split, 0, ph, cnt, fl);
         return;
     }
 }
@@ -2315,8 +2361,8 @@ extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* p
                                        int threads, int32_t* status, int32_t* phase_count, double* beta,
                                        double* zfactor, int32_t* branch, double* x, double* ln_phi,
                                        uint32_t* counters) {
-    if (nhint < 0 || nhint > 2 || (nhint > 0 && !hints)) { return 1; }
-    if (nfinal < 0 || nfinal > 2 || (nfinal > 0 && !final_hints)) { return 1; }
+    if (nhint < 0 || nhint > 3 || (nhint > 0 && !hints)) { return 1; }
+    if (nfinal < 0 || nfinal > 3 || (nfinal > 0 && !final_hints)) { return 1; }
     if (nc < 1 || nc > NCMAX || ncell < 0 || (max_phases != 2 && max_phases != 3)) { return 1; }
     const Mixture m = make_mixture(nc, tc, pc, omega, kij);
     const Out o{status, phase_count, beta, zfactor, branch, x, ln_phi, counters};
@@ -2325,9 +2371,9 @@ extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* p
     parallel_cells(ncell, threads, [&](int64_t c) {
         double feed[NCMAX];
         for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
-        double hw[2][NCMAX], fw[2][NCMAX];
-        const double* hs[2];
-        const double* fs[2];
+        double hw[3][NCMAX], fw[3][NCMAX];
+        const double* hs[3];
+        const double* fs[3];
         int nh = 0, nf = 0;
         for (int h = 0; h < nhint; ++h) {
             if (std::isnan(hints[(h * nc + 0) * ncell + c])) { continue; }
@@ -2354,6 +2400,114 @@ extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* p
         } catch (const Rejected&) {
             Phase ph[3];
             write_cell(o, nc, ncell, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+        }
+    });
+    return 0;
+}
+
+// solve_pr76_pt_continuation (pr76_pt_continuation.hpp:194-310) for npath paths at once: path j
+// has feed z[i*npath + j] and ordered states p, t [npoint][npath]. Point k of every path is a
+// max3 flash whose initial and final stability starts are the previous accepted point's phase
+// compositions (deduplicated; carry bit 0) and whose three-phase stage first tries the
+// previous point's three-phase state (carry bit 1). Records are npoint consecutive flash
+// records of npath cells; info[k*npath + j] = incoming hint kind (0 none, else the previous
+// accepted phase count) | carried start count << 4 | carried three-phase start << 8. A path
+// whose states fail the path check, or one of whose points the flash rejects, is rejected at
+// every point (the reference throws for the whole path).
+extern "C" int cpuref_pt_continuation(int nc, const double* tc, const double* pc, const double* omega,
+                                      const double* kij, int64_t npath, int npoint, const double* p, const double* t,
+                                      const double* z, int carry, double fugacity_tolerance, int threads,
+                                      int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
+                                      int32_t* branch, double* x, double* ln_phi, uint32_t* counters, int32_t* info) {
+    if (nc < 1 || nc > NCMAX || npath < 0 || npoint < 1 || npoint > 4096) { return 1; }
+    const Mixture m = make_mixture(nc, tc, pc, omega, kij);
+    SplitOptions so;
+    so.fug_tol = fugacity_tolerance;
+    const int64_t N = npath;
+    auto out_at = [&](int k) {
+        return Out{status + k * N, phase_count + k * N, beta + k * 3 * N, zfactor + k * 3 * N, branch + k * 3 * N,
+                   x + k * 3 * nc * N, ln_phi + k * 3 * nc * N, counters ? counters + k * 4 * N : nullptr};
+    };
+    parallel_cells(npath, threads, [&](int64_t j) {
+        double feed[NCMAX];
+        for (int i = 0; i < nc; ++i) { feed[i] = z[i * npath + j]; }
+        bool path_ok = true;
+        for (int k = 0; k < npoint; ++k) {
+            const double pk = p[k * N + j], tk = t[k * N + j];
+            if (!(std::isfinite(pk) && pk > 0 && std::isfinite(tk) && tk > 0)) { path_ok = false; }
+        }
+        // the path's feed is normalised once here and again by each point's stability test
+        // (pr76_pt_continuation.hpp:218-219)
+        try {
+            const double sum = check_composition(feed, nc);
+            for (int i = 0; i < nc; ++i) { feed[i] /= sum; }
+        } catch (const Rejected&) { path_ok = false; }
+        int prev_count = 0;           // accepted phase count of the previous point (0: none)
+        double prev[3][NCMAX];
+        double prev_frac[2] = {0, 0};
+        bool prev_three = false;
+        for (int k = 0; k < npoint && path_ok; ++k) {
+            const Out o = out_at(k);
+            const double* starts[3];
+            int ns = 0;
+            if (carry & 1) {
+                for (int h = 0; h < prev_count; ++h) {
+                    bool dup = false;
+                    for (int e = 0; e < ns; ++e) {
+                        bool same = true;
+                        for (int i = 0; i < nc; ++i) { same = same && starts[e][i] == prev[h][i]; }
+                        dup = dup || same;
+                    }
+                    if (!dup) { starts[ns++] = prev[h]; }
+                }
+            }
+            const bool three = (carry & 2) && prev_three;
+            double st3[3 * NCMAX];
+            if (three) {
+                for (int ph = 0; ph < 3; ++ph) { for (int i = 0; i < nc; ++i) { st3[ph * nc + i] = prev[ph][i]; } }
+            }
+            info[k * N + j] = prev_count | (((carry & 1) ? prev_count : 0) << 4) | (three ? 0x100 : 0);
+            Counters cnt;
+            auto M = std::make_unique<Max3Result>();
+            try {
+                solve_max3(m, p[k * N + j], t[k * N + j], feed, so, *M, cnt, starts, ns, starts, ns,
+                           three ? st3 : nullptr, three ? prev_frac : nullptr);
+            } catch (const Rejected&) {
+                path_ok = false;
+                break;
+            }
+            publish_max3(o, nc, N, j, *M, cnt);
+            prev_three = false;
+            switch (M->status) {
+            case M_SINGLE:
+                prev_count = 1;
+                for (int i = 0; i < nc; ++i) { prev[0][i] = M->base.init.feed[i]; }
+                break;
+            case M_TWO: {
+                const SplitState& pr = (M->use_neighbor ? M->neighbor : M->base).cand.pt;
+                prev_count = 2;
+                for (int i = 0; i < nc; ++i) { prev[0][i] = pr.fr.x[i]; prev[1][i] = pr.fr.y[i]; }
+                break;
+            }
+            case M_THREE:
+                prev_count = 3;
+                prev_three = true;
+                for (int ph = 0; ph < 3; ++ph) { for (int i = 0; i < nc; ++i) { prev[ph][i] = M->three.comp[ph][i]; } }
+                prev_frac[0] = M->three.frac[1];
+                prev_frac[1] = M->three.frac[2];
+                break;
+            default:
+                prev_count = 0;
+                break;
+            }
+        }
+        if (!path_ok) {
+            for (int k = 0; k < npoint; ++k) {
+                Phase ph[3];
+                Counters cnt;
+                write_cell(out_at(k), nc, N, j, 2, 0xF, 0xF, 0, ph, cnt, 0x2000);
+                info[k * N + j] = 0;
+            }
         }
     });
     return 0;

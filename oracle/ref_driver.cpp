@@ -22,6 +22,7 @@
 //                     modules/flash/include/mpmc/flash/pr76_sensitivity.hpp:51-390)
 #include <mpmc/flash/pr76_max3_phase_set.hpp>
 #include <mpmc/flash/pr76_phase_set.hpp>
+#include <mpmc/flash/pr76_pt_continuation.hpp>
 #include <mpmc/flash/pr76_three_phase.hpp>
 #include <mpmc/flow/pr76_selected_phase_property_closure.hpp>
 #include <mpmc/flow/two_phase_accumulation.hpp>
@@ -29,6 +30,8 @@
 #include <mpmc/flow_discretization/two_phase_tpfa.hpp>
 #include <mpmc/physics/pr76_thermodynamic_closure.hpp>
 #include <mpmc/thermodynamics/pr76_phase.hpp>
+
+#include "ref_params.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -47,40 +50,9 @@ namespace th = mpmc::thermodynamics;
 namespace fl = mpmc::flash;
 namespace ph = mpmc::physics;
 
+using namespace mpmc_oracle;
+
 namespace {
-
-th::Provenance synthetic(const std::string& locator) {
-    return {th::SourceKind::synthetic_test, "mpmc-hnu-amd://oracle", "v1", locator,
-            "Synthetic mixture for parity vectors", "generated", "none"};
-}
-
-th::SourcedScalar datum(double value, th::Unit unit, const std::string& locator) {
-    return {value, unit, synthetic(locator), "SI", "identity"};
-}
-
-th::PrParameterSet make_parameters(int nc, const double* tc, const double* pc,
-                                   const double* omega, const double* kij) {
-    std::vector<th::Component> catalog;
-    th::PrParameterInput input;
-    input.model_id = std::string(th::pr76_profile);
-    input.dataset_id = "mpmc-hnu-amd-synthetic";
-    input.revision = "v1";
-    input.applicability = {std::nullopt, std::nullopt, synthetic("bounds")};
-    std::vector<std::string> order;
-    for (int i = 0; i < nc; ++i) {
-        const std::string id = "c" + std::to_string(i);
-        order.push_back(id);
-        catalog.push_back({id, id, th::ComponentKind::pure, synthetic(id), {}});
-        input.pure.push_back({id, datum(tc[i], th::Unit::kelvin, id + ".tc"),
-                              datum(pc[i], th::Unit::pascal, id + ".pc"),
-                              datum(omega[i], th::Unit::dimensionless, id + ".omega")});
-        for (int j = 0; j < i; ++j) {
-            input.binary.push_back({id, "c" + std::to_string(j),
-                                    datum(kij[i * nc + j], th::Unit::dimensionless, id + ".kij")});
-        }
-    }
-    return th::PrParameterSet::create(catalog, order, input, th::DataPolicy::allow_synthetic_tests);
-}
 
 struct Outputs {
     int32_t* status;
@@ -108,10 +80,20 @@ struct Counts {
     }
 };
 
+// Status-word flags (mpmc_hnu_amd.h MPMC_FLAG_SPLIT_*, MPMC_FLAG_MAX3_ATTEMPTED) read from
+// the reference's own result records.
+int split_flags(const fl::PtSplitResult& r) {
+    int flags = r.candidate() != nullptr ? 0x8000 : 0;
+    for (const auto& attempt : r.attempts) {
+        if (attempt.status == fl::PtSplitAttemptStatus::phase_disappearance) { flags |= 0x4000; }
+    }
+    return flags;
+}
+
 void publish(const fl::PtPhaseSetResult& set, int nc, int64_t ncell, int64_t c, int max3,
-             int split, const Counts& counts, const Outputs& out) {
+             int split, const Counts& counts, const Outputs& out, int flags = 0) {
     const double nan = std::numeric_limits<double>::quiet_NaN();
-    out.status[c] = static_cast<int32_t>(set.status) | (max3 << 4) | (split << 8);
+    out.status[c] = static_cast<int32_t>(set.status) | (max3 << 4) | (split << 8) | flags;
     const auto* phases = set.candidate_phase_set ? &set.candidate_phase_set->phases : nullptr;
     out.phase_count[c] = phases ? static_cast<int32_t>(phases->size()) : 0;
     for (int ph = 0; ph < 3; ++ph) {
@@ -134,6 +116,22 @@ void publish(const fl::PtPhaseSetResult& set, int nc, int64_t ncell, int64_t c, 
         out.counters[2 * ncell + c] = 0;
         out.counters[3 * ncell + c] = 0;
     }
+}
+
+// Publication of one max3 result (pr76_pt_flash_backend.hpp:204-213); flags adds status bits.
+void publish_max3(const fl::Pr76PtMax3Result& source, int nc, int64_t ncell, int64_t c, const Outputs& out,
+                  int flags) {
+    Counts counts;
+    counts.split(source.base.solution);
+    for (const auto& attempt : source.attempts) {
+        counts.prop += attempt.equilibrium.evaluations;
+        if (attempt.final_stability) { counts.stability(*attempt.final_stability); }
+        if (attempt.boundary_neighbor) { counts.split(attempt.boundary_neighbor->solution); }
+    }
+    const auto published = fl::project_pr76_pt_max3_phase_set(source);
+    publish(published.solution, nc, ncell, c, static_cast<int>(source.status),
+            static_cast<int>(source.base.solution.status), counts, out,
+            flags | split_flags(source.base.solution) | (source.attempts.empty() ? 0 : 0x10000));
 }
 
 void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evaluator, int nc,
@@ -165,20 +163,12 @@ void solve_cell(const th::Pr76Phase<double>& model, fl::Pr76VleEvaluator& evalua
             counts.split(legacy.solution);
             const auto published = fl::project_pr76_pt_phase_set(legacy);
             publish(published.solution, nc, ncell, c, 0xF, static_cast<int>(legacy.solution.status),
-                    counts, out);
+                    counts, out, split_flags(legacy.solution));
         } else {
             fl::Pr76PtMax3Options options;
             options.two_phase.iteration.fugacity_tolerance = fugacity_tolerance;
             const auto source = fl::solve_pr76_pt_max3(p[c], t[c], feed, evaluator, options, initial, finals);
-            counts.split(source.base.solution);
-            for (const auto& attempt : source.attempts) {
-                counts.prop += attempt.equilibrium.evaluations;
-                if (attempt.final_stability) { counts.stability(*attempt.final_stability); }
-                if (attempt.boundary_neighbor) { counts.split(attempt.boundary_neighbor->solution); }
-            }
-            const auto published = fl::project_pr76_pt_max3_phase_set(source);
-            publish(published.solution, nc, ncell, c, static_cast<int>(source.status),
-                    static_cast<int>(source.base.solution.status), counts, out);
+            publish_max3(source, nc, ncell, c, out, 0);
         }
     } catch (const std::exception&) {
         fl::PtPhaseSetResult empty;
@@ -340,8 +330,8 @@ int ref_flash_batch_warm(int nc, const double* tc, const double* pc, const doubl
                          double fugacity_tolerance, int max_phases, int threads, int32_t* status,
                          int32_t* phase_count, double* beta, double* zfactor, int32_t* branch, double* x,
                          double* ln_phi, uint32_t* counters) {
-    if (nhint < 0 || nhint > 2 || (nhint > 0 && !hints)) { return 1; }
-    if (nfinal < 0 || nfinal > 2 || (nfinal > 0 && !final_hints)) { return 1; }
+    if (nhint < 0 || nhint > 3 || (nhint > 0 && !hints)) { return 1; }
+    if (nfinal < 0 || nfinal > 3 || (nfinal > 0 && !final_hints)) { return 1; }
     try {
         const auto parameters = make_parameters(nc, tc, pc, omega, kij);
         const auto model = th::Pr76Phase<double>::from_parameters(parameters);
@@ -564,6 +554,60 @@ int ref_assembly(int nc, const double* tc, const double* pc, const double* omega
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "ref_assembly: %s\n", e.what());
+        return 1;
+    }
+}
+
+// solve_pr76_pt_continuation (pr76_pt_continuation.hpp:194-310) over npath ordered paths;
+// layout and info word as cpuref_pt_continuation (oracle/cpuref.h). A path the reference
+// rejects (it throws) is published as rejected (0x2000) at every point.
+int ref_pt_continuation(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
+                        int64_t npath, int npoint, const double* p, const double* t, const double* z, int carry,
+                        double fugacity_tolerance, int threads, int32_t* status, int32_t* phase_count,
+                        double* beta, double* zfactor, int32_t* branch, double* x, double* ln_phi,
+                        uint32_t* counters, int32_t* info) {
+    if (npath < 0 || npoint < 1) { return 1; }
+    try {
+        const auto parameters = make_parameters(nc, tc, pc, omega, kij);
+        const auto model = th::Pr76Phase<double>::from_parameters(parameters);
+        const int64_t N = npath;
+        auto out_at = [&](int k) {
+            return Outputs{status + k * N, phase_count + k * N, beta + k * 3 * N, zfactor + k * 3 * N,
+                           branch + k * 3 * N, x + k * 3 * nc * N, ln_phi + k * 3 * nc * N,
+                           counters ? counters + k * 4 * N : nullptr};
+        };
+        parallel_cells(npath, threads, [&](auto&& run) {
+            fl::Pr76VleEvaluator evaluator(model);
+            run([&](int64_t j) {
+                std::vector<fl::Pr76PtPathState> path(static_cast<std::size_t>(npoint));
+                for (int k = 0; k < npoint; ++k) { path[k] = {p[k * N + j], t[k * N + j]}; }
+                std::vector<double> feed(nc);
+                for (int i = 0; i < nc; ++i) { feed[i] = z[i * npath + j]; }
+                fl::Pr76PtContinuationOptions options;
+                options.point_options.two_phase.iteration.fugacity_tolerance = fugacity_tolerance;
+                options.carry_previous_phase_compositions = (carry & 1) != 0;
+                options.carry_previous_three_phase_state = (carry & 2) != 0;
+                try {
+                    const auto result = fl::solve_pr76_pt_continuation(path, feed, evaluator, options);
+                    for (int k = 0; k < npoint; ++k) {
+                        const auto& point = result.points[static_cast<std::size_t>(k)];
+                        publish_max3(point.solve, nc, N, j, out_at(k),
+                                     point.carried_three_phase_start_consumed ? 0x20000 : 0);
+                        info[k * N + j] = static_cast<int32_t>(point.incoming_hint) |
+                                          static_cast<int32_t>(point.carried_stability_start_count << 4) |
+                                          (point.carried_three_phase_start ? 0x100 : 0);
+                    }
+                } catch (const std::exception&) {
+                    for (int k = 0; k < npoint; ++k) {
+                        fl::PtPhaseSetResult empty;
+                        publish(empty, nc, N, j, 0xF, 0xF, Counts{}, out_at(k), 0x2000);
+                        info[k * N + j] = 0;
+                    }
+                }
+            });
+        });
+        return 0;
+    } catch (const std::exception&) {
         return 1;
     }
 }

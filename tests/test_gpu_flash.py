@@ -67,6 +67,10 @@ def _check(name, mix, p, t, z, tol, maxph):
         print("  decision mismatches at cells", idx.tolist(), "gpu", g["status"][idx].tolist(), "oracle",
               o["status"][idx].tolist(), flush=True)
     assert nbad_dec == 0, rep
+    # search flags of the status word (MPMC_FLAG_SPLIT_*, MPMC_FLAG_MAX3_ATTEMPTED)
+    fdiff = np.nonzero(g["status"] != o["status"])[0]
+    print(f"  {name}: status words (flags included) differ at {fdiff.size} cells {fdiff[:8].tolist()}", flush=True)
+    assert fdiff.size == 0
     _values_within_stopping_error(name, g, o, bad, mix, p, t, z, tol, maxph)
     res = parity.fugacity_residual(g)
     acc = (g["status"] & 0xF) == 0

@@ -175,6 +175,32 @@ def test_oracle_matches_reference_live(nc):
     rep, bad = parity.compare(o, r, rel=0.0, abs_lnphi=0.0)
     assert not bad.any(), rep
     assert np.array_equal(o["counters"][:2], r["counters"][:2])
+    # the whole status word, search flags included (MPMC_FLAG_SPLIT_*, MPMC_FLAG_MAX3_ATTEMPTED)
+    assert np.array_equal(o["status"], r["status"])
+
+
+@pytest.mark.skipif(not ob.have_ref(), reason="reference build (oracle/_ref) not present")
+def test_search_flags_match_reference():
+    """The status-word search flags (a disappearance attempt, a converged candidate, a
+    three-phase attempt) equal the reference's, on the sour-gas sweep (three-phase attempts,
+    indeterminate splits with a candidate) and on feeds 5e-11 inside a two-phase boundary
+    (split attempts that end at a phase-fraction endpoint)."""
+    mix, p, t, z = W.sour_gas_sweep(2048)
+    idx = np.arange(0, 2048, 2)
+    p, t, z = p[idx], t[idx], np.ascontiguousarray(z[:, idx])
+    o = ob.flash(mix, p, t, z, 1e-11, 3)
+    r = ob.flash(mix, p, t, z, 1e-11, 3, which="ref")
+    assert np.array_equal(o["status"], r["status"])
+    assert np.any(o["status"] & 0x8000) and np.any(o["status"] & 0x10000)
+    indet = (o["status"] >> 8 & 0xF) == 3
+    assert np.any(indet & (o["status"] & 0x8000 != 0))
+    mix = W.mixture_c1()
+    p, t, z = W.cells_c1()
+    p, t, z = W.boundary_feeds(p, t, ob.flash(mix, p, t, z, 1e-12, 3), 5e-11)
+    o = ob.flash(mix, p, t, z, 1e-12, 3)
+    r = ob.flash(mix, p, t, z, 1e-12, 3, which="ref")
+    assert np.array_equal(o["status"], r["status"])
+    assert np.sum(o["status"] & 0x4000 != 0) > 50
 
 
 def test_planted_mistake_is_caught():
