@@ -185,7 +185,7 @@ int mpmc_flash_batch_static(mpmc_mixture_t mixture, int64_t ncell, const double*
                             const double* z, const mpmc_flash_options* options, const mpmc_flash_outputs* out,
                             void* stream);
 
-/* Warm-started batch flash: up to 3 compositions per cell, hints[(h*nc + i)*ncell + c]
+/* Warm-started batch flash: up to 6 compositions per cell, hints[(h*nc + i)*ncell + c]
  * (device memory), appended in order to the initial stability test's starts after the
  * generated ones, as Pr76PtFlashBackendOptions::initial_starts does for the
  * initial_stability_starts of PtSolveHints (pt_solve_hints.hpp:26-43,
@@ -199,7 +199,7 @@ int mpmc_flash_batch_hinted(mpmc_mixture_t mixture, int64_t ncell, const double*
                             const mpmc_flash_options* options, const mpmc_flash_outputs* out,
                             void* workspace, size_t workspace_bytes, void* stream);
 
-/* mpmc_flash_batch_hinted plus final-review warm starts: up to 3 compositions per cell,
+/* mpmc_flash_batch_hinted plus final-review warm starts: up to 6 compositions per cell,
  * final_hints[(h*nc + i)*ncell + c] (device memory), placed before the selected pair's two
  * compositions in the common-tangent review, as Pr76PtFlashBackendOptions::final_starts
  * does for the final_two_phase_stability_starts of PtSolveHints (pt_solve_hints.hpp:34-36,
@@ -215,28 +215,48 @@ int mpmc_flash_batch_warm(mpmc_mixture_t mixture, int64_t ncell, const double* p
  * of npath paths at once. Path j has the feed z[i*npath + j] and npoint ordered states
  * p[k*npath + j], t[k*npath + j] (device memory); point k of every path is a max3 flash
  * (options->max_phases must be 3; staged schedule) of the path feed normalised once, and
- * the points of one path run in order while the paths run side by side. carry bit 0
- * (carry_previous_phase_compositions): the previous point's accepted phase compositions
- * (deduplicated; a single phase carries its normalised feed) are the initial and the
- * final-review stability starts of the next point; bit 1 (carry_previous_three_phase_state):
- * a three-phase previous point's compositions and phase-1/2 fractions are the next point's
- * first three-phase start. After a point that accepts no phase set nothing is carried.
+ * the points of one path run in order while the paths run side by side.
+ * continuation options (NULL: mpmc_continuation_default_options):
+ *   carry_compositions (carry_previous_phase_compositions): the previous point's accepted
+ *     phase compositions (a single phase carries its normalised feed) lead the next point's
+ *     initial and final-review stability starts;
+ *   carry_three_phase (carry_previous_three_phase_state): a three-phase previous point's
+ *     compositions and phase-1/2 fractions are the next point's first three-phase start;
+ *   fallback_initial / fallback_final (fallback_initial_starts / _final_starts, at most 3
+ *     each): appended after the carried compositions; every start list is kept free of
+ *     repeats (exact equality);
+ *   three_phase_start (point_options.three_phase_starts with one entry): a fixed start
+ *     tried after the carried one, with seed start_seed.
+ * After a point that accepts no phase set nothing is carried.
  * out holds npoint consecutive flash records of npath cells (the mpmc_flash_outputs layout
  * with ncell = npath, point k at offset k times one record); MPMC_FLAG_START_CONSUMED marks a
  * point whose three-phase stage ran from the carried start. point_info (device, [npoint]
  * [npath], may be NULL) = incoming hint phase count (0: none) | carried stability start
  * count << 4 | carried three-phase start << 8. A path whose feed or states fail the
- * reference's path check, or one of whose points is rejected, is published as rejected
- * (MPMC_FLAG_REJECTED) at every point with point_info 0, as the reference rejects the whole
- * path. The paths' transition brackets follow from consecutive accepted phase counts.
- * workspace NULL: the mixture's own, else mpmc_pt_continuation_workspace_bytes(nc, npath)
- * bytes of device memory.
+ * reference's path check, or one of whose points is rejected (including by a fallback or
+ * fixed start the feed does not support), is published as rejected (MPMC_FLAG_REJECTED) at
+ * every point with point_info 0, as the reference rejects the whole path. Transition
+ * brackets follow from consecutive accepted phase counts. workspace NULL: the mixture's
+ * own, else mpmc_pt_continuation_workspace_bytes(nc, npath) bytes of device memory.
  */
+typedef struct mpmc_continuation_options {
+    int carry_compositions;  /* default 1 */
+    int carry_three_phase;   /* default 1 */
+    int nfallback_initial;   /* 0..3 */
+    int nfallback_final;     /* 0..3 */
+    int three_phase_start;   /* 0 or 1 */
+    double fallback_initial[3][MPMC_MAX_COMPONENTS]; /* [k][i], i < nc */
+    double fallback_final[3][MPMC_MAX_COMPONENTS];
+    double start[3][MPMC_MAX_COMPONENTS];            /* [phase][i] */
+    double start_seed[2];    /* fractions of phases 1, 2; default 1/3, 1/3 */
+} mpmc_continuation_options;
+
+void mpmc_continuation_default_options(mpmc_continuation_options* options);
 size_t mpmc_pt_continuation_workspace_bytes(int nc, int64_t npath);
 int mpmc_pt_continuation(mpmc_mixture_t mixture, int64_t npath, int npoint, const double* p, const double* t,
-                         const double* z, int carry, const mpmc_flash_options* options,
-                         const mpmc_flash_outputs* out, int32_t* point_info, void* workspace,
-                         size_t workspace_bytes, void* stream);
+                         const double* z, const mpmc_continuation_options* continuation,
+                         const mpmc_flash_options* options, const mpmc_flash_outputs* out, int32_t* point_info,
+                         void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Ragged multi-mixture batch: the cells of several mixtures with the same component count

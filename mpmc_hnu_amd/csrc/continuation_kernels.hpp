@@ -5,11 +5,11 @@
 // reject every point of a path the reference would have rejected as a whole.
 //
 // Per-path buffers (n = npath, cell index fastest):
-//   feed   [NC][n]     the path feed normalised once (pr76_pt_continuation.hpp:218-219)
-//   hints  [3][NC][n]  carried phase compositions, deduplicated, NaN in unused slots; passed
-//                      as both the initial and the final-review stability starts
-//   tpz    [3][NC][n]  carried three-phase compositions
-//   tpf    [2][n]      carried phase-fraction seed of phases 1, 2 (NaN: no three-phase start)
+//   feed    [NC][n]        the path feed normalised once (pr76_pt_continuation.hpp:218-219)
+//   hints   [6][NC][n]     initial stability starts (carried, then fallback), NaN past the list
+//   fhints  [6][NC][n]     final-review starts (carried, then fallback)
+//   tpz     [2][3][NC][n]  three-phase start slots: 0 carried state, 1 the fixed start
+//   tpf     [2][2][n]      their phase-fraction seeds of phases 1, 2 (NaN: slot empty)
 #pragma once
 
 #include "flash_kernels.hpp"
@@ -42,20 +42,52 @@ __global__ __launch_bounds__(256) void k_cont_feed(int64_t n, const double* __re
     }
 }
 
+// Batch-wide options of the continuation (Pr76PtContinuationOptions,
+// pr76_pt_continuation.hpp:27-34): carry bit 0 carries phase compositions, bit 1 the
+// three-phase state; fallback initial / final stability starts (fi, ff); one fixed
+// three-phase start (point_options.three_phase_starts) with its fraction seed.
+struct ContStarts {
+    int carry;
+    int nfi, nff;
+    int has_start;
+    double fi[3][12], ff[3][12], st[3][12], sf[2];
+};
+
+// Appends composition v to the cell's start list H ([slots][NC][n]) unless it equals (==,
+// component by component) an entry already there (pr76_pt_continuation_append_unique).
+template <int NC>
+__device__ inline void append_unique(double* H, int64_t n, int64_t c, int& count, const double* v) {
+    bool dup = false;
+    for (int e = 0; e < count; ++e) {
+        bool same = true;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { same = same && H[(e * NC + i) * n + c] == v[i]; }
+        dup = dup || same;
+    }
+    if (dup) { return; }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { H[(count * NC + i) * n + c] = v[i]; }
+    ++count;
+}
+
 // Warm starts of point k from point k-1's record `prev` (have_prev false at point 0), as
-// pr76_pt_continuation_phase_compositions / _three_phase_start (pr76_pt_continuation.hpp:
-// 110-161) read them from the reference's result: the accepted phase count comes from the
-// max3 status code (single 0, two 1, three 2; anything else accepts nothing); a single phase
-// carries the stability test's feed (the path feed normalised again), two phases carry the
-// published liquid and vapour compositions (those of the accepted boundary neighbour where
-// one was accepted), three phases carry the three compositions and the fractions of phases
-// 1 and 2. info = incoming hint phase count | carried start count << 4 | carried
-// three-phase start << 8.
+// solve_pr76_pt_continuation builds them (pr76_pt_continuation.hpp:226-251) from the
+// reference's result (:110-161): the accepted phase count comes from the max3 status code
+// (single 0, two 1, three 2; anything else accepts nothing); a single phase carries the
+// stability test's feed (the path feed normalised again), two phases the published liquid
+// and vapour compositions (those of the accepted boundary neighbour where one was
+// accepted), three phases the three compositions, and a three-phase point also the state
+// (compositions, fractions of phases 1 and 2) for start slot 0. The initial starts are the
+// carried compositions then the fallback initial starts, the final-review starts the carried
+// compositions then the fallback final starts, each list without repeats; slots up to nh /
+// nfh past the list are NaN. Start slot 1 holds the fixed three-phase start.
+// info = incoming hint phase count | carried start count << 4 | carried three-phase start << 8.
 template <int NC>
 __global__ __launch_bounds__(256) void k_cont_hints(int64_t n, FlashOut prev, int have_prev,
-                                                    const double* __restrict__ feed, int carry,
-                                                    double* __restrict__ hints, double* __restrict__ tpz,
-                                                    double* __restrict__ tpf, int32_t* __restrict__ info) {
+                                                    const double* __restrict__ feed, ContStarts o, int nh, int nfh,
+                                                    double* __restrict__ hints, double* __restrict__ fhints,
+                                                    double* __restrict__ tpz, double* __restrict__ tpf,
+                                                    int32_t* __restrict__ info) {
     const double nan = __builtin_nan("");
     for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -80,41 +112,46 @@ __global__ __launch_bounds__(256) void k_cont_hints(int64_t n, FlashOut prev, in
                 if (count == 3) { v2[i] = prev.x[(2 * NC + i) * n + c]; }
             }
         }
-        const bool carry_comp = (carry & 1) != 0;
-        const bool three = (carry & 2) != 0 && count == 3;
-        if (carry_comp) {
-            // pr76_pt_continuation_append_unique: a composition equal (==, component by
-            // component) to an earlier one is not added again
-            bool same01 = true, same02 = true, same12 = true;
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                same01 = same01 && v1[i] == v0[i];
-                same02 = same02 && v2[i] == v0[i];
-                same12 = same12 && v2[i] == v1[i];
+        const bool carry_comp = (o.carry & 1) != 0 && count > 0;
+        for (int list = 0; list < 2; ++list) {
+            double* H = list == 0 ? hints : fhints;
+            const int slots = list == 0 ? nh : nfh;
+            if (slots == 0) { continue; }
+            int filled = 0;
+            if (carry_comp) {
+                append_unique<NC>(H, n, c, filled, v0);
+                if (count >= 2) { append_unique<NC>(H, n, c, filled, v1); }
+                if (count == 3) { append_unique<NC>(H, n, c, filled, v2); }
             }
-            const bool use1 = count >= 2 && !same01;
-            const bool use2 = count == 3 && !same02 && !same12;
+            const int nfb = list == 0 ? o.nfi : o.nff;
+            for (int h = 0; h < nfb; ++h) {
+                double w[NC];
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const double h0 = count >= 1 ? v0[i] : nan;
-                const double h1 = use1 ? v1[i] : (use2 ? v2[i] : nan);
-                const double h2 = use1 && use2 ? v2[i] : nan;
-                hints[(0 * NC + i) * n + c] = h0;
-                hints[(1 * NC + i) * n + c] = h1;
-                hints[(2 * NC + i) * n + c] = h2;
+                for (int i = 0; i < NC; ++i) { w[i] = list == 0 ? o.fi[h][i] : o.ff[h][i]; }
+                append_unique<NC>(H, n, c, filled, w);
+            }
+            for (int e = filled; e < slots; ++e) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) { H[(e * NC + i) * n + c] = nan; }
             }
         }
-        if ((carry & 2) != 0) {
+        const bool three = (o.carry & 2) != 0 && count == 3;
+        if (tpz) {
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
                 tpz[(0 * NC + i) * n + c] = three ? v0[i] : nan;
                 tpz[(1 * NC + i) * n + c] = three ? v1[i] : nan;
                 tpz[(2 * NC + i) * n + c] = three ? v2[i] : nan;
+                tpz[(3 * NC + i) * n + c] = o.has_start ? o.st[0][i] : nan;
+                tpz[(4 * NC + i) * n + c] = o.has_start ? o.st[1][i] : nan;
+                tpz[(5 * NC + i) * n + c] = o.has_start ? o.st[2][i] : nan;
             }
-            tpf[c] = three ? prev.beta[1 * n + c] : nan;
-            tpf[n + c] = three ? prev.beta[2 * n + c] : nan;
+            tpf[0 * n + c] = three ? prev.beta[1 * n + c] : nan;
+            tpf[1 * n + c] = three ? prev.beta[2 * n + c] : nan;
+            tpf[2 * n + c] = o.has_start ? o.sf[0] : nan;
+            tpf[3 * n + c] = o.has_start ? o.sf[1] : nan;
         }
-        if (info) { info[c] = count | ((carry_comp ? count : 0) << 4) | (three ? 0x100 : 0); }
+        if (info) { info[c] = count | ((o.carry & 1) ? count << 4 : 0) | (three ? 0x100 : 0); }
     }
 }
 

@@ -344,8 +344,8 @@ int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, cons
                           int nhint, const double* hints, int nfinal, const double* final_hints,
                           const mpmc_flash_options* options, const mpmc_flash_outputs* out, void* workspace,
                           size_t workspace_bytes, void* stream) {
-    if (nhint < 0 || nhint > 3 || nfinal < 0 || nfinal > 3) {
-        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: nhint and nfinal must be 0..3");
+    if (nhint < 0 || nhint > 6 || nfinal < 0 || nfinal > 6) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: nhint and nfinal must be 0..6");
     }
     if ((nhint > 0 && ncell > 0 && !is_device_ptr(hints)) || (nfinal > 0 && ncell > 0 && !is_device_ptr(final_hints))) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_warm: hints must be device memory");
@@ -358,7 +358,7 @@ int mpmc_flash_batch_warm(mpmc_mixture_t m, int64_t ncell, const double* p, cons
 int mpmc_flash_batch_hinted(mpmc_mixture_t m, int64_t ncell, const double* p, const double* t, const double* z,
                             int nhint, const double* hints, const mpmc_flash_options* options,
                             const mpmc_flash_outputs* out, void* workspace, size_t workspace_bytes, void* stream) {
-    if (nhint < 0 || nhint > 3) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: nhint must be 0..3"); }
+    if (nhint < 0 || nhint > 6) { return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: nhint must be 0..6"); }
     if (nhint > 0 && ncell > 0 && !is_device_ptr(hints)) {
         return fail(MPMC_E_INVALID_ARGUMENT, "flash_batch_hinted: hints must be device memory");
     }
@@ -368,17 +368,26 @@ int mpmc_flash_batch_hinted(mpmc_mixture_t m, int64_t ncell, const double* p, co
 
 // ---------------------------------------------------------------- ordered-path continuation
 namespace {
-// Continuation buffers after the pipeline workspace: feed [NC][n], hints [3][NC][n],
-// three-phase start [3][NC][n] + [2][n] (continuation_kernels.hpp).
-size_t continuation_extra(int nc, int64_t n) {
-    const size_t N = static_cast<size_t>(n), NC = static_cast<size_t>(nc);
-    auto a = [](size_t v) { return (v + 255) & ~size_t{255}; };
-    return a(NC * N * 8) + a(3 * NC * N * 8) + a(3 * NC * N * 8) + a(2 * N * 8);
-}
+constexpr int kContSlots = 6; // carried compositions (3) + fallback starts (3) per start list
+// Continuation buffers after the pipeline workspace (continuation_kernels.hpp): feed
+// [NC][n], initial and final-review starts [6][NC][n] each, three-phase start slots
+// [2][3][NC][n] and seeds [2][2][n].
+size_t a256(size_t v) { return (v + 255) & ~size_t{255}; }
 size_t continuation_bytes(int nc, int64_t n) {
-    return ((workspace_bytes_for(nc, n) + 255) & ~size_t{255}) + continuation_extra(nc, n);
+    const size_t N = static_cast<size_t>(n), NC = static_cast<size_t>(nc);
+    return a256(workspace_bytes_for(nc, n)) + a256(NC * N * 8) + 2 * a256(kContSlots * NC * N * 8) +
+           a256(6 * NC * N * 8) + a256(4 * N * 8);
 }
 } // namespace
+
+void mpmc_continuation_default_options(mpmc_continuation_options* o) {
+    if (!o) { return; }
+    std::memset(o, 0, sizeof(*o));
+    o->carry_compositions = 1;
+    o->carry_three_phase = 1;
+    o->start_seed[0] = 1.0 / 3.0;
+    o->start_seed[1] = 1.0 / 3.0;
+}
 
 size_t mpmc_pt_continuation_workspace_bytes(int nc, int64_t npath) {
     if (nc < 1 || nc > MPMC_MAX_COMPONENTS || npath < 0) { return 0; }
@@ -386,16 +395,23 @@ size_t mpmc_pt_continuation_workspace_bytes(int nc, int64_t npath) {
 }
 
 int mpmc_pt_continuation(mpmc_mixture_t m, int64_t npath, int npoint, const double* p, const double* t,
-                         const double* z, int carry, const mpmc_flash_options* options,
-                         const mpmc_flash_outputs* out, int32_t* point_info, void* workspace, size_t workspace_bytes,
-                         void* stream) {
+                         const double* z, const mpmc_continuation_options* continuation,
+                         const mpmc_flash_options* options, const mpmc_flash_outputs* out, int32_t* point_info,
+                         void* workspace, size_t workspace_bytes, void* stream) {
     if (!m || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: null mixture or outputs"); }
     if (npath < 0 || npath > (int64_t{1} << 31)) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: bad npath"); }
     // Pr76PtContinuationOptions::max_points (pr76_pt_continuation.hpp:33, checked at :164-166)
     if (npoint < 1 || npoint > 4096) {
         return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: npoint must be 1..4096");
     }
-    if (carry < 0 || carry > 3) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: carry must be 0..3"); }
+    mpmc_continuation_options co;
+    mpmc_continuation_default_options(&co);
+    if (continuation) { co = *continuation; }
+    if (co.nfallback_initial < 0 || co.nfallback_initial > 3 || co.nfallback_final < 0 || co.nfallback_final > 3 ||
+        (co.three_phase_start != 0 && co.three_phase_start != 1)) {
+        return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: at most 3 fallback starts per list and 1 fixed "
+                                             "three-phase start");
+    }
     mpmc_flash_options o;
     if (const int rc = flash_options(options, o); rc != MPMC_OK) { return rc; }
     if (o.max_phases != 3) { return fail(MPMC_E_INVALID_ARGUMENT, "pt_continuation: the path points are max3 flashes"); }
@@ -415,15 +431,36 @@ int mpmc_pt_continuation(mpmc_mixture_t m, int64_t npath, int npoint, const doub
         rc != MPMC_OK) {
         return rc;
     }
-    const size_t pipe_bytes = (workspace_bytes_for(nc, npath) + 255) & ~size_t{255};
-    char* extra = static_cast<char*>(workspace) + pipe_bytes;
+    const size_t pipe_bytes = a256(workspace_bytes_for(nc, npath));
     const size_t N = static_cast<size_t>(npath), NC = static_cast<size_t>(nc);
-    auto a256 = [](size_t v) { return (v + 255) & ~size_t{255}; };
-    double* feed = reinterpret_cast<double*>(extra);
-    double* hints = reinterpret_cast<double*>(extra + a256(NC * N * 8));
-    double* tpz = reinterpret_cast<double*>(extra + a256(NC * N * 8) + a256(3 * NC * N * 8));
-    double* tpf = reinterpret_cast<double*>(extra + a256(NC * N * 8) + 2 * a256(3 * NC * N * 8));
-    const int nh = (carry & 1) ? 3 : 0;
+    char* at = static_cast<char*>(workspace) + pipe_bytes;
+    auto carve = [&](size_t bytes) {
+        double* ptr = reinterpret_cast<double*>(at);
+        at += a256(bytes);
+        return ptr;
+    };
+    double* feed = carve(NC * N * 8);
+    double* hints = carve(kContSlots * NC * N * 8);
+    double* fhints = carve(kContSlots * NC * N * 8);
+    double* tpz = carve(6 * NC * N * 8);
+    double* tpf = carve(4 * N * 8);
+    mpmc::dev::ContStarts cs{};
+    cs.carry = (co.carry_compositions ? 1 : 0) | (co.carry_three_phase ? 2 : 0);
+    cs.nfi = co.nfallback_initial;
+    cs.nff = co.nfallback_final;
+    cs.has_start = co.three_phase_start;
+    for (int k = 0; k < 3; ++k) {
+        for (int i = 0; i < nc; ++i) {
+            cs.fi[k][i] = co.fallback_initial[k][i];
+            cs.ff[k][i] = co.fallback_final[k][i];
+            cs.st[k][i] = co.start[k][i];
+        }
+    }
+    cs.sf[0] = co.start_seed[0];
+    cs.sf[1] = co.start_seed[1];
+    const int carried = (cs.carry & 1) ? 3 : 0;
+    const int nh = carried + cs.nfi, nfh = carried + cs.nff;
+    const bool slots3 = (cs.carry & 2) || cs.has_start;
     const mpmc::dev::FlashOut fo{out->status, out->phase_count, out->beta, out->zfactor, out->branch, out->x,
                                  out->ln_phi, out->counters};
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>((npath + 255) / 256, 2048));
@@ -435,14 +472,14 @@ int mpmc_pt_continuation(mpmc_mixture_t m, int64_t npath, int npoint, const doub
         for (int k = 0; k < npoint && e == hipSuccess; ++k) {                                                         \
             const int64_t off = static_cast<int64_t>(k) * npath;                                                      \
             hipLaunchKernelGGL(mpmc::dev::k_cont_hints<N>, dim3(grid), dim3(256), 0, s, npath,                        \
-                               mpmc::dev::point_slice<N>(fo, npath, k > 0 ? k - 1 : 0), k > 0 ? 1 : 0, feed, carry,   \
-                               hints, tpz, tpf, point_info ? point_info + off : nullptr);                             \
+                               mpmc::dev::point_slice<N>(fo, npath, k > 0 ? k - 1 : 0), k > 0 ? 1 : 0, feed, cs, nh,  \
+                               nfh, hints, fhints, slots3 ? tpz : nullptr, tpf, point_info ? point_info + off : nullptr); \
             e = hipGetLastError();                                                                                    \
             if (e == hipSuccess) {                                                                                    \
                 e = mpmc::dev::launch_pipeline<N>(m->dev_constants, 1, nullptr, workspace, pipe_bytes, npath, p + off, \
                                                   t + off, feed, 3, o.fugacity_tolerance, o.precision,                \
-                                                  mpmc::dev::point_slice<N>(fo, npath, k), s, nh, hints, nh, hints,   \
-                                                  (carry & 2) ? tpz : nullptr, (carry & 2) ? tpf : nullptr);          \
+                                                  mpmc::dev::point_slice<N>(fo, npath, k), s, nh, hints, nfh, fhints, \
+                                                  slots3 ? tpz : nullptr, slots3 ? tpf : nullptr);                    \
             }                                                                                                         \
         }                                                                                                             \
         if (e == hipSuccess) {                                                                                        \

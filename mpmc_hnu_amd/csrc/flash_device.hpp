@@ -1097,19 +1097,22 @@ struct Max3Tail {
 };
 
 // The part of solve_pr76_pt_max3 (pr76_three_phase.hpp:360-561) that runs once the
-// two-phase common-tangent review found lower-Gibbs evidence. A caller-supplied three-phase
-// start (Pr76PtMax3Options::three_phase_starts with one entry: the PR76 PT continuation's
-// carried state, pr76_pt_continuation.hpp:240-251) is tried first, over the eight root-side
-// combinations (:472-498); then negative final-review witnesses seed attempts (witness
-// sides from pr76_witness_root_sides), each followed by the three-phase final review or, on
+// two-phase common-tangent review found lower-Gibbs evidence. Caller-supplied three-phase
+// starts (Pr76PtMax3Options::three_phase_starts) are tried first, in slot order, each over
+// the eight root-side combinations (:472-498): slot 0 is the PR76 PT continuation's carried
+// state (inserted first, pr76_pt_continuation.hpp:240-251), slot 1 a fixed start; then
+// negative final-review witnesses seed attempts (witness sides from
+// pr76_witness_root_sides), each followed by the three-phase final review or, on
 // disappearance, a fresh two-phase neighbour solve. `witness(i, w, branch)` yields the i-th
-// negative witness of the review in trial order. start: three compositions [3][NC] or
-// nullptr; start_frac: the phase-fraction seed of phases 1 and 2.
+// negative witness of the review in trial order. start0 / start1: three compositions [3][NC]
+// or nullptr; frac0 / frac1: their phase-fraction seeds of phases 1 and 2. Two slots make at
+// most 16 supplied attempts, the reference's continuation quota (max_three_phase_attempts).
 template <int NC, bool kN, typename WitnessFn>
 __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, const double* z, const double* px,
                                        const double* py, double pbeta, int nwit, WitnessFn witness, double fug_tol,
-                                       Max3Tail<NC>& M, Counters& cnt, bool& rejected, const double* start = nullptr,
-                                       const double* start_frac = nullptr) {
+                                       Max3Tail<NC>& M, Counters& cnt, bool& rejected, const double* start0 = nullptr,
+                                       const double* frac0 = nullptr, const double* start1 = nullptr,
+                                       const double* frac1 = nullptr) {
     M.use_neighbor = false;
     M.attempts = 0;
     M.start_consumed = false;
@@ -1150,17 +1153,20 @@ __device__ __noinline__ void max3_tail(const Mix<NC>& m, double p, double t, con
         else if (F.status == S_NO_INSTABILITY) { M.status = M_THREE; M.three = E; return true; }
         return false;
     };
-    if (start) {
+    // true when an attempt from this start closes the solve; `carried`: slot 0
+    auto supplied = [&](const double* st, const double* sf, bool carried) -> bool {
         double l1[NC], l2[NC];
-        if (log_ratio<NC>(start + NC, start, z, l1) && log_ratio<NC>(start + 2 * NC, start, z, l2)) {
-            // side of phase 0 outermost, of phase 2 innermost; 0 = lower, 1 = upper admissible
-            for (int code = 0; code < 8; ++code) {
-                const int sd[3] = {code >> 2, (code >> 1) & 1, code & 1};
-                M.start_consumed = true;
-                if (attempt(l1, l2, start_frac[0], start_frac[1], sd)) { return; }
-            }
+        if (!(log_ratio<NC>(st + NC, st, z, l1) && log_ratio<NC>(st + 2 * NC, st, z, l2))) { return false; }
+        // side of phase 0 outermost, of phase 2 innermost; 0 = lower, 1 = upper admissible
+        for (int code = 0; code < 8; ++code) {
+            const int sd[3] = {code >> 2, (code >> 1) & 1, code & 1};
+            M.start_consumed = M.start_consumed || carried;
+            if (attempt(l1, l2, sf[0], sf[1], sd)) { return true; }
         }
-    }
+        return false;
+    };
+    if (start0 && supplied(start0, frac0, true)) { return; }
+    if (start1 && supplied(start1, frac1, false)) { return; }
     int automatic = 0;
     bool auto_limit = false;
     double src1[NC];
@@ -1205,8 +1211,8 @@ struct Max3Out {
     Max3Tail<NC> tail;
 };
 
-// solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), serial in one lane; start /
-// start_frac: one caller-supplied three-phase start (three_phase_starts), or nullptr.
+// solve_pr76_pt_max3 (pr76_three_phase.hpp:328-561), serial in one lane; start0 / frac0
+// and start1 / frac1: the supplied three-phase start slots of max3_tail, or nullptr.
 template <int NC, bool kN = false>
 // initial / final: extra starts of the initial stability test and of the base split's
 // common-tangent review (solve_pr76_pt_max3 initial_starts / final_starts,
@@ -1215,12 +1221,14 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
                                         Max3Out<NC>& M, Counters& cnt, bool& rejected,
                                         const double* const* initial = nullptr, int ninitial = 0,
                                         const double* const* final_starts = nullptr, int nfinal = 0,
-                                        const double* start = nullptr, const double* start_frac = nullptr) {
+                                        const double* start0 = nullptr, const double* frac0 = nullptr,
+                                        const double* start1 = nullptr, const double* frac1 = nullptr) {
     M.tail.use_neighbor = false;
     M.tail.attempts = 0;
     M.tail.start_consumed = false;
     (void)check_composition<NC>(feed, rejected);
-    if (start) { check_three_phase_start<NC>(feed, start, start_frac, rejected); }
+    if (start0) { check_three_phase_start<NC>(feed, start0, frac0, rejected); }
+    if (start1) { check_three_phase_start<NC>(feed, start1, frac1, rejected); }
     if (rejected) { return; }
     solve_vle<NC, kN>(m, p, t, feed, fug_tol, initial, ninitial, M.base, cnt, rejected, final_starts, nfinal);
     if (rejected) { return; }
@@ -1238,7 +1246,7 @@ __device__ __noinline__ void solve_max3(const Mix<NC>& m, double p, double t, co
                       for (int c = 0; c < NC; ++c) { w[c] = fin.wit[i].w[c]; }
                       branch = fin.wit[i].branch;
                   },
-                  fug_tol, M.tail, cnt, rejected, start, start_frac);
+                  fug_tol, M.tail, cnt, rejected, start0, frac0, start1, frac1);
     M.status = M.tail.status;
 }
 

@@ -53,7 +53,7 @@ namespace mpmc::dev {
 #endif
 
 constexpr int kAttempts = 16;   // PtSplitOptions::max_split_attempts (pt_split.hpp:49)
-constexpr int kMaxHints = 3;    // warm-start compositions per cell (extra initial stability starts)
+constexpr int kMaxHints = 6;    // warm-start compositions per cell (extra initial stability starts)
 constexpr int kWitnesses = 8;   // attempts per witness = 2 (vapor, liquid)
 
 // Outcomes of a reduced-precision trial or split attempt that FP64 must decide: an
@@ -86,7 +86,7 @@ struct Pipe {
     FlashOut out;
     double* feed;       // [NC][n]
     double* refl;       // [NC][n]
-    int32_t* refinfo;   // [n] branch | smooth << 8 | ok << 9 | active << 16 | hints << 24 | final hints << 26
+    int32_t* refinfo;   // [n] branch | smooth << 8 | ok << 9 | active << 16 | hints << 24 | final hints << 27
     int32_t* flags;     // [n]
     int32_t* cstat;     // [n] initial stability status
     int32_t* cell_u;    // [n]
@@ -132,10 +132,38 @@ struct Pipe {
     // before the selected pair's two compositions (pt_split.hpp:508-510)
     const double* fhints;
     int nfhint;
-    // one caller-supplied three-phase start per cell: compositions tpz[(ph*NC + i)*n + c] and
-    // phase-fraction seed tpf[k*n + c] of phases 1, 2 (NaN in tpf[c]: none); nullptr: none
+    // two caller-supplied three-phase start slots per cell (slot 0: a continuation's carried
+    // state, slot 1: a fixed start; tried in that order): compositions
+    // tpz[((s*3 + ph)*NC + i)*n + c] and phase-fraction seed tpf[(s*2 + k)*n + c] of phases
+    // 1, 2 (NaN in tpf[(s*2)*n + c]: slot s empty for cell c); nullptr: no slots
     const double* tpz;
     const double* tpf;
+};
+
+// A cell's supplied three-phase start slots, read from Pipe::tpz / tpf into registers.
+template <int NC>
+struct SuppliedStarts {
+    bool has0 = false, has1 = false;
+    double st0[3 * NC], st1[3 * NC];
+    double sf0[2], sf1[2];
+    __device__ void load(const Pipe<NC>& P, int64_t c) {
+        if (!P.tpz) { return; }
+        const int64_t n = P.n;
+        has0 = !__builtin_isnan(P.tpf[c]);
+        has1 = !__builtin_isnan(P.tpf[2 * n + c]);
+        sf0[0] = P.tpf[c];
+        sf0[1] = P.tpf[n + c];
+        sf1[0] = P.tpf[2 * n + c];
+        sf1[1] = P.tpf[3 * n + c];
+#pragma unroll
+        for (int k = 0; k < 3 * NC; ++k) {
+            st0[k] = P.tpz[k * n + c];
+            st1[k] = P.tpz[(3 * NC + k) * n + c];
+        }
+    }
+    __device__ const double* start(int slot) const {
+        return slot == 0 ? (has0 ? st0 : nullptr) : (has1 ? st1 : nullptr);
+    }
 };
 
 // ------------------------------------------------------------------ helpers
@@ -173,10 +201,10 @@ template <int NC>
 __device__ __forceinline__ int active_count(int32_t info) { return (info >> 16) & 0xFF; }
 
 template <int NC>
-__device__ __forceinline__ int hint_count(int32_t info) { return (info >> 24) & 0x3; }
+__device__ __forceinline__ int hint_count(int32_t info) { return (info >> 24) & 0x7; }
 
 template <int NC>
-__device__ __forceinline__ int final_hint_count(int32_t info) { return (info >> 26) & 0x3; }
+__device__ __forceinline__ int final_hint_count(int32_t info) { return (info >> 27) & 0x7; }
 
 // The m-th hint present for cell c in an [nh][NC][n] hint array (hints absent for a cell,
 // NaN in component 0, are skipped).
@@ -273,13 +301,17 @@ __global__ __launch_bounds__(64) void k_prep(Pipe<NC> P, const double* __restric
             for (int i = 0; i < NC; ++i) { if ((z[i] > 0) != (w[i] > 0)) { rejected = true; } }
             if (fin) { ++nfh; } else { ++nh; }
         }
-        info |= (nh << 24) | (nfh << 26);
-        if (P.tpz && !__builtin_isnan(P.tpf[c])) {
-            double st3[3 * NC];
-            const double sf[2] = {P.tpf[c], P.tpf[n + c]};
+        info |= (nh << 24) | (nfh << 27);
+        if (P.tpz) {
 #pragma unroll
-            for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[k * n + c]; }
-            check_three_phase_start<NC>(z, st3, sf, rejected);
+            for (int sl = 0; sl < 2; ++sl) {
+                if (__builtin_isnan(P.tpf[(sl * 2) * n + c])) { continue; }
+                double st3[3 * NC];
+                const double sf[2] = {P.tpf[(sl * 2) * n + c], P.tpf[(sl * 2 + 1) * n + c]};
+#pragma unroll
+                for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[(sl * 3 * NC + k) * n + c]; }
+                check_three_phase_start<NC>(z, st3, sf, rejected);
+            }
         }
         Activity<NC> ref;
         if (!rejected && !lost) {
@@ -947,14 +979,8 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
         Counters cnt{0, 0, 0, 0};
         bool rejected = false;
         Max3Tail<NC> M;
-        const bool has_start = P.tpz && !__builtin_isnan(P.tpf[c]);
-        double st3[3 * NC], sf[2];
-        if (has_start) {
-#pragma unroll
-            for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[k * n + c]; }
-            sf[0] = P.tpf[c];
-            sf[1] = P.tpf[n + c];
-        }
+        SuppliedStarts<NC> S;
+        S.load(P, c);
         max3_tail<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], z, x, y, beta, nwit,
                       [&](int i, double* w, int& branch) {
                           int seen = -1, k = 0;
@@ -965,7 +991,7 @@ __global__ __launch_bounds__(64) void k_max3_tail(Pipe<NC> P, const double* __re
                           for (int q = 0; q < NC; ++q) { w[q] = P.fw[(k * NC + q) * n + u]; }
                           branch = (P.fst[k * n + u] >> 8) & 0xFF;
                       },
-                      P.fug_tol, M, cnt, rejected, has_start ? st3 : nullptr, sf);
+                      P.fug_tol, M, cnt, rejected, S.start(0), S.sf0, S.start(1), S.sf1);
         atomicAdd(&P.cnt[0 * n + c], cnt.prop);
         atomicAdd(&P.cnt[1 * n + c], cnt.tpd);
         atomicAdd(&P.cnt[2 * n + c], cnt.rr);
@@ -1168,16 +1194,10 @@ __global__ __launch_bounds__(64) void k_fallback(Pipe<NC> P, const double* __res
             else { publish_vle<NC>(P.out, n, c, V, 0xF, cnt); }
         } else {
             Max3Out<NC> M;
-            const bool has_start = P.tpz && !__builtin_isnan(P.tpf[c]);
-            double st3[3 * NC], sf[2];
-            if (has_start) {
-#pragma unroll
-                for (int k = 0; k < 3 * NC; ++k) { st3[k] = P.tpz[k * n + c]; }
-                sf[0] = P.tpf[c];
-                sf[1] = P.tpf[n + c];
-            }
+            SuppliedStarts<NC> S;
+            S.load(P, c);
             solve_max3<NC, kN>(smix[mix_at(P, c)], P.p[c], P.t[c], feed, P.fug_tol, M, cnt, rejected, hs, nh, fs,
-                               nfh, has_start ? st3 : nullptr, sf);
+                               nfh, S.start(0), S.sf0, S.start(1), S.sf1);
             if (rejected) { write_cell<NC>(P.out, n, c, 2, 0xF, 0xF, 0, ph, cnt, 0x2000); }
             else { publish_max3<NC>(P.out, n, c, M, cnt); }
         }

@@ -105,7 +105,7 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
                 schedule: int = 0, workspace=None, hints=None, final_hints=None) -> dict:
     """Flash every cell (p[ncell] Pa, t[ncell] K, z[nc, ncell]); returns device tensors.
 
-    ``hints`` ([nhint, nc, ncell], nhint <= 3; NaN in component 0 marks an absent hint)
+    ``hints`` ([nhint, nc, ncell], nhint <= 6; NaN in component 0 marks an absent hint)
     are warm-start compositions appended to each cell's initial stability starts
     (mpmc_flash_batch_hinted); ``final_hints`` (same layout) are extra starts of the
     common-tangent review (mpmc_flash_batch_warm).
@@ -144,8 +144,8 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
                 hs.append(None)
                 continue
             h = _dev(h, torch.float64, device)
-            if h.dim() != 3 or h.shape[1:] != (mixture.nc, n) or h.shape[0] > 3:
-                raise ValueError("flash_batch: expected hints[nhint <= 3, nc, ncell]")
+            if h.dim() != 3 or h.shape[1:] != (mixture.nc, n) or h.shape[0] > 6:
+                raise ValueError("flash_batch: expected hints[nhint <= 6, nc, ncell]")
             hs.append(h)
         hints, final_hints = hs
         ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
@@ -160,8 +160,8 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
         return out
     if hints is not None:
         hints = _dev(hints, torch.float64, device)
-        if hints.dim() != 3 or hints.shape[1:] != (mixture.nc, n) or hints.shape[0] > 3:
-            raise ValueError("flash_batch: expected hints[nhint <= 3, nc, ncell]")
+        if hints.dim() != 3 or hints.shape[1:] != (mixture.nc, n) or hints.shape[0] > 6:
+            raise ValueError("flash_batch: expected hints[nhint <= 6, nc, ncell]")
         ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
             else (None, 0)
         rc = lib.mpmc_flash_batch_hinted(mixture.handle, n, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
@@ -180,6 +180,80 @@ def flash_batch(mixture: Mixture, p, t, z, fugacity_tolerance: float = 1e-11, ma
     native.check(rc, "flash_batch")
     out["_inputs"] = (p, t, z)  # keep inputs alive until the stream consumes them
     return out
+
+
+def pt_continuation(mixture: Mixture, p, t, z, carry_compositions: bool = True, carry_three_phase: bool = True,
+                    fallback_initial=None, fallback_final=None, start=None, seed=(1.0 / 3.0, 1.0 / 3.0),
+                    fugacity_tolerance: float = 1e-11, precision: int = 0, stream=None, counters: bool = True,
+                    workspace=None) -> dict:
+    """Ordered-path PT continuation (mpmc_pt_continuation; solve_pr76_pt_continuation,
+    pr76_pt_continuation.hpp:194-310) of npath paths: p, t [npoint, npath] ordered states,
+    z [nc, npath] path feeds. fallback_initial / fallback_final ([k <= 3, nc]) are the
+    fallback stability starts shared by every path, ``start`` ([3, nc]) a fixed three-phase
+    start with fraction seed ``seed``. Returns device tensors with a leading point axis
+    (status [npoint, npath], beta [npoint, 3, npath], ...) plus ``info`` [npoint, npath]
+    = incoming hint phase count | carried start count << 4 | carried three-phase start << 8."""
+    torch = _torch()
+    lib = native.library()
+    device = torch.device("cuda", torch.cuda.current_device())
+    p = _dev(p, torch.float64, device)
+    t = _dev(t, torch.float64, device)
+    z = _dev(z, torch.float64, device)
+    nc = mixture.nc
+    if p.dim() != 2 or t.shape != p.shape or z.dim() != 2 or z.shape != (nc, p.shape[1]):
+        raise ValueError("pt_continuation: expected p[npoint, npath], t[npoint, npath], z[nc, npath]")
+    m, n = p.shape
+    co = native.ContinuationOptions()
+    lib.mpmc_continuation_default_options(ctypes.byref(co))
+    co.carry_compositions = int(bool(carry_compositions))
+    co.carry_three_phase = int(bool(carry_three_phase))
+    for name, arr in (("fallback_initial", fallback_initial), ("fallback_final", fallback_final)):
+        if arr is None:
+            continue
+        arr = np.asarray(arr, dtype=np.float64)
+        if arr.ndim != 2 or arr.shape[1] != nc or arr.shape[0] > 3:
+            raise ValueError(f"pt_continuation: {name} must be [k <= 3, nc]")
+        setattr(co, "n" + name, arr.shape[0])
+        dst = getattr(co, name)
+        for k in range(arr.shape[0]):
+            for i in range(nc):
+                dst[k][i] = float(arr[k, i])
+    if start is not None:
+        st = np.asarray(start, dtype=np.float64)
+        if st.shape != (3, nc):
+            raise ValueError("pt_continuation: start must be [3, nc]")
+        co.three_phase_start = 1
+        for k in range(3):
+            for i in range(nc):
+                co.start[k][i] = float(st[k, i])
+        co.start_seed[0], co.start_seed[1] = float(seed[0]), float(seed[1])
+    f64, i32 = torch.float64, torch.int32
+    out = {
+        "status": torch.empty((m, n), dtype=i32, device=device),
+        "phase_count": torch.empty((m, n), dtype=i32, device=device),
+        "beta": torch.empty((m, 3, n), dtype=f64, device=device),
+        "zfactor": torch.empty((m, 3, n), dtype=f64, device=device),
+        "branch": torch.empty((m, 3, n), dtype=i32, device=device),
+        "x": torch.empty((m, 3, nc, n), dtype=f64, device=device),
+        "ln_phi": torch.empty((m, 3, nc, n), dtype=f64, device=device),
+        "counters": torch.empty((m, 4, n), dtype=i32, device=device) if counters else None,
+        "info": torch.empty((m, n), dtype=i32, device=device),
+    }
+    opts = _options(fugacity_tolerance, 3, precision)
+    o = _outputs_struct(out)
+    V = ctypes.c_void_p
+    ws, wsb = (V(workspace.data_ptr()), workspace.numel() * workspace.element_size()) if workspace is not None \
+        else (None, 0)
+    rc = lib.mpmc_pt_continuation(mixture.handle, n, m, V(p.data_ptr()), V(t.data_ptr()), V(z.data_ptr()),
+                                  ctypes.byref(co), ctypes.byref(opts), ctypes.byref(o), V(out["info"].data_ptr()), ws,
+                                  wsb, _stream_ptr(stream))
+    native.check(rc, "pt_continuation")
+    out["_inputs"] = (p, t, z)
+    return out
+
+
+def continuation_workspace_bytes(nc: int, npath: int) -> int:
+    return int(native.library().mpmc_pt_continuation_workspace_bytes(nc, npath))
 
 
 def workspace_bytes(nc: int, ncell: int) -> int:

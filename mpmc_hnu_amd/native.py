@@ -30,6 +30,9 @@ EXPORTED = (
     "mpmc_flash_multi",
     "mpmc_flash_batch_hinted",
     "mpmc_flash_batch_warm",
+    "mpmc_continuation_default_options",
+    "mpmc_pt_continuation_workspace_bytes",
+    "mpmc_pt_continuation",
     "mpmc_flash_plan_create",
     "mpmc_flash_plan_launch",
     "mpmc_flash_plan_info",
@@ -95,6 +98,20 @@ class FlashOutputs(ctypes.Structure):
         ("x", ctypes.c_void_p),
         ("ln_phi", ctypes.c_void_p),
         ("counters", ctypes.c_void_p),
+    ]
+
+
+class ContinuationOptions(ctypes.Structure):
+    _fields_ = [
+        ("carry_compositions", ctypes.c_int32),
+        ("carry_three_phase", ctypes.c_int32),
+        ("nfallback_initial", ctypes.c_int32),
+        ("nfallback_final", ctypes.c_int32),
+        ("three_phase_start", ctypes.c_int32),
+        ("fallback_initial", (ctypes.c_double * 12) * 3),
+        ("fallback_final", (ctypes.c_double * 12) * 3),
+        ("start", (ctypes.c_double * 12) * 3),
+        ("start_seed", ctypes.c_double * 2),
     ]
 
 
@@ -196,6 +213,13 @@ def library() -> ctypes.CDLL:
     lib.mpmc_flash_batch_warm.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int, P, ctypes.c_int, P,
                                           ctypes.POINTER(FlashOptions), ctypes.POINTER(FlashOutputs), P,
                                           ctypes.c_size_t, P]
+    lib.mpmc_continuation_default_options.argtypes = [ctypes.POINTER(ContinuationOptions)]
+    lib.mpmc_continuation_default_options.restype = None
+    lib.mpmc_pt_continuation_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64]
+    lib.mpmc_pt_continuation_workspace_bytes.restype = ctypes.c_size_t
+    lib.mpmc_pt_continuation.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, P, P, ctypes.POINTER(ContinuationOptions),
+                                         ctypes.POINTER(FlashOptions), ctypes.POINTER(FlashOutputs), P, P,
+                                         ctypes.c_size_t, P]
     lib.mpmc_flash_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64]
     lib.mpmc_flash_workspace_bytes.restype = ctypes.c_size_t
     lib.mpmc_flash_multi_max_mixtures.argtypes = [ctypes.c_int]

@@ -1223,22 +1223,24 @@ bool log_ratio(int n, const double* num, const double* den, const double* z, dou
 void solve_max3(const Mixture& m, double p, double t, const double* feed, const SplitOptions& so,
                 Max3Result& M, Counters& cnt, const double* const* initial = nullptr, int ninitial = 0,
                 const double* const* final_starts = nullptr, int nfinal = 0, const double* start = nullptr,
-                const double* start_frac = nullptr) {
+                const double* start_frac = nullptr, int nstart = 1) {
     const int n = m.n;
     (void)check_composition(feed, n);
     M.use_neighbor = false;
     M.attempts = 0;
     M.start_consumed = false;
-    // pr76_pt_max3_check_starts (pr76_three_phase.hpp:211-242): the supplied start is validated
-    // before any solve; a violation is the reference's domain_error
-    if (start) {
-        if (!(std::isfinite(start_frac[0]) && std::isfinite(start_frac[1]) && start_frac[0] >= 0.0 &&
-              start_frac[1] >= 0.0 && start_frac[0] + start_frac[1] <= 1.0)) {
+    // pr76_pt_max3_check_starts (pr76_three_phase.hpp:211-242): the supplied starts
+    // (start[s*3*n ...], seeds start_frac[2*s ...], s < nstart) are validated before any solve;
+    // a violation is the reference's domain_error
+    for (int sl = 0; start && sl < nstart; ++sl) {
+        const double* st = start + sl * 3 * n;
+        const double* sf = start_frac + 2 * sl;
+        if (!(std::isfinite(sf[0]) && std::isfinite(sf[1]) && sf[0] >= 0.0 && sf[1] >= 0.0 && sf[0] + sf[1] <= 1.0)) {
             throw Rejected{};
         }
         for (int ph = 0; ph < 3; ++ph) {
-            (void)check_composition(start + ph * n, n);
-            for (int i = 0; i < n; ++i) { if ((feed[i] > 0) != (start[ph * n + i] > 0)) { throw Rejected{}; } }
+            (void)check_composition(st + ph * n, n);
+            for (int i = 0; i < n; ++i) { if ((feed[i] > 0) != (st[ph * n + i] > 0)) { throw Rejected{}; } }
         }
     }
     solve_vle(m, p, t, feed, so, initial, ninitial, M.base, cnt, final_starts, nfinal);
@@ -1261,13 +1263,13 @@ void solve_max3(const Mixture& m, double p, double t, const double* feed, const 
     // returns true when an attempt closes the solve; supplied: the caller's start (its own
     // attempt budget, pr76_three_phase.hpp:385-394)
     auto run_attempt = [&](const double* l1, const double* l2, const double* fr, const int* sides,
-                           bool supplied = false) -> bool {
+                           bool supplied = false, int index = -1) -> bool {
         int& count = supplied ? continuation : automatic;
         bool& limit = supplied ? cont_limit : auto_limit;
         if (count >= 16) { limit = true; return false; }
         ++count;
         ++M.attempts;
-        if (supplied) { M.start_consumed = true; }
+        if (supplied && index == 0) { M.start_consumed = true; }
         P3Result E;
         iterate_three(m, p, t, z, l1, l2, fr, sides, po, E, cnt);
         if (E.status == P_DISAPPEARANCE) {
@@ -1296,15 +1298,16 @@ void solve_max3(const Mixture& m, double p, double t, const double* feed, const 
         else if (fs == S_NO_INSTABILITY) { M.status = M_THREE; M.three = c; return true; }
         return false;
     };
-    // the caller-supplied start first, over the eight root-side combinations, lower before
-    // upper, phase 0 outermost (pr76_three_phase.hpp:472-498)
-    if (start) {
+    // the caller-supplied starts first, in order, each over the eight root-side combinations,
+    // lower before upper, phase 0 outermost (pr76_three_phase.hpp:472-498); M.start_consumed
+    // records an attempt from start 0 (pr76_pt_continuation_supplied_start_consumed)
+    for (int sl = 0; start && sl < nstart && !cont_limit; ++sl) {
+        const double* st = start + sl * 3 * n;
         double sl1[NCMAX], sl2[NCMAX];
-        if (log_ratio(n, start + n, start, z, sl1) && log_ratio(n, start + 2 * n, start, z, sl2)) {
-            for (int code = 0; code < 8 && !cont_limit; ++code) {
-                const int sd[3] = {code >> 2, (code >> 1) & 1, code & 1};
-                if (run_attempt(sl1, sl2, start_frac, sd, true)) { return; }
-            }
+        if (!(log_ratio(n, st + n, st, z, sl1) && log_ratio(n, st + 2 * n, st, z, sl2))) { continue; }
+        for (int code = 0; code < 8 && !cont_limit; ++code) {
+            const int sd[3] = {code >> 2, (code >> 1) & 1, code & 1};
+            if (run_attempt(sl1, sl2, start_frac + 2 * sl, sd, true, sl)) { return; }
         }
     }
     double src1[NCMAX];
@@ -2361,8 +2364,8 @@ extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* p
                                        int threads, int32_t* status, int32_t* phase_count, double* beta,
                                        double* zfactor, int32_t* branch, double* x, double* ln_phi,
                                        uint32_t* counters) {
-    if (nhint < 0 || nhint > 3 || (nhint > 0 && !hints)) { return 1; }
-    if (nfinal < 0 || nfinal > 3 || (nfinal > 0 && !final_hints)) { return 1; }
+    if (nhint < 0 || nhint > 6 || (nhint > 0 && !hints)) { return 1; }
+    if (nfinal < 0 || nfinal > 6 || (nfinal > 0 && !final_hints)) { return 1; }
     if (nc < 1 || nc > NCMAX || ncell < 0 || (max_phases != 2 && max_phases != 3)) { return 1; }
     const Mixture m = make_mixture(nc, tc, pc, omega, kij);
     const Out o{status, phase_count, beta, zfactor, branch, x, ln_phi, counters};
@@ -2371,9 +2374,9 @@ extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* p
     parallel_cells(ncell, threads, [&](int64_t c) {
         double feed[NCMAX];
         for (int i = 0; i < nc; ++i) { feed[i] = z[i * ncell + c]; }
-        double hw[3][NCMAX], fw[3][NCMAX];
-        const double* hs[3];
-        const double* fs[3];
+        double hw[6][NCMAX], fw[6][NCMAX];
+        const double* hs[6];
+        const double* fs[6];
         int nh = 0, nf = 0;
         for (int h = 0; h < nhint; ++h) {
             if (std::isnan(hints[(h * nc + 0) * ncell + c])) { continue; }
@@ -2407,19 +2410,28 @@ extern "C" int cpuref_flash_batch_warm(int nc, const double* tc, const double* p
 
 // solve_pr76_pt_continuation (pr76_pt_continuation.hpp:194-310) for npath paths at once: path j
 // has feed z[i*npath + j] and ordered states p, t [npoint][npath]. Point k of every path is a
-// max3 flash whose initial and final stability starts are the previous accepted point's phase
-// compositions (deduplicated; carry bit 0) and whose three-phase stage first tries the
-// previous point's three-phase state (carry bit 1). Records are npoint consecutive flash
-// records of npath cells; info[k*npath + j] = incoming hint kind (0 none, else the previous
-// accepted phase count) | carried start count << 4 | carried three-phase start << 8. A path
-// whose states fail the path check, or one of whose points the flash rejects, is rejected at
-// every point (the reference throws for the whole path).
+// max3 flash of the path feed normalised once. Its initial stability starts are the previous
+// accepted point's phase compositions (carry bit 0) then the nfi fallback initial starts
+// fi[h*nc + i], its final-review starts the same compositions then the nff fallback final
+// starts, each list without repeats; its three-phase starts are the previous point's
+// three-phase state (carry bit 1) then, when has_start, the fixed start start[ph*nc + i]
+// with seed[2]. Records are npoint consecutive flash records of npath cells; info[k*npath +
+// j] = incoming hint kind (0 none, else the previous accepted phase count) | carried start
+// count << 4 | carried three-phase start << 8; status flag 0x20000 marks a consumed carried
+// start. A path whose states fail the path check, or one of whose points the flash rejects,
+// is rejected at every point (the reference throws for the whole path).
 extern "C" int cpuref_pt_continuation(int nc, const double* tc, const double* pc, const double* omega,
                                       const double* kij, int64_t npath, int npoint, const double* p, const double* t,
-                                      const double* z, int carry, double fugacity_tolerance, int threads,
-                                      int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
-                                      int32_t* branch, double* x, double* ln_phi, uint32_t* counters, int32_t* info) {
+                                      const double* z, int carry, int nfi, const double* fi, int nff,
+                                      const double* ff, int has_start, const double* start, const double* seed,
+                                      double fugacity_tolerance, int threads, int32_t* status, int32_t* phase_count,
+                                      double* beta, double* zfactor, int32_t* branch, double* x, double* ln_phi,
+                                      uint32_t* counters, int32_t* info) {
     if (nc < 1 || nc > NCMAX || npath < 0 || npoint < 1 || npoint > 4096) { return 1; }
+    if (nfi < 0 || nfi > 3 || nff < 0 || nff > 3 || (nfi > 0 && !fi) || (nff > 0 && !ff) ||
+        (has_start && (!start || !seed))) {
+        return 1;
+    }
     const Mixture m = make_mixture(nc, tc, pc, omega, kij);
     SplitOptions so;
     so.fug_tol = fugacity_tolerance;
@@ -2446,36 +2458,56 @@ extern "C" int cpuref_pt_continuation(int nc, const double* tc, const double* pc
         double prev[3][NCMAX];
         double prev_frac[2] = {0, 0};
         bool prev_three = false;
+        // pr76_pt_continuation_append_unique (pr76_pt_continuation.hpp:95-107)
+        auto append = [&](const double** list, int& count, const double* v) {
+            for (int e = 0; e < count; ++e) {
+                bool same = true;
+                for (int i = 0; i < nc; ++i) { same = same && list[e][i] == v[i]; }
+                if (same) { return; }
+            }
+            list[count++] = v;
+        };
         for (int k = 0; k < npoint && path_ok; ++k) {
             const Out o = out_at(k);
-            const double* starts[3];
-            int ns = 0;
+            const double* initial[6];
+            const double* finals[6];
+            int ni = 0, nf = 0;
             if (carry & 1) {
                 for (int h = 0; h < prev_count; ++h) {
-                    bool dup = false;
-                    for (int e = 0; e < ns; ++e) {
-                        bool same = true;
-                        for (int i = 0; i < nc; ++i) { same = same && starts[e][i] == prev[h][i]; }
-                        dup = dup || same;
-                    }
-                    if (!dup) { starts[ns++] = prev[h]; }
+                    append(initial, ni, prev[h]);
+                    append(finals, nf, prev[h]);
                 }
             }
+            for (int h = 0; h < nfi; ++h) { append(initial, ni, fi + h * nc); }
+            for (int h = 0; h < nff; ++h) { append(finals, nf, ff + h * nc); }
             const bool three = (carry & 2) && prev_three;
-            double st3[3 * NCMAX];
+            double st3[2 * 3 * NCMAX], sf3[4];
+            int ns = 0;
             if (three) {
                 for (int ph = 0; ph < 3; ++ph) { for (int i = 0; i < nc; ++i) { st3[ph * nc + i] = prev[ph][i]; } }
+                sf3[0] = prev_frac[0];
+                sf3[1] = prev_frac[1];
+                ++ns;
+            }
+            if (has_start) {
+                for (int q = 0; q < 3 * nc; ++q) { st3[ns * 3 * nc + q] = start[q]; }
+                sf3[2 * ns] = seed[0];
+                sf3[2 * ns + 1] = seed[1];
+                ++ns;
             }
             info[k * N + j] = prev_count | (((carry & 1) ? prev_count : 0) << 4) | (three ? 0x100 : 0);
             Counters cnt;
             auto M = std::make_unique<Max3Result>();
             try {
-                solve_max3(m, p[k * N + j], t[k * N + j], feed, so, *M, cnt, starts, ns, starts, ns,
-                           three ? st3 : nullptr, three ? prev_frac : nullptr);
+                solve_max3(m, p[k * N + j], t[k * N + j], feed, so, *M, cnt, initial, ni, finals, nf,
+                           ns ? st3 : nullptr, ns ? sf3 : nullptr, ns);
             } catch (const Rejected&) {
                 path_ok = false;
                 break;
             }
+            // the consumed flag reports the carried start only
+            // (carried_three_phase_start_consumed, pr76_pt_continuation.hpp:256-259)
+            M->start_consumed = M->start_consumed && three;
             publish_max3(o, nc, N, j, *M, cnt);
             prev_three = false;
             switch (M->status) {

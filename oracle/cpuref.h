@@ -21,7 +21,7 @@ int cpuref_flash_batch(int nc, const double* tc, const double* pc, const double*
                        int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
                        int32_t* branch, double* x, double* ln_phi, uint32_t* counters);
 
-/* cpuref_flash_batch with warm-start hints: hints[(h*nc + i)*ncell + c] (h < nhint <= 3)
+/* cpuref_flash_batch with warm-start hints: hints[(h*nc + i)*ncell + c] (h < nhint <= 6)
  * appended to each cell's initial stability starts; NaN in component 0: no hint. */
 int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const double* omega,
                               const double* kij, int64_t ncell, const double* p, const double* t,
@@ -31,7 +31,7 @@ int cpuref_flash_batch_hinted(int nc, const double* tc, const double* pc, const 
                               uint32_t* counters);
 
 /* cpuref_flash_batch_hinted plus final-review hints: final_hints[(h*nc + i)*ncell + c]
- * (h < nfinal <= 3) placed before the pair's compositions in the common-tangent review
+ * (h < nfinal <= 6) placed before the pair's compositions in the common-tangent review
  * (final_starts); NaN in component 0: no hint. */
 int cpuref_flash_batch_warm(int nc, const double* tc, const double* pc, const double* omega,
                             const double* kij, int64_t ncell, const double* p, const double* t,
@@ -43,12 +43,15 @@ int cpuref_flash_batch_warm(int nc, const double* tc, const double* pc, const do
 /* solve_pr76_pt_continuation over npath ordered paths of npoint states each: p, t and the
  * outputs are npoint consecutive cell batches of npath cells (point-major); z[i*npath + j] is
  * path j's feed. carry bit 0: carry the previous accepted point's phase compositions as
- * stability starts; bit 1: carry its three-phase state. info[k*npath + j] = incoming hint
- * phase count | carried start count << 4 | carried three-phase start << 8. */
+ * stability starts; bit 1: carry its three-phase state. fi [nfi][nc] / ff [nff][nc]: fallback
+ * initial / final starts (<= 3 each); has_start: fixed three-phase start start[3][nc] with
+ * seed[2]. info[k*npath + j] = incoming hint phase count | carried start count << 4 |
+ * carried three-phase start << 8. */
 int cpuref_pt_continuation(int nc, const double* tc, const double* pc, const double* omega,
                            const double* kij, int64_t npath, int npoint, const double* p, const double* t,
-                           const double* z, int carry, double fugacity_tolerance, int threads,
-                           int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
+                           const double* z, int carry, int nfi, const double* fi, int nff, const double* ff,
+                           int has_start, const double* start, const double* seed, double fugacity_tolerance,
+                           int threads, int32_t* status, int32_t* phase_count, double* beta, double* zfactor,
                            int32_t* branch, double* x, double* ln_phi, uint32_t* counters, int32_t* info);
 
 int cpuref_phase_batch(int nc, const double* tc, const double* pc, const double* omega,

@@ -330,8 +330,8 @@ int ref_flash_batch_warm(int nc, const double* tc, const double* pc, const doubl
                          double fugacity_tolerance, int max_phases, int threads, int32_t* status,
                          int32_t* phase_count, double* beta, double* zfactor, int32_t* branch, double* x,
                          double* ln_phi, uint32_t* counters) {
-    if (nhint < 0 || nhint > 3 || (nhint > 0 && !hints)) { return 1; }
-    if (nfinal < 0 || nfinal > 3 || (nfinal > 0 && !final_hints)) { return 1; }
+    if (nhint < 0 || nhint > 6 || (nhint > 0 && !hints)) { return 1; }
+    if (nfinal < 0 || nfinal > 6 || (nfinal > 0 && !final_hints)) { return 1; }
     try {
         const auto parameters = make_parameters(nc, tc, pc, omega, kij);
         const auto model = th::Pr76Phase<double>::from_parameters(parameters);
@@ -559,14 +559,15 @@ int ref_assembly(int nc, const double* tc, const double* pc, const double* omega
 }
 
 // solve_pr76_pt_continuation (pr76_pt_continuation.hpp:194-310) over npath ordered paths;
-// layout and info word as cpuref_pt_continuation (oracle/cpuref.h). A path the reference
-// rejects (it throws) is published as rejected (0x2000) at every point.
+// layout, options and info word as cpuref_pt_continuation (oracle/cpuref.h). A path the
+// reference rejects (it throws) is published as rejected (0x2000) at every point.
 int ref_pt_continuation(int nc, const double* tc, const double* pc, const double* omega, const double* kij,
                         int64_t npath, int npoint, const double* p, const double* t, const double* z, int carry,
-                        double fugacity_tolerance, int threads, int32_t* status, int32_t* phase_count,
-                        double* beta, double* zfactor, int32_t* branch, double* x, double* ln_phi,
-                        uint32_t* counters, int32_t* info) {
-    if (npath < 0 || npoint < 1) { return 1; }
+                        int nfi, const double* fi, int nff, const double* ff, int has_start, const double* start,
+                        const double* seed, double fugacity_tolerance, int threads, int32_t* status,
+                        int32_t* phase_count, double* beta, double* zfactor, int32_t* branch, double* x,
+                        double* ln_phi, uint32_t* counters, int32_t* info) {
+    if (npath < 0 || npoint < 1 || nfi < 0 || nff < 0) { return 1; }
     try {
         const auto parameters = make_parameters(nc, tc, pc, omega, kij);
         const auto model = th::Pr76Phase<double>::from_parameters(parameters);
@@ -576,6 +577,18 @@ int ref_pt_continuation(int nc, const double* tc, const double* pc, const double
                            branch + k * 3 * N, x + k * 3 * nc * N, ln_phi + k * 3 * nc * N,
                            counters ? counters + k * 4 * N : nullptr};
         };
+        fl::Pr76PtContinuationOptions options;
+        options.point_options.two_phase.iteration.fugacity_tolerance = fugacity_tolerance;
+        options.carry_previous_phase_compositions = (carry & 1) != 0;
+        options.carry_previous_three_phase_state = (carry & 2) != 0;
+        for (int h = 0; h < nfi; ++h) { options.fallback_initial_starts.emplace_back(fi + h * nc, fi + (h + 1) * nc); }
+        for (int h = 0; h < nff; ++h) { options.fallback_final_starts.emplace_back(ff + h * nc, ff + (h + 1) * nc); }
+        if (has_start) {
+            fl::Pr76PtThreePhaseStart st;
+            for (int ph = 0; ph < 3; ++ph) { st.compositions[ph].assign(start + ph * nc, start + (ph + 1) * nc); }
+            st.phase_fraction_seed = {seed[0], seed[1]};
+            options.point_options.three_phase_starts.push_back(st);
+        }
         parallel_cells(npath, threads, [&](auto&& run) {
             fl::Pr76VleEvaluator evaluator(model);
             run([&](int64_t j) {
@@ -583,10 +596,6 @@ int ref_pt_continuation(int nc, const double* tc, const double* pc, const double
                 for (int k = 0; k < npoint; ++k) { path[k] = {p[k * N + j], t[k * N + j]}; }
                 std::vector<double> feed(nc);
                 for (int i = 0; i < nc; ++i) { feed[i] = z[i * npath + j]; }
-                fl::Pr76PtContinuationOptions options;
-                options.point_options.two_phase.iteration.fugacity_tolerance = fugacity_tolerance;
-                options.carry_previous_phase_compositions = (carry & 1) != 0;
-                options.carry_previous_three_phase_state = (carry & 2) != 0;
                 try {
                     const auto result = fl::solve_pr76_pt_continuation(path, feed, evaluator, options);
                     for (int k = 0; k < npoint; ++k) {

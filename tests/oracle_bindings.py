@@ -21,8 +21,8 @@ _FLASH_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.
 _PHASE_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P] + [_P] * 5
 _FLASH_HINT_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P, ctypes.c_double,
                     ctypes.c_int, ctypes.c_int] + [_P] * 8
-_CONT_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_double,
-              ctypes.c_int] + [_P] * 9
+_CONT_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P,
+              ctypes.c_int, _P, ctypes.c_int, _P, _P, ctypes.c_double, ctypes.c_int] + [_P] * 9
 _FLASH_WARM_ARGS = [ctypes.c_int, _P, _P, _P, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int, _P, ctypes.c_int, _P,
                     ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_P] * 8
 
@@ -79,7 +79,7 @@ def _mix_arrays(mix):
 def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which="cpuref", hints=None,
           final_hints=None):
     """Run the checker flash on host arrays; returns a dict of SoA arrays. ``hints``
-    ([nhint <= 3, nc, ncell], NaN in component 0 = absent) are warm-start compositions
+    ([nhint <= 6, nc, ncell], NaN in component 0 = absent) are warm-start compositions
     appended to the initial stability starts; ``final_hints`` (same layout) are the extra
     starts of the common-tangent review."""
     L = lib(which)
@@ -104,7 +104,7 @@ def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which
         hints = np.zeros((0, nc, n)) if hints is None else np.ascontiguousarray(hints, dtype=np.float64)
         final_hints = np.ascontiguousarray(final_hints, dtype=np.float64)
         for h in (hints, final_hints):
-            assert h.ndim == 3 and h.shape[1:] == (nc, n) and h.shape[0] <= 3
+            assert h.ndim == 3 and h.shape[1:] == (nc, n) and h.shape[0] <= 6
         fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_warm")
         rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), int(hints.shape[0]),
                 _ptr(hints), int(final_hints.shape[0]), _ptr(final_hints), float(fugacity_tolerance), int(max_phases),
@@ -113,7 +113,7 @@ def flash(mix, p, t, z, fugacity_tolerance=1e-11, max_phases=3, threads=0, which
                 _ptr(out["counters"]))
     elif hints is not None:
         hints = np.ascontiguousarray(hints, dtype=np.float64)
-        assert hints.ndim == 3 and hints.shape[1:] == (nc, n) and hints.shape[0] <= 3
+        assert hints.ndim == 3 and hints.shape[1:] == (nc, n) and hints.shape[0] <= 6
         fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_flash_batch_hinted")
         rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, _ptr(p), _ptr(t), _ptr(z), int(hints.shape[0]),
                 _ptr(hints), float(fugacity_tolerance), int(max_phases), host_threads(threads), _ptr(out["status"]),
@@ -258,12 +258,14 @@ def phase_roots(mix):
     return roots
 
 
-def pt_continuation(mix, p, t, z, carry=3, fugacity_tolerance=1e-11, threads=0, which="cpuref"):
+def pt_continuation(mix, p, t, z, carry=3, fugacity_tolerance=1e-11, threads=0, which="cpuref",
+                    fallback_initial=None, fallback_final=None, start=None, seed=(1.0 / 3.0, 1.0 / 3.0)):
     """Ordered-path continuation (solve_pr76_pt_continuation) of npath paths: p, t are
     [npoint, npath] states, z [nc, npath] feeds. Returns the flash record arrays with a
     leading point axis plus ``info`` [npoint, npath] = incoming hint phase count | carried
     start count << 4 | carried three-phase start << 8. carry: bit 0 carries phase
-    compositions, bit 1 the three-phase state."""
+    compositions, bit 1 the three-phase state; fallback_initial / fallback_final: [k <= 3, nc]
+    starts shared by every path; start: a fixed three-phase start [3, nc] with ``seed``."""
     L = lib(which)
     if L is None:
         raise RuntimeError(f"checker library '{which}' is not built")
@@ -274,6 +276,11 @@ def pt_continuation(mix, p, t, z, carry=3, fugacity_tolerance=1e-11, threads=0, 
     nc, n = z.shape
     m = p.shape[0]
     assert p.shape == (m, n) and t.shape == (m, n)
+    fi = np.zeros((0, nc)) if fallback_initial is None else np.ascontiguousarray(fallback_initial, dtype=np.float64)
+    ff = np.zeros((0, nc)) if fallback_final is None else np.ascontiguousarray(fallback_final, dtype=np.float64)
+    st = np.zeros((3, nc)) if start is None else np.ascontiguousarray(start, dtype=np.float64)
+    sd = np.ascontiguousarray(seed, dtype=np.float64)
+    assert fi.shape[1:] == (nc,) and ff.shape[1:] == (nc,) and st.shape == (3, nc) and sd.shape == (2,)
     out = {
         "status": np.zeros((m, n), np.int32),
         "phase_count": np.zeros((m, n), np.int32),
@@ -286,10 +293,11 @@ def pt_continuation(mix, p, t, z, carry=3, fugacity_tolerance=1e-11, threads=0, 
         "info": np.zeros((m, n), np.int32),
     }
     fn = getattr(L, ("cpuref" if which == "cpuref" else "ref") + "_pt_continuation")
-    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, m, _ptr(p), _ptr(t), _ptr(z), int(carry),
-            float(fugacity_tolerance), host_threads(threads), _ptr(out["status"]), _ptr(out["phase_count"]),
-            _ptr(out["beta"]), _ptr(out["zfactor"]), _ptr(out["branch"]), _ptr(out["x"]), _ptr(out["ln_phi"]),
-            _ptr(out["counters"]), _ptr(out["info"]))
+    rc = fn(nc, _ptr(tc), _ptr(pc), _ptr(om), _ptr(kij), n, m, _ptr(p), _ptr(t), _ptr(z), int(carry), fi.shape[0],
+            _ptr(fi), ff.shape[0], _ptr(ff), int(start is not None), _ptr(st), _ptr(sd), float(fugacity_tolerance),
+            host_threads(threads), _ptr(out["status"]), _ptr(out["phase_count"]), _ptr(out["beta"]),
+            _ptr(out["zfactor"]), _ptr(out["branch"]), _ptr(out["x"]), _ptr(out["ln_phi"]), _ptr(out["counters"]),
+            _ptr(out["info"]))
     if rc != 0:
         raise RuntimeError(f"{which}_pt_continuation failed ({rc})")
     return out
