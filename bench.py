@@ -104,9 +104,11 @@ def cpu_baseline(mix, p, t, z, sample: int, sample_1core: int = 8192, repeats: i
 # HBM bytes of one Newton evaluation at nc=6 on 100x100x50, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE counters (k_asm_cell + k_asm_rows), each scaled by what the counter reads for
 # a plain 1 GiB streaming kernel (profiles/assembly_2p_nc6_100x100x50_kernel_stats.md).
-# HBM bytes of one FP64 flash of the 1M-cell nc=6 grid: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# summed over the launch sequence (profiles/flash_nc6_1M_hbm_traffic.md).
-FLASH_FP64_TRAFFIC_BYTES = 2369085 * 2.0 * 1024 + 12783116 * 1.0 * 1024
+# HBM bytes of one flash of the 1M-cell nc=6 grid: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# summed over the launch sequence (profiles/flash_nc6_1M_hbm_traffic.md), FP64 and
+# decision-exact mixed precision.
+FLASH_FP64_TRAFFIC_BYTES = 2164755 * 2.0 * 1024 + 1735990 * 1.0 * 1024
+FLASH_EXACT_TRAFFIC_BYTES = 3460919 * 2.0 * 1024 + 1903868 * 1.0 * 1024
 ASSEMBLY_TRAFFIC_BYTES = (36615 + 1072624) * 2.0 * 1024 + (728516 + 2578166) * 1.0 * 1024
 
 
@@ -582,7 +584,9 @@ def main(argv=None):
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved_mixed if exact_ok else achieved) / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": ((FLASH_EXACT_TRAFFIC_BYTES if exact_ok else FLASH_FP64_TRAFFIC_BYTES)
+                        if ncell == 1048576 and nc == 6 else None),
+            "traffic_source": "profiles/flash_nc6_1M_hbm_traffic.md",
             "kernel": ("decision-exact mixed flash, whole launch sequence" if exact_ok else
                        "staged FP64 flash, whole launch sequence"),
             "kernel_ms": exact_ms if exact_ok else fp64_ms,
