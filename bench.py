@@ -162,6 +162,19 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     torch.cuda.synchronize()
     plan_ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
     plan.close()
+    # Schur reduction of the assembled system to (nc+1) x (nc+1) blocks
+    red = A.reduce(g, nc, out)
+    for _ in range(warmup):
+        A.reduce(g, nc, out, out=red)
+    torch.cuda.synchronize()
+    for k in range(steps):
+        e0[k].record(stream)
+        A.reduce(g, nc, out, out=red)
+        e1[k].record(stream)
+    torch.cuda.synchronize()
+    red_ms = float(np.mean([e0[k].elapsed_time(e1[k]) for k in range(steps)]))
+    red_singular = int(torch.sum(red["status"] != 0).item())
+    del red
     # the post-SNES phase-transition scan of the same state: inventory -> feed, flash, decision
     from mpmc_hnu_amd import scan as S
 
@@ -183,8 +196,16 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
     nnz = ncell * Q * Q + 2 * nface * (nc + 1) * Q
     algo_bytes = 8.0 * (ncell * (Q + nc + 3) + nface * 2 + nnz + ncell * Q)
     gbs = algo_bytes / (ms * 1e-3) / 1e9
+    # reduction: reads the unreduced system (and each neighbour's M once per face side),
+    # writes M, the reduced residual and blocks
+    P = nc + 1
+    red_bytes = 8.0 * (ncell * (Q * Q + Q) + 2 * nface * (P * Q + nc * (P + 1))
+                       + ncell * (nc * (P + 1) + P + P * P) + 2 * nface * P * P)
+    red_gbs = red_bytes / (red_ms * 1e-3) / 1e9
     print(f"[bench] assembly: {ms:.3f} ms per Newton evaluation, {gbs:.0f} GB/s algorithmic "
-          f"({ncell} cells, {nface} faces); graph replay {plan_ms:.3f} ms", file=sys.stderr, flush=True)
+          f"({ncell} cells, {nface} faces); graph replay {plan_ms:.3f} ms; Schur reduction to "
+          f"{P}x{P} blocks {red_ms:.3f} ms ({red_gbs:.0f} GB/s, {red_singular} singular cells)",
+          file=sys.stderr, flush=True)
     g.close()
     m.close()
     out = {"grid": list(dims), "cells": ncell, "faces": int(nface), "unknowns_per_cell": Q, "ms": ms,
@@ -192,6 +213,10 @@ def assembly_leg(dims, steps: int, warmup: int) -> dict:
            "algorithmic_bytes": algo_bytes, "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS,
            "frac": gbs / PEAK_HBM_GBS,
            "what": "residual + block Jacobian (diagonal and off-diagonal Q x Q blocks), two-phase natural variables",
+           "reduced": {"ms": red_ms, "block": P, "algorithmic_bytes": red_bytes, "achieved_gbs": red_gbs,
+                       "frac": red_gbs / PEAK_HBM_GBS, "singular_cells": red_singular,
+                       "what": "Schur reduction of the assembled system to (nc+1) x (nc+1) blocks "
+                               "(mpmc_assembly_reduce_2p)"},
            "transition_scan": dict(scan, what="post-SNES phase-transition scan of the same state (inventory, feed, "
                                                "flash, per-cell decision; host clock, includes the summary sync)")}
     if tuple(dims) == (100, 100, 50):

@@ -62,9 +62,9 @@ enum mpmc_split_status { MPMC_SPLIT_SINGLE = 0, MPMC_SPLIT_TWO = 1, MPMC_SPLIT_U
  *   SPLIT_CANDIDATE      the base split selected a converged pair (PtSplitResult::candidate())
  *   MAX3_ATTEMPTED       the three-phase stage ran at least one attempt
  *                        (Pr76PtMax3Result::attempts not empty)
- *   START_CONSUMED       an attempt ran from the caller-supplied three-phase start
- *                        (pr76_pt_continuation_supplied_start_consumed,
- *                        pr76_pt_continuation.hpp:183-190) */
+ *   START_CONSUMED       an attempt ran from the three-phase start a continuation carried
+ *                        from the previous path point (carried_three_phase_start_consumed,
+ *                        pr76_pt_continuation.hpp:183-190,256-259) */
 #define MPMC_FLAG_SPLIT_DISAPPEARANCE 0x4000
 #define MPMC_FLAG_SPLIT_CANDIDATE 0x8000
 #define MPMC_FLAG_MAX3_ATTEMPTED 0x10000
@@ -454,6 +454,32 @@ int mpmc_assembly_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_a
 int mpmc_inventory_2p(mpmc_mixture_t mixture, mpmc_tpfa_grid_t grid, const mpmc_assembly_model* model,
                       const double* q, const int32_t* branch, int32_t* cell_status, double* inventory,
                       int32_t* domain_error, void* stream);
+
+/* Schur-reduced system with (nc+1) x (nc+1) blocks (the block-CSR Jacobian a solver on
+ * nc+1 unknowns per cell takes). No reference counterpart: the reference hands the
+ * unreduced Q x Q system to PETSc (two_phase_snes_assembly.hpp:1719-1784); this is an exact
+ * algebraic transform of that system, not an approximation. Each cell's unknowns split into
+ * nc secondary columns of q (secondary[k], host array, NULL: x_0 of component 0 and the
+ * nc-1 independent compositions of phase 1) and the nc+1 others, the primaries, in q order.
+ * The cell's equilibrium rows give dx_S = -(M_P dx_P + M_r) with
+ * M = J_ES^-1 [J_EP | r_E]; eliminating dx_S from the conservation rows leaves
+ *   A_cc = J_CP - J_CS M_P(c), A_cn = O_CP - O_CS M_P(n),
+ *   b_c = r_C - J_CS M_r(c) - sum_n O_CS M_r(n),       A dx_P = -b.
+ * Inputs: the outputs of mpmc_assembly_2p on this grid. Outputs (device, cell fastest):
+ *   residual[nc+1][ncell] = b, diag[(nc+1)^2][ncell] = A_cc (row-major),
+ *   offdiag[(nc+1)^2][max_degree][ncell] = A_cn in the slots of the unreduced layout (zeros
+ *   past a cell's degree), back[nc][nc+2][ncell] = M (column nc+1 is M_r), status[ncell]:
+ *   1 where J_ES is singular to working precision (pivot below 64 eps nc |J_ES|; that
+ *   cell's M and reduced entries are NaN). No allocation; two kernels on the stream. */
+typedef struct mpmc_reduced_outputs {
+    int32_t* status;
+    double* residual;
+    double* diag;
+    double* offdiag;
+    double* back;
+} mpmc_reduced_outputs;
+int mpmc_assembly_reduce_2p(mpmc_tpfa_grid_t grid, int nc, const int32_t* secondary,
+                            const mpmc_assembly_outputs* in, const mpmc_reduced_outputs* out, void* stream);
 
 /* Newton-evaluation plan: one mpmc_assembly_2p evaluation on these exact device buffers
  * (the per-cell selected-phase property closure, then the TPFA rows), captured once into a

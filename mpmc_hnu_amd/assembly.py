@@ -128,6 +128,42 @@ def assemble(mixture: Mixture, grid: Grid, model, q, branch, prev, dt: float, ou
     return out
 
 
+def default_secondary(nc: int) -> list:
+    """The secondary q columns of mpmc_assembly_reduce_2p's default split: x_0 of component 0
+    and the nc-1 independent compositions of phase 1."""
+    return [3] + [3 + (nc - 1) + k for k in range(nc - 1)]
+
+
+def reduce(grid: Grid, nc: int, assembled: dict, secondary=None, out: dict | None = None, stream=None) -> dict:
+    """Schur reduction of an assembled two-phase system to (nc+1) x (nc+1) blocks
+    (mpmc_assembly_reduce_2p): residual [nc+1, n], diag [(nc+1)^2, n], offdiag
+    [(nc+1)^2, max_degree, n], back [nc, nc+2, n] (dx_S = -(M_P dx_P + M_r)), status [n]."""
+    torch = _torch()
+    lib = native.library()
+    dev = assembled["residual"].device
+    n = grid.ncell
+    P = nc + 1
+    if out is None:
+        out = {
+            "status": torch.empty(n, dtype=torch.int32, device=dev),
+            "residual": torch.empty((P, n), dtype=torch.float64, device=dev),
+            "diag": torch.empty((P * P, n), dtype=torch.float64, device=dev),
+            "offdiag": torch.empty((P * P, max(1, grid.max_degree), n), dtype=torch.float64, device=dev),
+            "back": torch.empty((nc, nc + 2, n), dtype=torch.float64, device=dev),
+        }
+    i = native.AssemblyOutputs(assembled["cell_status"].data_ptr(), assembled["residual"].data_ptr(),
+                               assembled["diag"].data_ptr(), assembled["offdiag"].data_ptr())
+    o = native.ReducedOutputs(out["status"].data_ptr(), out["residual"].data_ptr(), out["diag"].data_ptr(),
+                              out["offdiag"].data_ptr(), out["back"].data_ptr())
+    sec = None
+    if secondary is not None:
+        arr = (ctypes.c_int32 * nc)(*[int(v) for v in secondary])
+        sec = ctypes.cast(arr, ctypes.c_void_p)
+    native.check(lib.mpmc_assembly_reduce_2p(grid.h, int(nc), sec, ctypes.byref(i), ctypes.byref(o),
+                                             _stream_ptr(stream)), "assembly_reduce_2p")
+    return out
+
+
 class NewtonPlan:
     """One Newton evaluation (mpmc_assembly_2p) on fixed device buffers, captured once into
     a HIP graph (mpmc_assembly_plan_*). Write the next iterate into ``q`` (and ``branch``,

@@ -2,6 +2,7 @@
 // phase evaluation. Argument checks mirror the reference's input contracts; numerical
 // outcomes are per-cell statuses written by the kernels.
 #include "assembly_device.hpp"
+#include "assembly_reduce.hpp"
 #include "closure_device.hpp"
 #include "continuation_kernels.hpp"
 #include "flash_pipeline.hpp"
@@ -1237,6 +1238,50 @@ int mpmc_inventory_2p(mpmc_mixture_t m, mpmc_tpfa_grid_t g, const mpmc_assembly_
                       void* stream) {
     const mpmc_assembly_outputs out{cell_status, inventory, nullptr, nullptr};
     return assembly_impl(m, g, md, q, branch, nullptr, 1.0, &out, domain_error, true, stream);
+}
+
+// ---------------------------------------------------------------- Schur-reduced system
+int mpmc_assembly_reduce_2p(mpmc_tpfa_grid_t g, int nc, const int32_t* secondary, const mpmc_assembly_outputs* in,
+                            const mpmc_reduced_outputs* out, void* stream) {
+    if (!g || !in || !out) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly_reduce: null argument"); }
+    if (nc < 2 || nc > MPMC_MAX_COMPONENTS) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly_reduce: nc must be 2..12"); }
+    const int Q = 2 * nc + 1;
+    mpmc::dev::RedColumns cols{};
+    bool is_sec[2 * MPMC_MAX_COMPONENTS + 1] = {};
+    for (int k = 0; k < nc; ++k) {
+        // default: x_0 of component 0 and the independent compositions of phase 1
+        const int col = secondary ? secondary[k] : (k == 0 ? 3 : 3 + (nc - 1) + (k - 1));
+        if (col < 0 || col >= Q || is_sec[col]) {
+            return fail(MPMC_E_INVALID_ARGUMENT, "assembly_reduce: secondary columns must be nc distinct q columns");
+        }
+        is_sec[col] = true;
+    }
+    for (int col = 0, s = 0, p = 0; col < Q; ++col) {
+        if (is_sec[col]) { cols.sec[s++] = col; }
+        else { cols.prim[p++] = col; }
+    }
+    const void* ptrs[] = {in->residual, in->diag, in->offdiag, out->status, out->residual, out->diag, out->offdiag,
+                          out->back};
+    for (const void* ptr : ptrs) {
+        if (!is_device_ptr(ptr)) { return fail(MPMC_E_INVALID_ARGUMENT, "assembly_reduce: buffers must be device memory"); }
+    }
+    if (g->ncell == 0) { return MPMC_OK; }
+    const mpmc::dev::RedIo io{g->ncell, g->max_degree, g->slot_other, in->residual, in->diag, in->offdiag,
+                              out->back, out->residual, out->diag, out->offdiag, out->status};
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((g->ncell + 63) / 64, 65536));
+    switch (nc) {
+#define MPMC_CASE(N)                                                                                              \
+    case N:                                                                                                       \
+        hipLaunchKernelGGL(mpmc::dev::k_red_cell<N>, dim3(blocks), dim3(64), 0, s, cols, io);                     \
+        hipLaunchKernelGGL(mpmc::dev::k_red_rows<N>, dim3(blocks), dim3(64), 0, s, cols, io);                     \
+        break;
+        MPMC_FOR_EACH_NC2(MPMC_CASE)
+#undef MPMC_CASE
+    default: return fail(MPMC_E_INVALID_ARGUMENT, "assembly_reduce: unsupported nc");
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MPMC_OK : hip_fail(e, "assembly_reduce launch");
 }
 
 // ---------------------------------------------------------------- Newton-evaluation plan

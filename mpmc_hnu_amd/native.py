@@ -49,6 +49,7 @@ EXPORTED = (
     "mpmc_assembly_default_model",
     "mpmc_assembly_2p",
     "mpmc_inventory_2p",
+    "mpmc_assembly_reduce_2p",
     "mpmc_assembly_plan_create",
     "mpmc_assembly_plan_launch",
     "mpmc_assembly_plan_domain_error",
@@ -172,6 +173,10 @@ class AssemblyOutputs(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("cell_status", "residual", "diag", "offdiag")]
 
 
+class ReducedOutputs(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("status", "residual", "diag", "offdiag", "back")]
+
+
 class MpmcError(RuntimeError):
     pass
 
@@ -247,6 +252,8 @@ def library() -> ctypes.CDLL:
     lib.mpmc_assembly_default_model.restype = None
     lib.mpmc_assembly_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, ctypes.c_double,
                                      ctypes.POINTER(AssemblyOutputs), ctypes.POINTER(ctypes.c_int32), P]
+    lib.mpmc_assembly_reduce_2p.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(AssemblyOutputs),
+                                            ctypes.POINTER(ReducedOutputs), P]
     lib.mpmc_inventory_2p.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, P,
                                       ctypes.POINTER(ctypes.c_int32), P]
     lib.mpmc_assembly_plan_create.argtypes = [P, P, ctypes.POINTER(AssemblyModel), P, P, P, ctypes.c_double,
