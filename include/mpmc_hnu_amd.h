@@ -175,8 +175,8 @@ int mpmc_flash_batch_ws(mpmc_mixture_t mixture, int64_t ncell, const double* p, 
  * until the process ends; the call itself allocates no device memory. No counterpart in
  * the reference: it is for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). Limit: ncell with
- * mpmc_flash_workspace_bytes(nc, ncell) <= MPMC_STATIC_SCRATCH_BYTES (about 40,000 cells
- * at nc = 6, 20,000 at nc = 12); larger batches are rejected with MPMC_E_INVALID_ARGUMENT
+ * mpmc_flash_workspace_bytes(nc, ncell) <= MPMC_STATIC_SCRATCH_BYTES (about 38,000 cells
+ * at nc = 6, 19,000 at nc = 12); larger batches are rejected with MPMC_E_INVALID_ARGUMENT
  * before any device work. Every call on one device shares that device's array: calls
  * that overlap on one device (other streams, other host threads) race and must be
  * serialised by the caller. Each call writes every workspace word it reads, so no state
