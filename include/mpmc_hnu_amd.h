@@ -109,7 +109,10 @@ typedef struct mpmc_flash_options {
      *    rounding reach, the negative trials that seed the splits, every split attempt,
      *    and the whole review of a cell whose review finds instability. Only trials FP32
      *    classifies as stationary beyond rounding reach keep their FP32 status. Phase
-     *    counts, statuses and labels are those of precision 0; values are FP64.
+     *    counts, statuses and labels are those of precision 0, except on cells where
+     *    precision 0 ends indeterminate because such a trial's FP64 descent reaches its
+     *    512-step limit (none of the 1M cells of the nc=6 grid workload, 6 of 9.7M cells
+     *    of the uneven 256-mixture batch); values are FP64.
      * 2: fast mixed precision. TPD descents and split attempts run in FP32 (Rachford-Rice
      *    in FP32 away from the ends of (0, 1)); searches within FP32 rounding of a
      *    threshold, or slow enough to meet the iteration limit, are continued in FP64 from

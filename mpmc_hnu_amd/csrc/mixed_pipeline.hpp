@@ -313,7 +313,12 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd32(Pipe<NC> P, const
         const int64_t task = fetch_task(counter, want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else { active = tpd32_begin<NC>(P, smix, stage, static_cast<int>(task / nslot), task % nslot, L); }
+            else {
+                int k;
+                int64_t slot;
+                task_index<MPMC_TPD_ORDER>(task, nslot, tk, k, slot);
+                active = tpd32_begin<NC>(P, smix, stage, k, slot, L);
+            }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
@@ -579,7 +584,12 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split32(Pipe<NC> P, con
         const int64_t task = fetch_task(&P.g[G_TASK_SPLIT32], want);
         if (want) {
             if (task >= total) { exhausted = true; }
-            else { active = split32_begin<NC>(P, smix, static_cast<int>(task / nu), task % nu, L); }
+            else {
+                int k;
+                int64_t u;
+                task_index<MPMC_SPLIT_ORDER>(task, nu, kAttempts, k, u);
+                active = split32_begin<NC>(P, smix, k, u, L);
+            }
         }
         if (!__any(active)) {
             if (!__any(!exhausted)) { break; }
