@@ -401,7 +401,7 @@ struct TpdLane {
 // stored point and iteration count; mode 2: the trials marked T_REDO, from their start
 // (mixed precision).
 template <int NC>
-__device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int mode, TpdLane<NC>& L) {
+__device__ __forceinline__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int mode, TpdLane<NC>& L) {
     const int64_t n = P.n;
     int64_t c;
     int32_t* st_arr = stage == 0 ? P.tst : P.fst;
@@ -495,7 +495,7 @@ __device__ bool tpd_begin(const Pipe<NC>& P, int stage, int k, int64_t slot, int
 }
 
 template <int NC>
-__device__ void tpd_finish(const Pipe<NC>& P, int stage, TpdLane<NC>& L, int status) {
+__device__ __forceinline__ void tpd_finish(const Pipe<NC>& P, int stage, TpdLane<NC>& L, int status) {
     const int64_t n = P.n;
     const int64_t idx = L.k * n + L.slot;
     (stage == 0 ? P.tst : P.fst)[idx] = status | (L.pt.branch << 8);
@@ -536,6 +536,79 @@ __device__ __forceinline__ int tpd_check(TpdLane<NC>& L) {
     return -1;
 }
 
+// One step of an active TPD lane: one evaluation, or one rejected backtrack without
+// evaluation. Returns false when the lane's trial has ended (recorded by tpd_finish).
+template <int NC, bool kN>
+__device__ __forceinline__ bool tpd_step(const Pipe<NC>& P, const Mix<NC>* smix, int stage, TpdLane<NC>& L) {
+    // one evaluation (or one rejected backtrack without evaluation)
+    bool do_eval = true;
+    if (L.mode == TM_BT) {
+        double largest = -kInf;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            if (L.z[i] > 0) {
+                L.cand[i] = L.lw[i] + L.alpha * L.dir[i];
+                largest = smax(largest, L.cand[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { L.cand[i] = L.z[i] > 0 ? exp(L.cand[i] - largest) : 0.0; }
+        const double s = ksum<NC>(L.cand);
+        bool positive = finite(s) && s > 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            L.cand[i] /= s;
+            positive = positive && (L.z[i] == 0 || L.cand[i] > 0);
+        }
+        do_eval = positive;
+    }
+    TpdPoint<NC> nx;
+    int e = ISSUE_NONE;
+    bool rejected = false;
+    if (do_eval) {
+        ++L.evals;
+        Activity<NC> ph;
+        e = stability_provider<NC, kN>(smix[L.mid], L.p, L.t, L.cand, ph);
+        if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
+        if (e == REJECTED || rejected) {
+            atomicOr(&P.flags[L.cell], CF_REJECTED);
+            tpd_finish<NC>(P, stage, L, T_PROPERTY_FAILURE);
+            return false;
+        }
+    }
+    int done = -1;
+    if (L.mode == TM_START) {
+        if (e != ISSUE_NONE) { done = T_PROPERTY_FAILURE; }
+        else { L.pt = nx; done = tpd_check<NC>(L); }
+    } else {
+        bool ok = false;
+        if (do_eval && e == ISSUE_NONE) {
+            ok = tpd_negative(nx.value, nx.guard, L.tol);
+            if (!ok) {
+                // stability_accept_step (pt_stability.hpp:312-326)
+                const double pred = -L.alpha * L.slope;
+                const double gd = L.pt.guard + nx.guard;
+                if (pred <= gd) { ok = nx.value <= L.pt.value + gd && L.alpha / L.scale > 0 && nx.stat < L.pt.stat; }
+                else { ok = nx.value <= L.pt.value - 1e-4 * pred; }
+            }
+        }
+        if (ok) {
+            L.pt = nx;
+            ++L.it;
+            done = tpd_check<NC>(L);
+        } else {
+            ++L.bt;
+            L.alpha *= 0.5;
+            if (L.bt >= 32) { done = T_LINE_SEARCH_FAILED; }
+        }
+    }
+    if (done >= 0) {
+        tpd_finish<NC>(P, stage, L, done);
+        return false;
+    }
+    return true;
+}
+
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage,
                                                               int resume) {
@@ -569,85 +642,21 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
             continue;
         }
         if (!active) { continue; }
-        // one evaluation (or one rejected backtrack without evaluation)
-        bool do_eval = true;
-        if (L.mode == TM_BT) {
-            double largest = -kInf;
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                if (L.z[i] > 0) {
-                    L.cand[i] = L.lw[i] + L.alpha * L.dir[i];
-                    largest = smax(largest, L.cand[i]);
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < NC; ++i) { L.cand[i] = L.z[i] > 0 ? exp(L.cand[i] - largest) : 0.0; }
-            const double s = ksum<NC>(L.cand);
-            bool positive = finite(s) && s > 0;
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                L.cand[i] /= s;
-                positive = positive && (L.z[i] == 0 || L.cand[i] > 0);
-            }
-            do_eval = positive;
-        }
-        TpdPoint<NC> nx;
-        int e = ISSUE_NONE;
-        bool rejected = false;
-        if (do_eval) {
-            ++L.evals;
-            Activity<NC> ph;
-            e = stability_provider<NC, kN>(smix[L.mid], L.p, L.t, L.cand, ph);
-            if (e == ISSUE_NONE) { e = tpd_point<NC>(L.cand, L.z, ph, L.ref, nx, rejected); }
-            if (e == REJECTED || rejected) {
-                atomicOr(&P.flags[L.cell], CF_REJECTED);
-                tpd_finish<NC>(P, stage, L, T_PROPERTY_FAILURE);
-                active = false;
-                continue;
-            }
-        }
-        int done = -1;
-        if (L.mode == TM_START) {
-            if (e != ISSUE_NONE) { done = T_PROPERTY_FAILURE; }
-            else { L.pt = nx; done = tpd_check<NC>(L); }
-        } else {
-            bool ok = false;
-            if (do_eval && e == ISSUE_NONE) {
-                ok = tpd_negative(nx.value, nx.guard, L.tol);
-                if (!ok) {
-                    // stability_accept_step (pt_stability.hpp:312-326)
-                    const double pred = -L.alpha * L.slope;
-                    const double gd = L.pt.guard + nx.guard;
-                    if (pred <= gd) { ok = nx.value <= L.pt.value + gd && L.alpha / L.scale > 0 && nx.stat < L.pt.stat; }
-                    else { ok = nx.value <= L.pt.value - 1e-4 * pred; }
-                }
-            }
-            if (ok) {
-                L.pt = nx;
-                ++L.it;
-                done = tpd_check<NC>(L);
-            } else {
-                ++L.bt;
-                L.alpha *= 0.5;
-                if (L.bt >= 32) { done = T_LINE_SEARCH_FAILED; }
-            }
-        }
-        if (done >= 0) {
-            tpd_finish<NC>(P, stage, L, done);
-            active = false;
-        }
+        active = tpd_step<NC, kN>(P, smix, stage, L);
     }
 }
 
 // ------------------------------------------------------------------ stability reduce
+// The initial stability result of cell c from its trials (status, budget, fallback) and,
+// for an unstable cell that continues on the staged path, its compact index u and
+// witnesses. Returns u, or -1.
 template <int NC>
-__global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
+__device__ __forceinline__ int64_t init_reduce_cell(const Pipe<NC>& P, int64_t c) {
     const int64_t n = P.n;
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
-         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        if (P.flags[c] & CF_REJECTED) { continue; }
+    {
+        if (P.flags[c] & CF_REJECTED) { return -1; }
         const int32_t info = P.refinfo[c];
-        if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; continue; }
+        if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; return -1; }
         const int active = active_count<NC>(info);
         const int ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info);
         uint64_t evals = 1;
@@ -675,7 +684,17 @@ __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
             const int nw = nwit < kWitnesses ? nwit : kWitnesses;
             P.u_nwit[u] = nw;
             for (int j = 0; j < nw; ++j) { P.u_wit[j * n + u] = wit[j]; }
+            return u;
         }
+    }
+    return -1;
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < P.n;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        (void)init_reduce_cell<NC>(P, c);
     }
 }
 
@@ -714,7 +733,7 @@ struct SplitLane {
 // resume: FP64 continuation of the reduced-precision attempts marked A_RESUME, from their
 // stored logK and iteration count (mixed precision).
 template <int NC>
-__device__ bool split_begin(const Pipe<NC>& P, int j, int64_t u, bool resume, SplitLane<NC>& L) {
+__device__ __forceinline__ bool split_begin(const Pipe<NC>& P, int j, int64_t u, bool resume, SplitLane<NC>& L) {
     const int64_t n = P.n;
     if (resume) {
         if (j >= 2 * P.u_nwit[u] || (P.a_st[j * n + u] & 0xFF) != A_RESUME) { return false; }
@@ -757,7 +776,7 @@ __device__ bool split_begin(const Pipe<NC>& P, int j, int64_t u, bool resume, Sp
 }
 
 template <int NC>
-__device__ void split_finish(const Pipe<NC>& P, SplitLane<NC>& L, int status) {
+__device__ __forceinline__ void split_finish(const Pipe<NC>& P, SplitLane<NC>& L, int status) {
     const int64_t n = P.n;
     const int64_t idx = L.j * n + L.u;
     P.a_ev[idx] = L.evals;
@@ -822,6 +841,59 @@ __device__ __forceinline__ int split_check(SplitLane<NC>& L, double fug_tol) {
     return -1;
 }
 
+// One step of an active split lane: one evaluation of a trial point. Returns false when
+// the lane's attempt has ended (recorded by split_finish).
+template <int NC, bool kN>
+__device__ __forceinline__ bool split_step(const Pipe<NC>& P, const Mix<NC>* smix, SplitLane<NC>& L) {
+    double lk[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { lk[i] = L.mode == TM_START ? L.lk0[i] : L.pt.lk[i] + (L.alpha / L.scale) * L.pt.r[i]; }
+    SplitState<NC> nx;
+    bool rejected = false;
+    const int e = split_evaluate<NC, kN>(smix[mix_at(P, L.cell)], L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
+                                     L.mode == TM_START ? 0.5 : L.pt.fr.beta);
+    if (rejected) {
+        atomicOr(&P.flags[L.cell], CF_REJECTED);
+        split_finish<NC>(P, L, A_PROPERTY_FAILURE);
+        return false;
+    }
+    int done = -1;
+    if (L.mode == TM_START) {
+        if (e != -1) { done = e; }
+        else { L.pt = nx; done = split_check<NC>(L, P.fug_tol); }
+    } else if (e == -1) {
+        const double pred = -L.alpha * L.slope;
+        const double gd = L.pt.guard + nx.guard;
+        bool ok;
+        // split_accept_step (pt_split.hpp:232-244)
+        if (pred > gd) { ok = nx.gibbs <= L.pt.gibbs - 1e-4 * pred; }
+        else {
+            ok = nx.gibbs <= L.pt.gibbs + gd &&
+                 (nx.norm <= P.fug_tol || L.pt.norm - nx.norm > 1e-4 * (L.alpha / L.scale) * L.pt.norm);
+        }
+        if (ok) {
+            L.pt = nx;
+            ++L.it;
+            done = split_check<NC>(L, P.fug_tol);
+        } else {
+            ++L.bt;
+            L.alpha *= 0.5;
+            if (L.bt >= 24) { done = A_LINE_SEARCH_FAILED; }
+        }
+    } else if (e == A_EVALUATION_LIMIT) {
+        done = A_EVALUATION_LIMIT;
+    } else {
+        ++L.bt;
+        L.alpha *= 0.5;
+        if (L.bt >= 24) { done = A_LINE_SEARCH_FAILED; }
+    }
+    if (done >= 0) {
+        split_finish<NC>(P, L, done);
+        return false;
+    }
+    return true;
+}
+
 template <int NC, bool kN>
 __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc, int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
@@ -846,70 +918,22 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const
             continue;
         }
         if (!active) { continue; }
-        double lk[NC];
-#pragma unroll
-        for (int i = 0; i < NC; ++i) { lk[i] = L.mode == TM_START ? L.lk0[i] : L.pt.lk[i] + (L.alpha / L.scale) * L.pt.r[i]; }
-        SplitState<NC> nx;
-        bool rejected = false;
-        const int e = split_evaluate<NC, kN>(smix[mix_at(P, L.cell)], L.p, L.t, L.z, lk, L.evals, 20000u, nx, L.cnt, rejected,
-                                         L.mode == TM_START ? 0.5 : L.pt.fr.beta);
-        if (rejected) {
-            atomicOr(&P.flags[L.cell], CF_REJECTED);
-            split_finish<NC>(P, L, A_PROPERTY_FAILURE);
-            active = false;
-            continue;
-        }
-        int done = -1;
-        if (L.mode == TM_START) {
-            if (e != -1) { done = e; }
-            else { L.pt = nx; done = split_check<NC>(L, P.fug_tol); }
-        } else if (e == -1) {
-            const double pred = -L.alpha * L.slope;
-            const double gd = L.pt.guard + nx.guard;
-            bool ok;
-            // split_accept_step (pt_split.hpp:232-244)
-            if (pred > gd) { ok = nx.gibbs <= L.pt.gibbs - 1e-4 * pred; }
-            else {
-                ok = nx.gibbs <= L.pt.gibbs + gd &&
-                     (nx.norm <= P.fug_tol || L.pt.norm - nx.norm > 1e-4 * (L.alpha / L.scale) * L.pt.norm);
-            }
-            if (ok) {
-                L.pt = nx;
-                ++L.it;
-                done = split_check<NC>(L, P.fug_tol);
-            } else {
-                ++L.bt;
-                L.alpha *= 0.5;
-                if (L.bt >= 24) { done = A_LINE_SEARCH_FAILED; }
-            }
-        } else if (e == A_EVALUATION_LIMIT) {
-            done = A_EVALUATION_LIMIT;
-        } else {
-            ++L.bt;
-            L.alpha *= 0.5;
-            if (L.bt >= 24) { done = A_LINE_SEARCH_FAILED; }
-        }
-        if (done >= 0) {
-            split_finish<NC>(P, L, done);
-            active = false;
-        }
+        active = split_step<NC, kN>(P, smix, L);
     }
 }
 
 // Selection of the lowest-Gibbs converged attempt in attempt order, the feed Gibbs
 // comparison and the review tolerance (pt_split.hpp:459-507).
 template <int NC>
-__global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
+__device__ __forceinline__ void split_select_cell(const Pipe<NC>& P, int64_t u) {
     const int64_t n = P.n;
-    const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
-    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < nu;
-         u += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    {
         const int64_t c = P.u_cell[u];
         P.u_sel[u] = -1;
         P.u_v[u] = V_INDETERMINATE;
         P.u_ftol[u] = __builtin_nan("");
         P.u_dgbad[u] = 0;
-        if (P.flags[c] & CF_REJECTED) { continue; }
+        if (P.flags[c] & CF_REJECTED) { return; }
         const int na = 2 * P.u_nwit[u];
         uint64_t cum = 0;
         int best = -1;
@@ -927,11 +951,11 @@ __global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
             }
             cum += ev;
         }
-        if (fallback) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
+        if (fallback) { atomicOr(&P.flags[c], CF_FALLBACK); return; }
         P.u_sel[u] = best;
         // bit 0: the pair's Gibbs energy exceeds the feed's; bit 1: a disappearance attempt
         P.u_dgbad[u] = disappearance ? 2 : 0;
-        if (best < 0) { continue; }
+        if (best < 0) { return; }
         double feed_g = 0, corr = 0, mag = 1;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
@@ -945,7 +969,7 @@ __global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
         const double dg = best_g - feed_g;
         const double guard = P.a_s[(best * 6 + 5) * n + u] + 256 * kEps * mag;
         const double ftol = 1e-10 + 0.5 * P.a_s[(best * 6 + 4) * n + u];
-        if (!finite(dg) || !finite(guard) || !finite(ftol)) { continue; }
+        if (!finite(dg) || !finite(guard) || !finite(ftol)) { return; }
         P.u_v[u] = -1;
         P.u_ftol[u] = ftol;
         P.u_dgbad[u] |= dg > guard ? 1 : 0;
@@ -953,14 +977,21 @@ __global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
-    const int64_t n = P.n;
+__global__ __launch_bounds__(64) void k_split_select(Pipe<NC> P) {
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
     for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < nu;
          u += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        if (P.u_v[u] != -1) { continue; }
+        split_select_cell<NC>(P, u);
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ void review_reduce_cell(const Pipe<NC>& P, int64_t u) {
+    const int64_t n = P.n;
+    {
+        if (P.u_v[u] != -1) { return; }
         const int64_t c = P.u_cell[u];
-        if (P.flags[c] & (CF_REJECTED | CF_FALLBACK)) { continue; }
+        if (P.flags[c] & (CF_REJECTED | CF_FALLBACK)) { return; }
         const int ntr = review_trials<NC>(P.refinfo[c]);
         uint64_t evals = 0;
         bool neg = false, all_stat = true, ambiguous = false;
@@ -971,16 +1002,25 @@ __global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
             all_stat = all_stat && st == T_STATIONARY;
             ambiguous = ambiguous || st == T_AMBIG;
         }
-        if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); continue; }
+        if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); return; }
         const int fin = neg ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
         if (P.mixed == 1 && fin == S_UNSTABLE) {
             // mixed precision: a three-phase candidate's review is searched again in FP64 so
             // that the three-phase tail starts from the reference's witnesses
             for (int k = 0; k < ntr; ++k) { P.fst[k * n + u] = T_REDO; }
-            continue;
+            return;
         }
         P.u_v[u] = fin == S_UNSTABLE ? V_UNSTABLE
                  : ((P.u_dgbad[u] & 1) ? V_INDETERMINATE : (fin == S_NO_INSTABILITY ? V_TWO : V_INDETERMINATE));
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void k_review_reduce(Pipe<NC> P) {
+    const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
+    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < nu;
+         u += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        review_reduce_cell<NC>(P, u);
     }
 }
 
