@@ -588,6 +588,7 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split32(Pipe<NC> P, con
                 int k;
                 int64_t u;
                 task_index<MPMC_SPLIT_ORDER>(task, nu, kAttempts, k, u);
+                if (MPMC_SPLIT_LPT) { u = P.u_order[u]; }
                 active = split32_begin<NC>(P, smix, k, u, L);
             }
         }
