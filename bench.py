@@ -107,8 +107,8 @@ def cpu_baseline(mix, p, t, z, sample: int, sample_1core: int = 8192, repeats: i
 # HBM bytes of one flash of the 1M-cell nc=6 grid: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # summed over the launch sequence (profiles/flash_nc6_1M_hbm_traffic.md), FP64 and
 # decision-exact mixed precision.
-FLASH_FP64_TRAFFIC_BYTES = 2164755 * 2.0 * 1024 + 1735990 * 1.0 * 1024
-FLASH_EXACT_TRAFFIC_BYTES = 3460919 * 2.0 * 1024 + 1903868 * 1.0 * 1024
+FLASH_FP64_TRAFFIC_BYTES = 2329324 * 2.0 * 1024 + 1766400 * 1.0 * 1024
+FLASH_EXACT_TRAFFIC_BYTES = 3380342 * 2.0 * 1024 + 1914209 * 1.0 * 1024
 ASSEMBLY_TRAFFIC_BYTES = (36615 + 1072624) * 2.0 * 1024 + (728516 + 2578166) * 1.0 * 1024
 
 
