@@ -51,6 +51,23 @@ namespace mpmc::dev {
 #else
 #define MPMC_SEARCH_ATTR
 #endif
+// The same hint for the FP64 stability (k_tpd) and split (k_split) kernels alone.
+#ifndef MPMC_TPD_WAVES_PER_EU
+#define MPMC_TPD_WAVES_PER_EU MPMC_SEARCH_WAVES_PER_EU
+#endif
+#ifndef MPMC_SPLIT_WAVES_PER_EU
+#define MPMC_SPLIT_WAVES_PER_EU MPMC_SEARCH_WAVES_PER_EU
+#endif
+#if MPMC_TPD_WAVES_PER_EU > 0
+#define MPMC_TPD_ATTR __attribute__((amdgpu_waves_per_eu(MPMC_TPD_WAVES_PER_EU)))
+#else
+#define MPMC_TPD_ATTR
+#endif
+#if MPMC_SPLIT_WAVES_PER_EU > 0
+#define MPMC_SPLIT_ATTR __attribute__((amdgpu_waves_per_eu(MPMC_SPLIT_WAVES_PER_EU)))
+#else
+#define MPMC_SPLIT_ATTR
+#endif
 
 // Order in which a persistent queue hands out its (k, slot) tasks: 0 slot-fastest (the k-th
 // trial or attempt of every cell, then the next k), 1 k-fastest (all trials or attempts of
@@ -618,7 +635,7 @@ __device__ __forceinline__ bool tpd_step(const Pipe<NC>& P, const Mix<NC>* smix,
 }
 
 template <int NC, bool kN>
-__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage,
+__global__ __launch_bounds__(64) MPMC_TPD_ATTR void k_tpd(Pipe<NC> P, const double* __restrict__ mixc, int stage,
                                                               int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int tk = stage == 0 ? NC + 2 + P.nhint : NC + 4 + P.nfhint;
@@ -655,55 +672,81 @@ __global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_tpd(Pipe<NC> P, const d
 }
 
 // ------------------------------------------------------------------ stability reduce
-// The initial stability result of cell c from its trials (status, budget, fallback) and,
-// for an unstable cell that continues on the staged path, its compact index u and
-// witnesses. Returns u, or -1.
+// The initial stability result of cell c from its trials (status, budget, fallback).
+// Returns the cell's witness count (at most kWitnesses, the trials in wit) when it is
+// unstable and continues on the staged path, else -1.
 template <int NC>
-__device__ __forceinline__ int64_t init_reduce_cell(const Pipe<NC>& P, int64_t c) {
+__device__ __forceinline__ int init_reduce_cell(const Pipe<NC>& P, int64_t c, int (&wit)[kWitnesses]) {
     const int64_t n = P.n;
-    {
-        if (P.flags[c] & CF_REJECTED) { return -1; }
-        const int32_t info = P.refinfo[c];
-        if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; return -1; }
-        const int active = active_count<NC>(info);
-        const int ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info);
-        uint64_t evals = 1;
-        bool neg = false, all_stat = true, ambiguous = false;
-        int nwit = 0;
-        int wit[kWitnesses];
-        for (int k = 0; k < ntr; ++k) {
-            const int st = P.tst[k * n + c] & 0xFF;
-            ambiguous = ambiguous || st == T_AMBIG;
-            evals += P.tev[k * n + c];
-            if (st == T_NEGATIVE) {
-                neg = true;
-                if (nwit < kWitnesses) { wit[nwit] = k; }
-                ++nwit;
-            }
-            all_stat = all_stat && st == T_STATIONARY;
+    if (P.flags[c] & CF_REJECTED) { return -1; }
+    const int32_t info = P.refinfo[c];
+    if (!(info & 0x200)) { P.cstat[c] = S_INDETERMINATE; return -1; }
+    const int active = active_count<NC>(info);
+    const int ntr = (active > 1 ? active + 2 : 1) + hint_count<NC>(info);
+    uint64_t evals = 1;
+    bool neg = false, all_stat = true, ambiguous = false;
+    int nwit = 0;
+    for (int k = 0; k < ntr; ++k) {
+        const int st = P.tst[k * n + c] & 0xFF;
+        ambiguous = ambiguous || st == T_AMBIG;
+        evals += P.tev[k * n + c];
+        if (st == T_NEGATIVE) {
+            neg = true;
+            if (nwit < kWitnesses) { wit[nwit] = k; }
+            ++nwit;
         }
-        if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); }
-        const int status = neg ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
-        P.cstat[c] = status;
-        if (status == S_UNSTABLE && !(P.flags[c] & CF_FALLBACK)) {
-            const int64_t u = static_cast<int64_t>(atomicAdd(&P.g[G_NU], 1ull));
-            P.cell_u[c] = static_cast<int32_t>(u);
-            P.u_cell[u] = static_cast<int32_t>(c);
-            const int nw = nwit < kWitnesses ? nwit : kWitnesses;
-            P.u_nwit[u] = nw;
-            for (int j = 0; j < nw; ++j) { P.u_wit[j * n + u] = wit[j]; }
-            if (MPMC_SPLIT_LPT) { atomicAdd(&P.g[G_BUCKET + nw], 1ull); }
-            return u;
-        }
+        all_stat = all_stat && st == T_STATIONARY;
     }
+    if (evals > 100000 || ambiguous) { atomicOr(&P.flags[c], CF_FALLBACK); }
+    const int status = neg ? S_UNSTABLE : (all_stat ? S_NO_INSTABILITY : S_INDETERMINATE);
+    P.cstat[c] = status;
+    if (status == S_UNSTABLE && !(P.flags[c] & CF_FALLBACK)) { return nwit < kWitnesses ? nwit : kWitnesses; }
     return -1;
 }
 
+// Wave-aggregated count by key: for every lane with key >= 0, atomicAdd(&counts[key], 1)
+// with one atomic per distinct key in the wave. Returns the lane's previous count (the
+// count before the wave plus the lane's rank among the wave's lanes of that key), or 0.
+__device__ __forceinline__ unsigned long long count_keyed(unsigned long long* counts, int key) {
+    const int lane = static_cast<int>(__lane_id());
+    unsigned long long pending = __ballot(key >= 0);
+    unsigned long long mine = 0;
+    while (pending != 0ull) {
+        const int leader = __ffsll(pending) - 1;
+        const int k = __shfl(key, leader);
+        const unsigned long long same = __ballot(key == k) & pending;
+        unsigned long long base = 0;
+        if (lane == leader) { base = atomicAdd(&counts[k], static_cast<unsigned long long>(__popcll(same))); }
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) { mine = base + static_cast<unsigned long long>(__popcll(same & ((1ull << lane) - 1ull))); }
+        pending &= ~same;
+    }
+    return mine;
+}
+
+// The compact record of unstable cell c under index u: cell, witness count, witnesses.
+template <int NC>
+__device__ __forceinline__ void record_u(const Pipe<NC>& P, int64_t c, int64_t u, int nw, const int (&wit)[kWitnesses]) {
+    const int64_t n = P.n;
+    P.cell_u[c] = static_cast<int32_t>(u);
+    P.u_cell[u] = static_cast<int32_t>(c);
+    P.u_nwit[u] = nw;
+    for (int j = 0; j < nw; ++j) { P.u_wit[j * n + u] = wit[j]; }
+}
+
+// Per cell: the stability result, and for the unstable cells that continue on the staged
+// path a compact index u (one atomic per wave) with their witnesses; with MPMC_SPLIT_LPT the
+// cells per witness count.
 template <int NC>
 __global__ __launch_bounds__(64) void k_init_reduce(Pipe<NC> P) {
-    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < P.n;
+    const int64_t n = P.n;
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        (void)init_reduce_cell<NC>(P, c);
+        int wit[kWitnesses];
+        const int nw = init_reduce_cell<NC>(P, c, wit);
+        const int64_t u = fetch_task(&P.g[G_NU], nw >= 0);
+        if (nw >= 0) { record_u<NC>(P, c, u, nw, wit); }
+        if (MPMC_SPLIT_LPT) { (void)count_keyed(&P.g[G_BUCKET], nw); }
     }
 }
 
@@ -717,8 +760,7 @@ __global__ __launch_bounds__(64) void k_order_u(Pipe<NC> P) {
         const int nw = P.u_nwit[u];
         unsigned long long base = 0;
         for (int b = nw + 1; b <= kWitnesses; ++b) { base += P.g[G_BUCKET + b]; }
-        const unsigned long long pos = base + atomicAdd(&P.g[G_CURSOR + nw], 1ull);
-        P.u_order[pos] = static_cast<int32_t>(u);
+        P.u_order[base + count_keyed(&P.g[G_CURSOR], nw)] = static_cast<int32_t>(u);
     }
 }
 
@@ -919,7 +961,7 @@ __device__ __forceinline__ bool split_step(const Pipe<NC>& P, const Mix<NC>* smi
 }
 
 template <int NC, bool kN>
-__global__ __launch_bounds__(64) MPMC_SEARCH_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc, int resume) {
+__global__ __launch_bounds__(64) MPMC_SPLIT_ATTR void k_split(Pipe<NC> P, const double* __restrict__ mixc, int resume) {
     Mix<NC>* const smix = stage_mixes<NC>(mixc, P.nmix);
     const int64_t nu = static_cast<int64_t>(P.g[G_NU]);
     const int64_t total = kAttempts * nu;
