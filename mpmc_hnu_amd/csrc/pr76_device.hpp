@@ -172,17 +172,20 @@ MPMC_HOT RootSet pr76_roots(double a, double b) {
             // The stopping tests are combined without short-circuit (each || is a branch
             // on the wave's execution mask) and the bracket is updated by selects: the
             // same arithmetic and decisions in about two thirds of the time.
+            // y is the midpoint of two finite ends, so fmax(y, DBL_MIN) is the reference's
+            // max(y, DBL_MIN) (one instruction); fy is nonzero past the stop test, so its
+            // sign bit is fy < 0.
             const bool lneg = __builtin_signbit(fl);
             for (int it = 0; it < 2048; ++it) {
                 y = (left + right) / 2;
                 const double fy = poly(y, c);
                 const bool stop = (fy == 0) | (y == left) | (y == right) |
-                                  (right - left <= 4.0 * kEps * smax(y, DBL_MIN));
+                                  (right - left <= 4.0 * kEps * __builtin_fmax(y, DBL_MIN));
                 if (stop) {
                     conv = true;
                     break;
                 }
-                const bool to_left = static_cast<bool>(__builtin_signbit(fy)) == lneg;
+                const bool to_left = (fy < 0) == lneg;
                 left = to_left ? y : left;
                 right = to_left ? right : y;
             }
